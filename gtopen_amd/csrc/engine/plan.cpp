@@ -1,0 +1,245 @@
+// Level plan construction (the role of crates/solver/src/gpu/plan.rs:100-387,
+// redesigned: compact reach-owner slots, per-board showdown rank tables and
+// per-card prefix indexes instead of per-hand binary searches and blocker
+// loops).
+#include "plan.h"
+
+#include <algorithm>
+#include <map>
+#include <stdexcept>
+
+namespace gto {
+
+namespace {
+constexpr uint32_t kSent = 0xFFFFFFFFu;
+
+struct Item {
+    uint32_t node;
+    uint32_t level;
+    uint8_t dealt[2];
+    uint8_t ndealt;
+    uint32_t src[2];
+};
+
+uint32_t river_key_of(const gto_spot_view& v, const Item& it) {
+    if (v.root_street == 2) return 0;
+    if (v.root_street == 1) return it.dealt[0];
+    uint32_t a = std::min(it.dealt[0], it.dealt[1]), b = std::max(it.dealt[0], it.dealt[1]);
+    return a * 52 + b;
+}
+}  // namespace
+
+Plan build_plan(const gto_spot_view& v) {
+    Plan pl;
+    const uint32_t N = v.num_nodes;
+    pl.num_nodes = N;
+    pl.nh[0] = v.num_hands[0];
+    pl.nh[1] = v.num_hands[1];
+    pl.nh_max = std::max(pl.nh[0], pl.nh[1]);
+    pl.owner[0].assign(N, 0);
+    pl.owner[1].assign(N, 0);
+    pl.cfv_slot.assign(N, kSent);
+    pl.ch_off.assign(N, 0);
+    pl.ch_cnt.assign(N, 0);
+    pl.show_slot.assign(N, kSent);
+    std::vector<uint32_t> level(N, kSent);
+    std::vector<uint32_t> key(N, kSent);
+
+    // depth-first walk over the reachable tree (chance nodes skip cards
+    // already dealt, as cfr.rs:556-565 does)
+    std::vector<Item> stack;
+    pl.num_slots[0] = pl.num_slots[1] = 1;  // slot 0 = root reach
+    Item root{0, 0, {0, 0}, 0, {0, 0}};
+    stack.push_back(root);
+    uint32_t max_level = 0;
+    std::vector<std::vector<uint32_t>> edges_p, edges_c, edges_card;
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        const uint32_t n = it.node;
+        level[n] = it.level;
+        pl.owner[0][n] = it.src[0];
+        pl.owner[1][n] = it.src[1];
+        max_level = std::max(max_level, it.level);
+        const uint8_t kind = v.node_kind[n];
+        const uint32_t cs = v.node_children_start[n];
+        if (kind == GTO_KIND_ACTION) {
+            const int actor = v.node_player[n];
+            const uint32_t na = v.node_num_children[n];
+            pl.reached_entries += (uint64_t)na * v.num_hands[actor];
+            pl.reached_entries_by_player[actor] += (uint64_t)na * v.num_hands[actor];
+            for (uint32_t a = na; a-- > 0;) {
+                Item c = it;
+                c.node = v.children[cs + a];
+                c.level = it.level + 1;
+                c.src[actor] = pl.num_slots[actor]++;
+                stack.push_back(c);
+            }
+        } else if (kind == GTO_KIND_CHANCE) {
+            pl.ch_off[n] = (uint32_t)pl.ch_child.size();
+            std::vector<Item> kids;
+            for (uint32_t c = 0; c < 52; ++c) {
+                uint32_t child = v.children[cs + c];
+                if (child == kSent) continue;
+                bool dealt = false;
+                for (int d = 0; d < it.ndealt; ++d) dealt |= it.dealt[d] == c;
+                if (dealt) continue;
+                pl.ch_child.push_back(child);
+                pl.ch_card.push_back((uint8_t)c);
+                Item k = it;
+                k.node = child;
+                k.level = it.level + 1;
+                k.dealt[k.ndealt++] = (uint8_t)c;
+                k.src[0] = pl.num_slots[0]++;
+                k.src[1] = pl.num_slots[1]++;
+                kids.push_back(k);
+                if (edges_p.size() <= it.level) {
+                    edges_p.resize(it.level + 1);
+                    edges_c.resize(it.level + 1);
+                    edges_card.resize(it.level + 1);
+                }
+                edges_p[it.level].push_back(n);
+                edges_c[it.level].push_back(child);
+                edges_card[it.level].push_back(c);
+            }
+            pl.ch_cnt[n] = (uint32_t)kids.size();
+            for (size_t k = kids.size(); k-- > 0;) stack.push_back(kids[k]);
+        } else if (kind == GTO_KIND_SHOWDOWN) {
+            key[n] = river_key_of(v, it);
+        }
+    }
+
+    // level lists (node-index order within a level)
+    const uint32_t L = max_level + 1;
+    pl.levels.resize(L);
+    uint32_t next_cfv = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        LevelLists& ll = pl.levels[l];
+        if (l < edges_p.size()) {
+            ll.edge_parent = edges_p[l];
+            ll.edge_child = edges_c[l];
+            ll.edge_card = edges_card[l];
+        }
+    }
+    for (uint32_t n = 0; n < N; ++n) {
+        if (level[n] == kSent) continue;
+        LevelLists& ll = pl.levels[level[n]];
+        switch (v.node_kind[n]) {
+            case GTO_KIND_ACTION: ll.act[v.node_player[n]].push_back(n); break;
+            case GTO_KIND_CHANCE: ll.chance.push_back(n); break;
+            case GTO_KIND_FOLD: ll.fold.push_back(n); break;
+            default: ll.show.push_back(n); break;
+        }
+    }
+    // cfv slots in level order so one launch touches a contiguous range
+    for (uint32_t l = 0; l < L; ++l) {
+        LevelLists& ll = pl.levels[l];
+        for (auto* lst : {&ll.act[0], &ll.act[1], &ll.chance, &ll.fold, &ll.show})
+            for (uint32_t n : *lst) pl.cfv_slot[n] = next_cfv++;
+    }
+    pl.num_reached = next_cfv;
+
+    // river boards: dense slots in first-seen order of showdown nodes; the
+    // showdown lists are then sorted by board so neighbouring workgroups
+    // share a board's tables in cache
+    std::map<uint32_t, uint32_t> slot_of;
+    for (uint32_t n = 0; n < N; ++n) {
+        if (key[n] == kSent) continue;
+        auto it = slot_of.find(key[n]);
+        if (it == slot_of.end()) {
+            uint32_t s = (uint32_t)pl.board_key.size();
+            slot_of[key[n]] = s;
+            pl.board_key.push_back(key[n]);
+            pl.show_slot[n] = s;
+        } else {
+            pl.show_slot[n] = it->second;
+        }
+    }
+    for (auto& ll : pl.levels)
+        std::stable_sort(ll.show.begin(), ll.show.end(),
+                         [&](uint32_t a, uint32_t b) { return pl.show_slot[a] < pl.show_slot[b]; });
+    pl.num_boards = (uint32_t)pl.board_key.size();
+
+    for (int q = 0; q < 2; ++q) {
+        pl.sd_off[q].resize(pl.num_boards);
+        pl.sd_cnt[q].resize(pl.num_boards);
+        pl.sd_coff[q].resize((size_t)pl.num_boards * 53);
+        pl.sd_cpos_base[q].resize(pl.num_boards);
+        for (uint32_t b = 0; b < pl.num_boards; ++b) {
+            uint32_t k = pl.board_key[b];
+            int64_t off = v.river_off[q][k];
+            if (off < 0) throw std::runtime_error("river table missing for a reachable board");
+            uint32_t cnt = v.river_cnt[q][k];
+            pl.sd_off[q][b] = (uint32_t)pl.sd_idx[q].size();
+            pl.sd_cnt[q][b] = cnt;
+            pl.max_board_cnt[q] = std::max(pl.max_board_cnt[q], cnt);
+            for (uint32_t t = 0; t < cnt; ++t) pl.sd_idx[q].push_back(v.river_idx[q][off + t]);
+            // per-card position lists of q's sorted hands on this board
+            std::vector<std::vector<uint16_t>> by_card(52);
+            for (uint32_t t = 0; t < cnt; ++t) {
+                uint16_t h = v.river_idx[q][off + t];
+                by_card[v.hand_c1[q][h]].push_back((uint16_t)t);
+                by_card[v.hand_c2[q][h]].push_back((uint16_t)t);
+            }
+            pl.sd_cpos_base[q][b] = (uint32_t)pl.sd_cpos[q].size();
+            uint32_t acc = 0;
+            for (int c = 0; c < 52; ++c) {
+                pl.sd_coff[q][(size_t)b * 53 + c] = acc;
+                for (uint16_t pos : by_card[c]) pl.sd_cpos[q].push_back(pos);
+                acc += (uint32_t)by_card[c].size();
+            }
+            pl.sd_coff[q][(size_t)b * 53 + 52] = acc;
+        }
+    }
+    // rank indexes of each sorted hand of p against the opponent's sorted list
+    for (int p = 0; p < 2; ++p) {
+        const int o = 1 - p;
+        pl.sd_lbub[p].resize(pl.sd_idx[p].size());
+        pl.sd_cnt4[p].resize(pl.sd_idx[p].size());
+        for (uint32_t b = 0; b < pl.num_boards; ++b) {
+            uint32_t k = pl.board_key[b];
+            const uint32_t* pstr = v.river_str[p] + v.river_off[p][k];
+            const uint32_t* ostr = v.river_str[o] + v.river_off[o][k];
+            uint32_t mp = pl.sd_cnt[p][b], mo = pl.sd_cnt[o][b];
+            const uint32_t* coff = &pl.sd_coff[o][(size_t)b * 53];
+            const uint16_t* cpos = pl.sd_cpos[o].data() + pl.sd_cpos_base[o][b];
+            for (uint32_t t = 0; t < mp; ++t) {
+                uint32_t m = pstr[t];
+                uint32_t lb = (uint32_t)(std::lower_bound(ostr, ostr + mo, m) - ostr);
+                uint32_t ub = (uint32_t)(std::upper_bound(ostr, ostr + mo, m) - ostr);
+                uint16_t h = pl.sd_idx[p][pl.sd_off[p][b] + t];
+                uint8_t cc[2] = {v.hand_c1[p][h], v.hand_c2[p][h]};
+                uint32_t counts[4];
+                for (int x = 0; x < 2; ++x) {
+                    const uint16_t* s0 = cpos + coff[cc[x]];
+                    const uint16_t* s1 = cpos + coff[cc[x] + 1];
+                    counts[x] = (uint32_t)(std::lower_bound(s0, s1, (uint16_t)lb) - s0);
+                    counts[2 + x] = (uint32_t)(std::lower_bound(s0, s1, (uint16_t)ub) - s0);
+                }
+                for (int x = 0; x < 4; ++x)
+                    if (counts[x] > 255) throw std::runtime_error("per-card list longer than 255");
+                pl.sd_lbub[p][pl.sd_off[p][b] + t] = lb | (ub << 16);
+                pl.sd_cnt4[p][pl.sd_off[p][b] + t] = counts[0] | (counts[1] << 8) | (counts[2] << 16) | (counts[3] << 24);
+            }
+        }
+    }
+    // per-card hand lists (fold terminals)
+    for (int q = 0; q < 2; ++q) {
+        std::vector<std::vector<uint16_t>> by_card(52);
+        for (uint32_t h = 0; h < v.num_hands[q]; ++h) {
+            by_card[v.hand_c1[q][h]].push_back((uint16_t)h);
+            by_card[v.hand_c2[q][h]].push_back((uint16_t)h);
+        }
+        pl.hc_off[q].resize(53);
+        uint32_t acc = 0;
+        for (int c = 0; c < 52; ++c) {
+            pl.hc_off[q][c] = acc;
+            for (uint16_t h : by_card[c]) pl.hc_idx[q].push_back(h);
+            acc += (uint32_t)by_card[c].size();
+        }
+        pl.hc_off[q][52] = acc;
+    }
+    return pl;
+}
+
+}  // namespace gto

@@ -1,0 +1,71 @@
+// Level plan: the spot's reachable tree flattened into per-depth work lists
+// for level-synchronous sweeps, plus the static tables the showdown and
+// fold kernels index. Built once per engine on the host.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../../../include/gtopen_amd.h"
+
+namespace gto {
+
+struct LevelLists {
+    // action nodes split by actor, fold, showdown and chance nodes at this depth
+    std::vector<uint32_t> act[2], fold, show, chance;
+    // chance edges whose parent sits at this depth: (parent, child, card)
+    std::vector<uint32_t> edge_parent, edge_child, edge_card;
+};
+
+struct Plan {
+    uint32_t num_nodes = 0;
+    uint32_t nh[2] = {0, 0};
+    uint32_t nh_max = 0;
+    uint32_t num_reached = 0;
+    std::vector<LevelLists> levels;
+
+    // Reach vectors are stored once per "owner": the root, every child of an
+    // action node for its actor, every chance child for both players. A
+    // node reads player q's reach (and its alive flag) from owner[q][node].
+    uint32_t num_slots[2] = {0, 0};
+    std::vector<uint32_t> owner[2];
+    // compact per-node index into the counterfactual-value buffer
+    std::vector<uint32_t> cfv_slot;
+
+    // chance nodes: valid (card, child) lists in card order
+    std::vector<uint32_t> ch_off, ch_cnt;  // per node
+    std::vector<uint32_t> ch_child;
+    std::vector<uint8_t> ch_card;
+
+    // showdown nodes -> dense river-board slot
+    std::vector<uint32_t> show_slot;  // per node (kSentinel elsewhere)
+    uint32_t num_boards = 0;
+    std::vector<uint32_t> board_key;   // per slot
+    // per player q and board slot: q's strength-sorted hands on that board
+    std::vector<uint32_t> sd_off[2], sd_cnt[2];
+    std::vector<uint16_t> sd_idx[2];   // hand index, sorted ascending by strength
+    // per sorted hand of q (as the traverser): lb | ub << 16, where lb/ub =
+    // number of opponent hands strictly weaker / weaker-or-equal
+    std::vector<uint32_t> sd_lbub[2];
+    // per sorted hand of q: counts of entries before lb (c1, c2) and before
+    // ub (c1, c2) in the opponent's per-card lists, one byte each
+    std::vector<uint32_t> sd_cnt4[2];
+    // per player q and board slot: 53 offsets (relative to sd_cpos_base) of
+    // the per-card lists of q's sorted positions holding card c
+    std::vector<uint32_t> sd_coff[2];
+    std::vector<uint32_t> sd_cpos_base[2];
+    std::vector<uint16_t> sd_cpos[2];
+    uint32_t max_board_cnt[2] = {0, 0};
+
+    // per player: hand indices holding card c, ascending (fold card sums)
+    std::vector<uint32_t> hc_off[2];  // 53 entries
+    std::vector<uint16_t> hc_idx[2];
+
+    // algorithmic arena entries over reached action nodes (sum na*nh_actor)
+    uint64_t reached_entries = 0;
+    uint64_t reached_entries_by_player[2] = {0, 0};
+};
+
+Plan build_plan(const gto_spot_view& v);
+
+}  // namespace gto

@@ -1,0 +1,230 @@
+/*
+ * gtopen_amd — C ABI of the MI355X DCFR engine for heads-up postflop spots.
+ *
+ * Two halves, one shared library (libgtopen_amd.so):
+ *
+ *   gto_spot_*  host-side spot construction (board, ranges, betting tree,
+ *               river strength tables). Replaces `Spot::new*`
+ *               (crates/solver/src/game.rs:103-133) and exposes the built
+ *               spot as plain arrays through `gto_spot_view`, the same data a
+ *               foreign caller (the reference's Rust `Spot`) would flatten.
+ *
+ *   gto_hip_*   the GPU engine. Replaces `solver::gpu::GpuSolver`
+ *               (crates/solver/src/gpu/mod.rs:60-997): create from a spot,
+ *               iterate, exploitability, arena download/upload, locks,
+ *               synchronize, VRAM estimate/info, profile, game value.
+ *
+ * Conventions: every function returning `int` returns 0 on success and a
+ * negative code on failure; `gto_last_error()` then holds a thread-local
+ * message (the reference returns `Err(String)`, gpu/mod.rs:56-58). All
+ * arrays are caller-owned host memory; nothing in this header uses C++ or
+ * torch types. One engine handle must be used by one thread at a time
+ * (the reference's `&mut self` rule).
+ */
+#ifndef GTOPEN_AMD_H
+#define GTOPEN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GTO_OK 0
+#define GTO_ERR_INVALID (-1)
+#define GTO_ERR_DEVICE (-2)
+#define GTO_ERR_MEMORY (-3)
+#define GTO_ERR_IO (-4)
+
+/* Node kinds (crates/solver/src/tree.rs:172-175). */
+#define GTO_KIND_ACTION 0
+#define GTO_KIND_CHANCE 1
+#define GTO_KIND_FOLD 2
+#define GTO_KIND_SHOWDOWN 3
+#define GTO_SENTINEL 0xFFFFFFFFu
+
+/* Arena storage (crates/solver/src/store.rs:69-76). */
+#define GTO_STORAGE_F32 0
+#define GTO_STORAGE_COMPRESSED 1 /* i16 regrets, u16 strategy sums, f32 scale per node */
+
+/* Algorithms (crates/solver/src/cfr.rs:13-35); the engine runs DCFR and CFR+. */
+#define GTO_ALGO_DCFR 0
+#define GTO_ALGO_CFRPLUS 1
+
+/* Tree build strictness (crates/solver/src/tree.rs:213-243). */
+#define GTO_STRICT 0
+#define GTO_LENIENT_LOAD 1
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char* gto_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Spot: host-side construction                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gto_spot gto_spot;
+
+/*
+ * Flattened spot. Field meanings follow the reference structs:
+ *   nodes   -> tree.rs:178-198 (`Node`), children -> tree.rs:201-211
+ *   hands   -> game.rs:13-19 (`HandInfo`, combo_index order, game.rs:174-202)
+ *   same_combo -> game.rs:93-95; river tables -> game.rs:29-61
+ * River table keys follow `RiverTable::key` (game.rs:43-56): root street 2 has
+ * one key (0); root street 1 is keyed by the river card (52 keys); root street
+ * 0 by lo*52+hi of the two dealt cards (2704 keys). river_off[p][key] is -1
+ * for keys without an entry; otherwise the entry occupies
+ * [river_off, river_off + river_cnt) of river_str/river_idx, sorted ascending
+ * by (strength, hand index).
+ */
+typedef struct gto_spot_view {
+    uint32_t num_nodes;
+    const uint8_t* node_kind;
+    const uint8_t* node_player;   /* action: actor; fold: the player who folded */
+    const uint8_t* node_street;   /* chance: street dealt into */
+    const uint8_t* node_num_children;
+    const uint32_t* node_children_start;
+    const uint64_t* node_data_offset; /* action nodes: element offset into the actor's arenas */
+    const double* node_t_win;
+    const double* node_t_lose;
+    const double* node_t_tie;
+    const double* node_put;       /* 2 per node: chips put in by OOP, IP */
+    uint32_t num_children_total;
+    const uint32_t* children;     /* chance nodes own 52 slots (card -> child or SENTINEL) */
+    uint64_t data_size[2];
+    uint32_t root_street;
+    double starting_pot;
+    double effective_stack;
+    double rake_pct;
+    double rake_cap;
+    uint32_t board_len;
+    uint8_t board[5];
+    uint64_t board_mask;
+    uint32_t num_hands[2];
+    const uint8_t* hand_c1[2];    /* higher card of the combo */
+    const uint8_t* hand_c2[2];
+    const float* weights[2];
+    const uint32_t* same_combo[2];
+    uint32_t num_river_keys;
+    const int64_t* river_off[2];
+    const uint32_t* river_cnt[2];
+    const uint32_t* river_str[2];
+    const uint16_t* river_idx[2];
+} gto_spot_view;
+
+/* Build a spot from the reference's SpotConfig JSON (game.rs:21-27, serde
+ * field names and enum tags: {"PotPct":33}, {"PrevMult":2.5}, "AllIn").
+ * `max_nodes` 0 = no node budget (game.rs:108-110). Replaces Spot::new,
+ * Spot::new_with_limit, Spot::new_lenient[_with_limit] (game.rs:103-133). */
+int gto_spot_build(const char* config_json, int strictness, uint64_t max_nodes, gto_spot** out);
+void gto_spot_free(gto_spot* spot);
+/* Stable view into the spot's arrays; valid until gto_spot_free. */
+const gto_spot_view* gto_spot_view_of(const gto_spot* spot);
+/* Number of action nodes (game.rs:375-381). */
+uint64_t gto_spot_num_action_nodes(const gto_spot* spot);
+/* Arena bytes under a storage mode (game.rs:341-358). */
+uint64_t gto_spot_arena_bytes(const gto_spot* spot, int storage);
+/* Suit-symmetry group size (game.rs:262-317); 1 = only the identity. */
+uint32_t gto_spot_num_suit_perms(const gto_spot* spot);
+
+/* Card helpers (crates/solver/src/cards.rs:100-114, evaluator.rs:68-175). */
+uint32_t gto_combo_index(uint8_t c1, uint8_t c2);
+uint32_t gto_evaluate7(const uint8_t* cards7);
+/* Parse a range text into 1326 weights (range.rs:28-58). */
+int gto_range_parse(const char* text, float* weights1326);
+/* Canonical flop classes (cards.rs:236-311): writes up to `cap` rows of
+ * 3 cards + weight; returns the row count through *n. n_subset 0 = all. */
+int gto_canonical_flops(uint32_t n_subset, uint8_t* cards3, uint32_t* weights, uint32_t cap,
+                        uint32_t* n);
+
+/* ------------------------------------------------------------------------ */
+/* Engine: the GPU DCFR hot path                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gto_hip gto_hip;
+
+typedef struct gto_hip_options {
+    int storage;        /* GTO_STORAGE_F32 or GTO_STORAGE_COMPRESSED (device arenas) */
+    int algo;           /* GTO_ALGO_DCFR or GTO_ALGO_CFRPLUS */
+    int use_graphs;     /* capture each sweep into a hipGraph after the first iteration */
+    int device;         /* HIP device ordinal */
+} gto_hip_options;
+
+/* Create an engine over `spot` (copied to the device; the view may be freed
+ * afterwards). Arenas start at zero and iteration at 0, as a fresh
+ * `Solver::with_storage` (cfr.rs:122-146). Replaces GpuSolver::new
+ * (gpu/mod.rs:146-294). */
+int gto_hip_create(const gto_spot_view* spot, const gto_hip_options* opts, gto_hip** out);
+void gto_hip_destroy(gto_hip* h);
+
+/* Run `n` DCFR iterations (alternating updates, traverser 0 then 1), as
+ * GpuSolver::iterate (gpu/mod.rs:304-346) / Solver::iterate (cfr.rs:170-190). */
+int gto_hip_iterate(gto_hip* h, uint32_t n);
+int gto_hip_get_iteration(const gto_hip* h, uint32_t* iteration);
+int gto_hip_set_iteration(gto_hip* h, uint32_t iteration);
+
+/* Exploitability of the average strategy in chips (best-response sweeps on
+ * the device, f64 root dot), as GpuSolver::exploitability (gpu/mod.rs:367-396)
+ * and Solver::exploitability (best_response.rs:255-287). */
+int gto_hip_exploitability(gto_hip* h, double* chips);
+
+/* Root counterfactual values of player p (nh_p floats) under: mode 0 = p
+ * best-responds, mode 1 = both play the average strategy, mode 2 = both play
+ * the current (regret-matched) strategy. Mirrors traverse_br/traverse_avg
+ * (best_response.rs:12-223). */
+int gto_hip_root_cfv(gto_hip* h, int p, int mode, float* out);
+
+/* Arena transfer as f32 (compressed device arenas are decoded/encoded with
+ * the per-node scales). Layout = tree data_offset order (tree.rs:546-548);
+ * lengths data_size[0]/[1]. Replaces sync_to_cpu / sync_strategy and the
+ * upload in GpuSolver::new (gpu/mod.rs:188-199, 845-886, 976-997). Any
+ * pointer may be NULL to skip that arena. */
+int gto_hip_download_arenas(gto_hip* h, float* regrets0, float* regrets1, float* strat0,
+                            float* strat1);
+int gto_hip_upload_arenas(gto_hip* h, const float* regrets0, const float* regrets1,
+                          const float* strat0, const float* strat1);
+/* Raw compressed arenas (i16 regrets, u16 strategy sums, per-node f32
+ * scales indexed by node) when storage is COMPRESSED; store.rs:79-92. */
+int gto_hip_download_compressed(gto_hip* h, int p, int16_t* regrets_q, uint16_t* strat_q,
+                                float* regret_scale, float* strat_scale);
+
+/* Node locks: lock_off[num_nodes] (-1 = unlocked, else offset into sigma),
+ * sigma = concatenated na*nh_actor blocks. Replaces update_locks /
+ * build_lock_table (gpu/mod.rs:894-905, 961-973). n_sigma may be 0. */
+int gto_hip_update_locks(gto_hip* h, const int64_t* lock_off, const float* sigma,
+                         uint64_t n_sigma);
+
+int gto_hip_synchronize(gto_hip* h);
+
+/* Per-kernel-kind device time over the last `gto_hip_profile_iterations`
+ * call, in ms (HIP events around each launch group). Replaces
+ * iterate_profiled / Profile (gpu/mod.rs:22-54, 351-362). */
+#define GTO_PROF_KINDS 8
+typedef struct gto_hip_profile {
+    double ms[GTO_PROF_KINDS];
+    uint32_t launches[GTO_PROF_KINDS];
+} gto_hip_profile;
+const char* gto_hip_profile_kind_name(int k);
+int gto_hip_profile_iterations(gto_hip* h, uint32_t n, gto_hip_profile* out);
+
+/* Average device time (ms) of the action-node update kernel over the last
+ * timed `gto_hip_iterate` calls, and its algorithmic bytes per launch
+ * group; used for the roofline line of the benchmark. */
+int gto_hip_kernel_stats(gto_hip* h, double* update_ms_per_iter, double* arena_bytes_per_iter,
+                         double* total_ms_per_iter);
+int gto_hip_enable_timing(gto_hip* h, int on);
+
+/* Algorithmic arena entries D = sum over reached action nodes of
+ * na * nh_actor, the unit of the roofline bytes (20 B per entry per f32
+ * iteration, 10 B compressed). */
+uint64_t gto_hip_reached_entries(const gto_hip* h);
+
+/* VRAM the engine would allocate for `spot` (gpu/mod.rs:946; game.rs:365-373). */
+uint64_t gto_hip_estimate_vram(const gto_spot_view* spot, int storage);
+/* Free/total device memory in MB (gpu/mod.rs:953-958). */
+int gto_hip_vram_info(int device, uint64_t* free_mb, uint64_t* total_mb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GTOPEN_AMD_H */
