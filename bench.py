@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""DCFR benchmark on one MI355X: flop SRP spot (1,445,723-node tree).
+
+A step is one DCFR iteration (both traversers, alternating updates) over the
+whole tree, with the arenas resident in HBM. Prints one JSON line:
+iterations/s (value), seconds to 0.3% pot exploitability, the roofline of the
+dominant kernel and the CPU oracle's rate on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="flop_srp", choices=["river", "turn", "flop_srp"])
+    ap.add_argument("--target", type=float, default=0.3, help="exploitability target, % of pot")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-target", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the tree would shard by turn card across GPUs; not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    cfg = workloads.CONFIGS[args.config]()
+    t0 = time.time()
+    spot = Spot(cfg)
+    log(f"bench: {args.config} spot built in {time.time() - t0:.2f}s: {spot.num_nodes} nodes, "
+        f"{spot.num_action_nodes} action nodes, hands {spot.num_hands}")
+    pot = spot.starting_pot
+
+    gpu = GpuSolver(spot, storage="f32")
+    D = gpu.reached_entries
+    gpu.iterate(args.warmup)
+    gpu.synchronize()
+    log(f"bench: warmup {args.warmup} done")
+    t0 = time.perf_counter()
+    gpu.iterate(args.steps)
+    gpu.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    its = args.steps / dt
+    e_after = gpu.exploitability()
+    log(f"bench: {args.steps} steps in {dt:.3f}s = {ms_per_step:.3f} ms/iter, "
+        f"exploitability after {gpu.iteration} iterations {e_after / pot * 100:.4f}% pot")
+
+    # per-kernel split with HIP events on the engine's stream, over the same
+    # number of steps (eager launches, one event pair per launch)
+    prof = gpu.profile(args.steps)
+    kinds = {k: v for k, v in prof.items() if v[1] > 0}
+    dom = max(kinds, key=lambda k: kinds[k][0])
+    dom_ms_iter = kinds[dom][0] / args.steps
+    dom_launches = kinds[dom][1]
+    log("bench: kernel split (ms/iter, launches/iter): " +
+        ", ".join(f"{k} {v[0] / args.steps:.3f} ({v[1] // args.steps})" for k, v in kinds.items()))
+    # algorithmic bytes: regret/strategy arena entries D; the update kernel
+    # reads+writes R and S (16 B per entry per iteration, both sweeps), the
+    # down sweep reads R once more (4 B): 20*D per iteration in total.
+    alg = {"up_action": 16.0 * D, "down_action": 4.0 * D}
+    dom_bytes = alg.get(dom)
+    roofline = None
+    if dom_bytes is not None:
+        achieved = dom_bytes / (dom_ms_iter * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_us": round(kinds[dom][0] * 1e3 / dom_launches, 2),
+                    "algorithmic_bytes_per_iter": dom_bytes}
+    whole = 20.0 * D / (ms_per_step * 1e-3) / 1e9
+
+    sec_to_target, iters_to_target, e_final = None, None, None
+    if not args.no_target:
+        g2 = GpuSolver(spot, storage="f32")
+        t0 = time.perf_counter()
+        while g2.iteration < 2000:
+            g2.iterate(10)
+            e_final = g2.exploitability() / pot * 100.0
+            if e_final <= args.target:
+                break
+        sec_to_target = time.perf_counter() - t0
+        iters_to_target = g2.iteration
+        log(f"bench: {iters_to_target} iterations to {e_final:.4f}% pot in {sec_to_target:.3f}s (checks every 10)")
+        g2.close()
+    gpu.close()
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        threads = os.cpu_count() or 1
+        oracle.set_threads(threads)
+        o = oracle.Oracle(spot)
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            o.iterate(1)
+            n += 1
+            if time.perf_counter() - t0 >= args.cpu_seconds or n >= 50:
+                break
+        cdt = time.perf_counter() - t0
+        cpu = {"value": round(n / cdt, 4), "unit": "DCFR iters/s", "cores": threads, "kind": "port",
+               "sample": f"first {n} DCFR iterations of the same {args.config} spot, f32 arenas, "
+                         f"C++ oracle (restated reference CPU engine) on {threads} threads"}
+        log(f"bench: cpu oracle {n} iterations in {cdt:.2f}s on {threads} threads")
+
+    out = {
+        "metric": "DCFR iters/s on the flop SRP spot (f32 arenas); sec to 0.3%-pot exploitability alongside",
+        "value": round(its, 3),
+        "unit": "DCFR iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
+                   "hands": list(spot.num_hands), "arena_entries_D": D, "parallelism": "single GPU"},
+        "sec_to_target": None if sec_to_target is None else round(sec_to_target, 3),
+        "iters_to_target": iters_to_target,
+        "target_pct_pot": args.target,
+        "whole_iteration_arena_GBps": round(whole, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

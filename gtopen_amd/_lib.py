@@ -1,0 +1,141 @@
+"""ctypes binding of libgtopen_amd.so (declarations mirror include/gtopen_amd.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C gtopen_amd/csrc``). Loading fails loudly when it is missing: there
+is no fallback path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgtopen_amd.so")
+
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_u16p = ctypes.POINTER(ctypes.c_uint16)
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_f32p = ctypes.POINTER(ctypes.c_float)
+c_f64p = ctypes.POINTER(ctypes.c_double)
+
+GTO_OK = 0
+KIND_ACTION, KIND_CHANCE, KIND_FOLD, KIND_SHOWDOWN = 0, 1, 2, 3
+SENTINEL = 0xFFFFFFFF
+STORAGE_F32, STORAGE_COMPRESSED = 0, 1
+ALGO_DCFR, ALGO_CFRPLUS = 0, 1
+STRICT, LENIENT_LOAD = 0, 1
+PROF_KINDS = 8
+
+
+class SpotView(ctypes.Structure):
+    _fields_ = [
+        ("num_nodes", ctypes.c_uint32),
+        ("node_kind", c_u8p),
+        ("node_player", c_u8p),
+        ("node_street", c_u8p),
+        ("node_num_children", c_u8p),
+        ("node_children_start", c_u32p),
+        ("node_data_offset", c_u64p),
+        ("node_t_win", c_f64p),
+        ("node_t_lose", c_f64p),
+        ("node_t_tie", c_f64p),
+        ("node_put", c_f64p),
+        ("num_children_total", ctypes.c_uint32),
+        ("children", c_u32p),
+        ("data_size", ctypes.c_uint64 * 2),
+        ("root_street", ctypes.c_uint32),
+        ("starting_pot", ctypes.c_double),
+        ("effective_stack", ctypes.c_double),
+        ("rake_pct", ctypes.c_double),
+        ("rake_cap", ctypes.c_double),
+        ("board_len", ctypes.c_uint32),
+        ("board", ctypes.c_uint8 * 5),
+        ("board_mask", ctypes.c_uint64),
+        ("num_hands", ctypes.c_uint32 * 2),
+        ("hand_c1", c_u8p * 2),
+        ("hand_c2", c_u8p * 2),
+        ("weights", c_f32p * 2),
+        ("same_combo", c_u32p * 2),
+        ("num_river_keys", ctypes.c_uint32),
+        ("river_off", c_i64p * 2),
+        ("river_cnt", c_u32p * 2),
+        ("river_str", c_u32p * 2),
+        ("river_idx", c_u16p * 2),
+    ]
+
+
+class HipOptions(ctypes.Structure):
+    _fields_ = [
+        ("storage", ctypes.c_int),
+        ("algo", ctypes.c_int),
+        ("use_graphs", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class HipProfile(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * PROF_KINDS), ("launches", ctypes.c_uint32 * PROF_KINDS)]
+
+
+# (name, restype, argtypes) for every symbol declared in include/gtopen_amd.h
+SIGNATURES = [
+    ("gto_last_error", ctypes.c_char_p, []),
+    ("gto_spot_build", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("gto_spot_free", None, [ctypes.c_void_p]),
+    ("gto_spot_view_of", ctypes.POINTER(SpotView), [ctypes.c_void_p]),
+    ("gto_spot_num_action_nodes", ctypes.c_uint64, [ctypes.c_void_p]),
+    ("gto_spot_arena_bytes", ctypes.c_uint64, [ctypes.c_void_p, ctypes.c_int]),
+    ("gto_spot_num_suit_perms", ctypes.c_uint32, [ctypes.c_void_p]),
+    ("gto_combo_index", ctypes.c_uint32, [ctypes.c_uint8, ctypes.c_uint8]),
+    ("gto_evaluate7", ctypes.c_uint32, [c_u8p]),
+    ("gto_range_parse", ctypes.c_int, [ctypes.c_char_p, c_f32p]),
+    ("gto_canonical_flops", ctypes.c_int, [ctypes.c_uint32, c_u8p, c_u32p, ctypes.c_uint32, c_u32p]),
+    ("gto_hip_create", ctypes.c_int, [ctypes.POINTER(SpotView), ctypes.POINTER(HipOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    ("gto_hip_destroy", None, [ctypes.c_void_p]),
+    ("gto_hip_iterate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gto_hip_get_iteration", ctypes.c_int, [ctypes.c_void_p, c_u32p]),
+    ("gto_hip_set_iteration", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gto_hip_exploitability", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("gto_hip_root_cfv", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_f32p]),
+    ("gto_hip_download_arenas", ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p, c_f32p, c_f32p]),
+    ("gto_hip_upload_arenas", ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p, c_f32p, c_f32p]),
+    ("gto_hip_download_compressed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int16), c_u16p, c_f32p, c_f32p]),
+    ("gto_hip_update_locks", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_f32p, ctypes.c_uint64]),
+    ("gto_hip_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("gto_hip_profile_kind_name", ctypes.c_char_p, [ctypes.c_int]),
+    ("gto_hip_profile_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(HipProfile)]),
+    ("gto_hip_kernel_stats", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),
+    ("gto_hip_enable_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("gto_hip_reached_entries", ctypes.c_uint64, [ctypes.c_void_p]),
+    ("gto_hip_estimate_vram", ctypes.c_uint64, [ctypes.POINTER(SpotView), ctypes.c_int]),
+    ("gto_hip_vram_info", ctypes.c_int, [ctypes.c_int, c_u64p, c_u64p]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libgtopen_amd.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() or make -C gtopen_amd/csrc"
+            )
+        h = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, res, args in SIGNATURES:
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+class GtoError(RuntimeError):
+    pass
+
+
+def check(rc, what):
+    if rc != GTO_OK:
+        msg = lib().gto_last_error()
+        raise GtoError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

@@ -1,0 +1,664 @@
+// MI355X DCFR engine: device state, level-synchronous sweeps, hipGraph
+// replay of steady-state iterations, best-response exploitability, and the
+// gto_hip_* C ABI (replacing solver::gpu::GpuSolver,
+// crates/solver/src/gpu/mod.rs:60-997).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/gtopen_amd.h"
+#include "kernels.hip"
+#include "plan.h"
+
+namespace gto {
+void set_last_error(const std::string& msg);
+}
+
+namespace {
+
+using namespace gto;
+
+#define HIP_OK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw std::runtime_error(std::string("hip: ") + hipGetErrorString(e_) + " in " #expr); \
+    } while (0)
+
+enum ProfKind { PK_ROOT = 0, PK_DOWN_ACTION, PK_DOWN_CHANCE, PK_UP_FOLD, PK_UP_SHOW, PK_UP_CHANCE, PK_UP_ACTION, PK_OTHER };
+const char* kProfNames[GTO_PROF_KINDS] = {"root", "down_action", "down_chance", "up_fold",
+                                          "up_show", "up_chance", "up_action", "other"};
+
+struct DevLevel {
+    uint32_t* act[2] = {nullptr, nullptr};
+    uint32_t n_act[2] = {0, 0};
+    uint32_t* fold = nullptr;
+    uint32_t n_fold = 0;
+    uint32_t* show = nullptr;
+    uint32_t n_show = 0;
+    uint32_t* chance = nullptr;
+    uint32_t n_chance = 0;
+    uint32_t* e_parent = nullptr;
+    uint32_t* e_child = nullptr;
+    uint32_t* e_card = nullptr;
+    uint32_t n_edge = 0;
+};
+
+// Discounts::for_iteration (cfr.rs:44-63), computed in f64 and stored as f32.
+void discounts(int algo, uint32_t it, float* out3) {
+    double t = (double)it;
+    double q = t / (t + 1.0);
+    if (algo == GTO_ALGO_DCFR) {
+        double ta = std::pow(t, 1.5);
+        out3[0] = (float)(ta / (ta + 1.0));
+        out3[1] = 0.5f;
+    } else {
+        out3[0] = 1.0f;
+        out3[1] = 0.0f;
+    }
+    out3[2] = (float)(q * q);
+}
+
+class Engine {
+   public:
+    Engine(const gto_spot_view& v, const gto_hip_options& o) : opts_(o) {
+        if (o.storage != GTO_STORAGE_F32)
+            throw std::runtime_error("compressed device arenas are not enabled in this build");
+        HIP_OK(hipSetDevice(o.device));
+        HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        plan_ = build_plan(v);
+        num_nodes_ = v.num_nodes;
+        data_size_[0] = v.data_size[0];
+        data_size_[1] = v.data_size[1];
+        rake_ = v.rake_pct;
+        for (int q = 0; q < 2; ++q) {
+            weights_[q].assign(v.weights[q], v.weights[q] + v.num_hands[q]);
+            c1_[q].assign(v.hand_c1[q], v.hand_c1[q] + v.num_hands[q]);
+            c2_[q].assign(v.hand_c2[q], v.hand_c2[q] + v.num_hands[q]);
+            same_[q].assign(v.same_combo[q], v.same_combo[q] + v.num_hands[q]);
+        }
+        upload_tree(v);
+        alloc_state();
+        reach_valid_[0] = reach_valid_[1] = false;
+    }
+
+    ~Engine() {
+        drop_graph();
+        for (void* p : allocs_) (void)hipFree(p);
+        if (stream_) (void)hipStreamDestroy(stream_);
+        for (auto& e : evs_) (void)hipEventDestroy(e);
+    }
+
+    uint32_t iteration() const { return iteration_; }
+    void set_iteration(uint32_t t) {
+        iteration_ = t;
+        ensure_disc(t + 1);
+        HIP_OK(hipMemcpyAsync(S_.iter, &iteration_, sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+
+    void iterate(uint32_t n) {
+        ensure_disc(iteration_ + n + 1);
+        for (uint32_t k = 0; k < n; ++k) {
+            bool steady = reach_valid_[0] && !reach_valid_[1];
+            if (opts_.use_graphs && steady && !prof_) {
+                if (!graph_) capture_graph();
+                HIP_OK(hipGraphLaunch(graph_, stream_));
+                reach_valid_[0] = true;
+                reach_valid_[1] = false;
+            } else {
+                launch_iteration();
+            }
+            iteration_ += 1;
+        }
+    }
+
+    double exploitability() {
+        double denom = pair_weight_sum();
+        if (denom <= 0.0) return NAN;
+        double br[2], val[2];
+        std::vector<float> c;
+        for (int p = 0; p < 2; ++p) {
+            root_cfv(p, 0, c);
+            br[p] = dot(p, c) / denom;
+        }
+        if (rake_ > 0.0) {
+            for (int p = 0; p < 2; ++p) {
+                root_cfv(p, 1, c);
+                val[p] = dot(p, c) / denom;
+            }
+            return ((br[0] - val[0]) + (br[1] - val[1])) / 2.0;
+        }
+        return (br[0] + br[1]) / 2.0;
+    }
+
+    // mode 0 = best response, 1 = average strategy, 2 = current strategy
+    void root_cfv(int p, int mode, std::vector<float>& out) {
+        const int o = 1 - p;
+        if (mode == 2) {
+            // current strategy for both players: the opponent's reach under
+            // regret matching, values weighted by the traverser's sigma
+            launch_root(1 << o);
+            launch_down(1 << o, kCurrentF32);
+            reach_valid_[o] = true;
+            launch_up(p, 3);
+        } else {
+            launch_root(1 << o);
+            launch_down(1 << o, kAverageF32);
+            reach_valid_[o] = false;
+            launch_up(p, mode == 0 ? 1 : 2);
+        }
+        out.resize(plan_.nh[p]);
+        HIP_OK(hipMemcpyAsync(out.data(), S_.cfv + (size_t)plan_.cfv_slot[0] * plan_.nh_max,
+                              sizeof(float) * plan_.nh[p], hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+
+    void download(int which, int p, float* dst) {
+        if (!dst || data_size_[p] == 0) return;
+        HIP_OK(hipMemcpyAsync(dst, which == 0 ? S_.R[p] : S_.S[p], sizeof(float) * data_size_[p],
+                              hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+    void upload(int which, int p, const float* src) {
+        if (!src || data_size_[p] == 0) return;
+        HIP_OK(hipMemcpyAsync(which == 0 ? S_.R[p] : S_.S[p], src, sizeof(float) * data_size_[p],
+                              hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        reach_valid_[0] = reach_valid_[1] = false;
+    }
+
+    void update_locks(const int64_t* lock_off, const float* sigma, uint64_t n_sigma) {
+        HIP_OK(hipStreamSynchronize(stream_));
+        drop_graph();
+        std::vector<int64_t> lo(num_nodes_, -1);
+        if (lock_off) std::copy(lock_off, lock_off + num_nodes_, lo.begin());
+        HIP_OK(hipMemcpy((void*)T_.lock_off, lo.data(), sizeof(int64_t) * num_nodes_, hipMemcpyHostToDevice));
+        if (lock_sigma_buf_) {
+            HIP_OK(hipFree(lock_sigma_buf_));
+            allocs_.erase(std::find(allocs_.begin(), allocs_.end(), (void*)lock_sigma_buf_));
+            lock_sigma_buf_ = nullptr;
+        }
+        if (n_sigma) {
+            lock_sigma_buf_ = (float*)dalloc(sizeof(float) * n_sigma);
+            HIP_OK(hipMemcpy(lock_sigma_buf_, sigma, sizeof(float) * n_sigma, hipMemcpyHostToDevice));
+        }
+        T_.lock_sigma = lock_sigma_buf_;
+        reach_valid_[0] = reach_valid_[1] = false;
+    }
+
+    void synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }
+
+    void profile(uint32_t n, gto_hip_profile* out) {
+        std::memset(out, 0, sizeof(*out));
+        prof_ = out;
+        ensure_disc(iteration_ + n + 1);
+        for (uint32_t k = 0; k < n; ++k) {
+            launch_iteration();
+            iteration_ += 1;
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+        flush_prof();
+        prof_ = nullptr;
+    }
+
+    uint64_t reached_entries() const { return plan_.reached_entries; }
+
+    static uint64_t estimate(const gto_spot_view& v) {
+        // arenas + reach slots + value rows + tables (upper bound)
+        uint64_t n = v.num_nodes;
+        uint64_t nh0 = v.num_hands[0], nh1 = v.num_hands[1], nhm = std::max(nh0, nh1);
+        uint64_t arenas = (v.data_size[0] + v.data_size[1]) * 8;
+        uint64_t reach = n * (nh0 + nh1) * 4;
+        uint64_t cfv = n * nhm * 4;
+        return arenas + reach + cfv + n * 64 + (256ull << 20);
+    }
+
+   private:
+    gto_hip_options opts_;
+    hipStream_t stream_ = nullptr;
+    Plan plan_;
+    uint32_t num_nodes_ = 0;
+    uint64_t data_size_[2] = {0, 0};
+    double rake_ = 0.0;
+    std::vector<float> weights_[2];
+    std::vector<uint8_t> c1_[2], c2_[2];
+    std::vector<uint32_t> same_[2];
+    std::vector<void*> allocs_;
+    DevTree T_{};
+    DevState S_{};
+    std::vector<DevLevel> lv_;
+    uint32_t iteration_ = 0;
+    bool reach_valid_[2] = {false, false};
+    uint32_t disc_cap_ = 0;
+    float* disc_buf_ = nullptr;
+    float* lock_sigma_buf_ = nullptr;
+    size_t show_lds_[2] = {0, 0};
+    hipGraphExec_t graph_ = nullptr;
+    gto_hip_profile* prof_ = nullptr;
+    std::vector<hipEvent_t> evs_;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_;
+
+    void* dalloc(size_t bytes) {
+        void* p = nullptr;
+        if (bytes == 0) bytes = 16;
+        HIP_OK(hipMalloc(&p, bytes));
+        allocs_.push_back(p);
+        return p;
+    }
+    template <class T>
+    T* upload_vec(const std::vector<T>& v) {
+        T* d = (T*)dalloc(sizeof(T) * std::max<size_t>(v.size(), 1));
+        if (!v.empty()) HIP_OK(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+        return d;
+    }
+    template <class T>
+    T* upload_arr(const T* src, size_t n) {
+        T* d = (T*)dalloc(sizeof(T) * std::max<size_t>(n, 1));
+        if (n) HIP_OK(hipMemcpy(d, src, sizeof(T) * n, hipMemcpyHostToDevice));
+        return d;
+    }
+
+    void upload_tree(const gto_spot_view& v) {
+        const uint32_t N = v.num_nodes;
+        T_.player = upload_arr(v.node_player, N);
+        T_.num_children = upload_arr(v.node_num_children, N);
+        T_.street = upload_arr(v.node_street, N);
+        T_.children_start = upload_arr(v.node_children_start, N);
+        T_.data_offset = upload_arr(v.node_data_offset, N);
+        std::vector<float> tw(N), tl(N), tt(N);
+        for (uint32_t n = 0; n < N; ++n) {
+            tw[n] = (float)v.node_t_win[n];
+            tl[n] = (float)v.node_t_lose[n];
+            tt[n] = (float)v.node_t_tie[n];
+        }
+        T_.t_win = upload_vec(tw);
+        T_.t_lose = upload_vec(tl);
+        T_.t_tie = upload_vec(tt);
+        T_.children = upload_arr(v.children, v.num_children_total);
+        for (int q = 0; q < 2; ++q) T_.owner[q] = upload_vec(plan_.owner[q]);
+        T_.cfv_slot = upload_vec(plan_.cfv_slot);
+        T_.ch_off = upload_vec(plan_.ch_off);
+        T_.ch_cnt = upload_vec(plan_.ch_cnt);
+        T_.ch_child = upload_vec(plan_.ch_child);
+        T_.ch_card = upload_vec(plan_.ch_card);
+        T_.show_slot = upload_vec(plan_.show_slot);
+        std::vector<int64_t> lo(N, -1);
+        T_.lock_off = upload_vec(lo);
+        T_.lock_sigma = nullptr;
+        T_.nh_max = plan_.nh_max;
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t nh = v.num_hands[q];
+            T_.nh[q] = nh;
+            T_.c1[q] = upload_arr(v.hand_c1[q], nh);
+            T_.c2[q] = upload_arr(v.hand_c2[q], nh);
+            std::vector<uint64_t> m(nh);
+            for (uint32_t i = 0; i < nh; ++i) m[i] = (1ull << v.hand_c1[q][i]) | (1ull << v.hand_c2[q][i]);
+            T_.hmask[q] = upload_vec(m);
+            T_.same[q] = upload_arr(v.same_combo[q], nh);
+            T_.weights[q] = upload_arr(v.weights[q], nh);
+            T_.hc_off[q] = upload_vec(plan_.hc_off[q]);
+            T_.hc_idx[q] = upload_vec(plan_.hc_idx[q]);
+            T_.sd_off[q] = upload_vec(plan_.sd_off[q]);
+            T_.sd_cnt[q] = upload_vec(plan_.sd_cnt[q]);
+            T_.sd_idx[q] = upload_vec(plan_.sd_idx[q]);
+            T_.sd_lbub[q] = upload_vec(plan_.sd_lbub[q]);
+            T_.sd_cnt4[q] = upload_vec(plan_.sd_cnt4[q]);
+            T_.sd_coff[q] = upload_vec(plan_.sd_coff[q]);
+            T_.sd_cpos_base[q] = upload_vec(plan_.sd_cpos_base[q]);
+            T_.sd_cpos[q] = upload_vec(plan_.sd_cpos[q]);
+        }
+        // showdown LDS for traverser p depends on the opponent's largest board list
+        for (int p = 0; p < 2; ++p) {
+            size_t mo = plan_.max_board_cnt[1 - p];
+            show_lds_[p] = 16 * (mo + 1) + 16 * (2 * mo + 52) + 4 * mo + 16;
+        }
+        size_t lds_max = std::max(show_lds_[0], show_lds_[1]);
+        if (lds_max > 160 * 1024 - 8 * 1024)
+            throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
+        HIP_OK(hipFuncSetAttribute((const void*)k_up_show, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)std::max<size_t>(lds_max, 1024)));
+        // level lists
+        lv_.resize(plan_.levels.size());
+        for (size_t l = 0; l < plan_.levels.size(); ++l) {
+            const LevelLists& L = plan_.levels[l];
+            DevLevel& D = lv_[l];
+            for (int q = 0; q < 2; ++q) {
+                D.n_act[q] = (uint32_t)L.act[q].size();
+                if (D.n_act[q]) D.act[q] = upload_vec(L.act[q]);
+            }
+            D.n_fold = (uint32_t)L.fold.size();
+            if (D.n_fold) D.fold = upload_vec(L.fold);
+            D.n_show = (uint32_t)L.show.size();
+            if (D.n_show) D.show = upload_vec(L.show);
+            D.n_chance = (uint32_t)L.chance.size();
+            if (D.n_chance) D.chance = upload_vec(L.chance);
+            D.n_edge = (uint32_t)L.edge_parent.size();
+            if (D.n_edge) {
+                D.e_parent = upload_vec(L.edge_parent);
+                D.e_child = upload_vec(L.edge_child);
+                D.e_card = upload_vec(L.edge_card);
+            }
+        }
+    }
+
+    void alloc_state() {
+        for (int q = 0; q < 2; ++q) {
+            size_t ds = std::max<uint64_t>(data_size_[q], 1);
+            S_.R[q] = (float*)dalloc(sizeof(float) * ds);
+            S_.S[q] = (float*)dalloc(sizeof(float) * ds);
+            HIP_OK(hipMemset(S_.R[q], 0, sizeof(float) * ds));
+            HIP_OK(hipMemset(S_.S[q], 0, sizeof(float) * ds));
+            S_.reach[q] = (float*)dalloc(sizeof(float) * (size_t)plan_.num_slots[q] * plan_.nh[q]);
+            S_.alive[q] = (uint8_t*)dalloc(plan_.num_slots[q]);
+            HIP_OK(hipMemset(S_.alive[q], 0, plan_.num_slots[q]));
+        }
+        S_.cfv = (float*)dalloc(sizeof(float) * (size_t)plan_.num_reached * plan_.nh_max);
+        S_.iter = (uint32_t*)dalloc(sizeof(uint32_t));
+        HIP_OK(hipMemset(S_.iter, 0, sizeof(uint32_t)));
+        ensure_disc(1024);
+        HIP_OK(hipDeviceSynchronize());
+    }
+
+    void ensure_disc(uint32_t need) {
+        if (need < disc_cap_) return;
+        uint32_t cap = std::max<uint32_t>(disc_cap_ * 2, 1024);
+        while (cap <= need) cap *= 2;
+        std::vector<float> h((size_t)cap * 3);
+        h[0] = h[1] = h[2] = 0.0f;
+        for (uint32_t t = 1; t < cap; ++t) discounts(opts_.algo, t, &h[3 * (size_t)t]);
+        HIP_OK(hipStreamSynchronize(stream_));
+        float* d = (float*)dalloc(sizeof(float) * h.size());
+        HIP_OK(hipMemcpy(d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+        if (disc_buf_) {
+            drop_graph();
+            HIP_OK(hipFree(disc_buf_));
+            allocs_.erase(std::find(allocs_.begin(), allocs_.end(), (void*)disc_buf_));
+        }
+        disc_buf_ = d;
+        S_.disc = d;
+        disc_cap_ = cap;
+    }
+
+    void drop_graph() {
+        if (graph_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+    }
+
+    // --- launch helpers with optional per-kind event timing ---------------
+    template <class K, class... Args>
+    void launch(int kind, K kernel, uint32_t grid, size_t lds, Args... args) {
+        if (grid == 0) return;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (prof_) {
+            e0 = event();
+            HIP_OK(hipEventRecord(e0, stream_));
+        }
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, stream_, args...);
+        HIP_OK(hipGetLastError());
+        if (prof_) {
+            e1 = event();
+            HIP_OK(hipEventRecord(e1, stream_));
+            pending_.push_back({kind, {e0, e1}});
+        }
+    }
+    hipEvent_t event() {
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        evs_.push_back(e);
+        return e;
+    }
+    void flush_prof() {
+        for (auto& pe : pending_) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+            prof_->ms[pe.first] += ms;
+            prof_->launches[pe.first] += 1;
+        }
+        pending_.clear();
+        for (auto& e : evs_) (void)hipEventDestroy(e);
+        evs_.clear();
+    }
+
+    void launch_root(int mask) { launch(PK_ROOT, k_root, 1, 0, T_, S_, mask); }
+
+    void launch_down(int mask, int kind) {
+        for (size_t l = 0; l < lv_.size(); ++l) {
+            const DevLevel& D = lv_[l];
+            for (int q = 0; q < 2; ++q) {
+                if (!((mask >> q) & 1) || !D.n_act[q]) continue;
+                if (kind == kAverageF32)
+                    launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, D.n_act[q], 0, T_, S_, (const uint32_t*)D.act[q], q);
+                else
+                    launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, D.n_act[q], 0, T_, S_, (const uint32_t*)D.act[q], q);
+            }
+            if (D.n_edge)
+                launch(PK_DOWN_CHANCE, k_down_chance, D.n_edge, 0, T_, S_, (const uint32_t*)D.e_parent,
+                       (const uint32_t*)D.e_child, (const uint32_t*)D.e_card, mask);
+        }
+    }
+
+    // mode 0 = DCFR update, 1 = best response, 2 = average values, 3 = current values
+    void launch_up(int p, int mode) {
+        for (size_t l = lv_.size(); l-- > 0;) {
+            const DevLevel& D = lv_[l];
+            launch(PK_UP_FOLD, k_up_fold, D.n_fold, 0, T_, S_, (const uint32_t*)D.fold, p);
+            launch(PK_UP_SHOW, k_up_show, D.n_show, show_lds_[p], T_, S_, (const uint32_t*)D.show, p);
+            launch(PK_UP_CHANCE, k_up_chance, D.n_chance, 0, T_, S_, (const uint32_t*)D.chance, p);
+            for (int q = 0; q < 2; ++q) {
+                if (!D.n_act[q]) continue;
+                const uint32_t* nodes = D.act[q];
+                switch (mode) {
+                    case 0: launch(PK_UP_ACTION, k_up_action<0, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                    case 1: launch(PK_UP_ACTION, k_up_action<1, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                    case 2: launch(PK_UP_ACTION, k_up_action<2, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                    default: launch(PK_UP_ACTION, k_up_action<3, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                }
+            }
+        }
+    }
+
+    // One DCFR iteration: traverser 0 then 1 (cfr.rs:170-190). Each sweep
+    // recomputes only the reach vectors whose regrets changed since they
+    // were last pushed down.
+    void launch_iteration() {
+        launch(PK_OTHER, k_begin_iteration, 1, 0, S_);
+        for (int p = 0; p < 2; ++p) {
+            int mask = 0;
+            for (int q = 0; q < 2; ++q)
+                if (!reach_valid_[q]) mask |= 1 << q;
+            if (mask) {
+                launch_root(mask);
+                launch_down(mask, kCurrentF32);
+            }
+            reach_valid_[0] = reach_valid_[1] = true;
+            launch_up(p, 0);
+            reach_valid_[p] = false;
+        }
+    }
+
+    void capture_graph() {
+        HIP_OK(hipStreamSynchronize(stream_));
+        bool v0 = reach_valid_[0], v1 = reach_valid_[1];
+        HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            launch_iteration();
+        } catch (...) {
+            hipGraph_t g;
+            (void)hipStreamEndCapture(stream_, &g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        HIP_OK(hipStreamEndCapture(stream_, &g));
+        HIP_OK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+        HIP_OK(hipGraphDestroy(g));
+        reach_valid_[0] = v0;
+        reach_valid_[1] = v1;
+    }
+
+    double pair_weight_sum() const {  // best_response.rs:227-251
+        double t = 0.0, s[52];
+        std::fill(s, s + 52, 0.0);
+        for (size_t j = 0; j < weights_[1].size(); ++j) {
+            double r = (double)weights_[1][j];
+            t += r;
+            s[c1_[1][j]] += r;
+            s[c2_[1][j]] += r;
+        }
+        double total = 0.0;
+        for (size_t i = 0; i < weights_[0].size(); ++i) {
+            uint32_t sm = same_[0][i];
+            double same_r = sm != 0xFFFFFFFFu ? (double)weights_[1][sm] : 0.0;
+            double valid = t - s[c1_[0][i]] - s[c2_[0][i]] + same_r;
+            total += (double)weights_[0][i] * valid;
+        }
+        return total;
+    }
+
+    double dot(int p, const std::vector<float>& c) const {
+        double acc = 0.0;
+        for (size_t i = 0; i < c.size(); ++i) acc += (double)weights_[p][i] * (double)c[i];
+        return acc;
+    }
+};
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return GTO_OK;
+    } catch (const std::bad_alloc&) {
+        set_last_error("out of host memory");
+        return GTO_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return GTO_ERR_DEVICE;
+    }
+}
+
+}  // namespace
+
+struct gto_hip {
+    std::unique_ptr<Engine> e;
+};
+
+extern "C" {
+
+int gto_hip_create(const gto_spot_view* spot, const gto_hip_options* opts, gto_hip** out) {
+    if (!spot || !out) {
+        set_last_error("gto_hip_create: null argument");
+        return GTO_ERR_INVALID;
+    }
+    *out = nullptr;
+    gto_hip_options o = {GTO_STORAGE_F32, GTO_ALGO_DCFR, 1, 0};
+    if (opts) o = *opts;
+    if (o.algo != GTO_ALGO_DCFR && o.algo != GTO_ALGO_CFRPLUS) {
+        set_last_error("GPU engine supports dcfr and cfr+ only");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        auto h = std::make_unique<gto_hip>();
+        h->e = std::make_unique<Engine>(*spot, o);
+        *out = h.release();
+    });
+}
+
+void gto_hip_destroy(gto_hip* h) { delete h; }
+
+int gto_hip_iterate(gto_hip* h, uint32_t n) {
+    return guarded([&] { h->e->iterate(n); });
+}
+
+int gto_hip_get_iteration(const gto_hip* h, uint32_t* it) {
+    *it = h->e->iteration();
+    return GTO_OK;
+}
+
+int gto_hip_set_iteration(gto_hip* h, uint32_t it) {
+    return guarded([&] { h->e->set_iteration(it); });
+}
+
+int gto_hip_exploitability(gto_hip* h, double* chips) {
+    return guarded([&] { *chips = h->e->exploitability(); });
+}
+
+int gto_hip_root_cfv(gto_hip* h, int p, int mode, float* out) {
+    if (p < 0 || p > 1 || mode < 0 || mode > 2) {
+        set_last_error("gto_hip_root_cfv: bad player or mode");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        std::vector<float> c;
+        h->e->root_cfv(p, mode, c);
+        std::copy(c.begin(), c.end(), out);
+    });
+}
+
+int gto_hip_download_arenas(gto_hip* h, float* r0, float* r1, float* s0, float* s1) {
+    return guarded([&] {
+        h->e->download(0, 0, r0);
+        h->e->download(0, 1, r1);
+        h->e->download(1, 0, s0);
+        h->e->download(1, 1, s1);
+    });
+}
+
+int gto_hip_upload_arenas(gto_hip* h, const float* r0, const float* r1, const float* s0, const float* s1) {
+    return guarded([&] {
+        h->e->upload(0, 0, r0);
+        h->e->upload(0, 1, r1);
+        h->e->upload(1, 0, s0);
+        h->e->upload(1, 1, s1);
+    });
+}
+
+int gto_hip_download_compressed(gto_hip*, int, int16_t*, uint16_t*, float*, float*) {
+    set_last_error("engine was created with f32 arenas");
+    return GTO_ERR_INVALID;
+}
+
+int gto_hip_update_locks(gto_hip* h, const int64_t* lock_off, const float* sigma, uint64_t n_sigma) {
+    return guarded([&] { h->e->update_locks(lock_off, sigma, n_sigma); });
+}
+
+int gto_hip_synchronize(gto_hip* h) {
+    return guarded([&] { h->e->synchronize(); });
+}
+
+const char* gto_hip_profile_kind_name(int k) { return (k >= 0 && k < GTO_PROF_KINDS) ? kProfNames[k] : "?"; }
+
+int gto_hip_profile_iterations(gto_hip* h, uint32_t n, gto_hip_profile* out) {
+    return guarded([&] { h->e->profile(n, out); });
+}
+
+int gto_hip_kernel_stats(gto_hip*, double*, double*, double*) {
+    set_last_error("use gto_hip_profile_iterations");
+    return GTO_ERR_INVALID;
+}
+
+int gto_hip_enable_timing(gto_hip*, int) { return GTO_OK; }
+
+uint64_t gto_hip_reached_entries(const gto_hip* h) { return h->e->reached_entries(); }
+
+uint64_t gto_hip_estimate_vram(const gto_spot_view* spot, int) { return Engine::estimate(*spot); }
+
+int gto_hip_vram_info(int device, uint64_t* free_mb, uint64_t* total_mb) {
+    return guarded([&] {
+        HIP_OK(hipSetDevice(device));
+        size_t fr = 0, tot = 0;
+        HIP_OK(hipMemGetInfo(&fr, &tot));
+        *free_mb = fr >> 20;
+        *total_mb = tot >> 20;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""GpuSolver: the MI355X engine behind the reference's GpuSolver interface
+(crates/solver/src/gpu/mod.rs:60-997): new / iterate / exploitability /
+sync (download) / update_locks / synchronize / iteration, over the C ABI.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class GpuSolver:
+    def __init__(self, spot, storage="f32", algo="dcfr", use_graphs=True, device=0):
+        self.spot = spot  # keeps the view's arrays alive
+        opts = L.HipOptions(
+            L.STORAGE_F32 if storage == "f32" else L.STORAGE_COMPRESSED,
+            L.ALGO_DCFR if algo == "dcfr" else L.ALGO_CFRPLUS,
+            1 if use_graphs else 0,
+            device,
+        )
+        self.storage = storage
+        h = ctypes.c_void_p()
+        L.check(L.lib().gto_hip_create(spot.view_ptr, ctypes.byref(opts), ctypes.byref(h)), "gpu create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().gto_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def iteration(self):
+        it = ctypes.c_uint32()
+        L.check(L.lib().gto_hip_get_iteration(self._h, ctypes.byref(it)), "get iteration")
+        return it.value
+
+    @iteration.setter
+    def iteration(self, t):
+        L.check(L.lib().gto_hip_set_iteration(self._h, t), "set iteration")
+
+    def iterate(self, n=1):
+        L.check(L.lib().gto_hip_iterate(self._h, n), "gpu iterate")
+
+    def exploitability(self):
+        """Exploitability in chips (best_response.rs:255-287)."""
+        e = ctypes.c_double()
+        L.check(L.lib().gto_hip_exploitability(self._h, ctypes.byref(e)), "gpu exploitability")
+        return e.value
+
+    def root_cfv(self, p, mode=0):
+        out = np.zeros(self.spot.num_hands[p], dtype=np.float32)
+        L.check(
+            L.lib().gto_hip_root_cfv(self._h, p, mode, out.ctypes.data_as(L.c_f32p)), "gpu root cfv"
+        )
+        return out
+
+    def download_arenas(self):
+        """(regrets[2], strat[2]) as f32 numpy arrays (sync_to_cpu)."""
+        ds = self.spot.data_size
+        r = [np.zeros(ds[p], dtype=np.float32) for p in range(2)]
+        s = [np.zeros(ds[p], dtype=np.float32) for p in range(2)]
+        L.check(
+            L.lib().gto_hip_download_arenas(
+                self._h, *[a.ctypes.data_as(L.c_f32p) for a in (r[0], r[1], s[0], s[1])]
+            ),
+            "download arenas",
+        )
+        return r, s
+
+    def upload_arenas(self, regrets, strat):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (regrets[0], regrets[1], strat[0], strat[1])]
+        L.check(
+            L.lib().gto_hip_upload_arenas(self._h, *[a.ctypes.data_as(L.c_f32p) for a in arrs]), "upload arenas"
+        )
+
+    def update_locks(self, locks):
+        """locks: {node_index: sigma (na*nh float array)} (gpu/mod.rs:894-905)."""
+        off = np.full(self.spot.num_nodes, -1, dtype=np.int64)
+        parts = []
+        pos = 0
+        for node in sorted(locks):
+            sig = np.ascontiguousarray(locks[node], dtype=np.float32)
+            off[node] = pos
+            parts.append(sig)
+            pos += sig.size
+        sigma = np.concatenate(parts) if parts else np.zeros(1, dtype=np.float32)
+        L.check(
+            L.lib().gto_hip_update_locks(
+                self._h, off.ctypes.data_as(L.c_i64p), sigma.ctypes.data_as(L.c_f32p), pos
+            ),
+            "update locks",
+        )
+
+    def synchronize(self):
+        L.check(L.lib().gto_hip_synchronize(self._h), "synchronize")
+
+    def profile(self, n=1):
+        prof = L.HipProfile()
+        L.check(L.lib().gto_hip_profile_iterations(self._h, n, ctypes.byref(prof)), "profile")
+        return {
+            L.lib().gto_hip_profile_kind_name(k).decode(): (prof.ms[k], prof.launches[k]) for k in range(L.PROF_KINDS)
+        }
+
+    @property
+    def reached_entries(self):
+        return int(L.lib().gto_hip_reached_entries(self._h))
+
+
+def vram_info(device=0):
+    fr, tot = ctypes.c_uint64(), ctypes.c_uint64()
+    L.check(L.lib().gto_hip_vram_info(device, ctypes.byref(fr), ctypes.byref(tot)), "vram info")
+    return fr.value, tot.value

@@ -1,0 +1,123 @@
+"""GPU parity: the HIP engine against the CPU oracle and the reference's
+golden trajectory. Exploitability must agree within 1e-4 of the pot
+(f32 arithmetic on both sides; the GPU sums some f64 totals in a different
+order, so per-element arenas agree to float rounding, not bit for bit).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+POT_TOL = 1e-4
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _close_arenas(a, b, scale, rtol=2e-4):
+    for p in range(2):
+        x, y = a[p], b[p]
+        err = np.abs(x.astype(np.float64) - y.astype(np.float64))
+        lim = rtol * np.abs(y.astype(np.float64)) + rtol * scale
+        assert np.all(err <= lim), f"player {p}: max err {err.max()} at {int(err.argmax())}"
+
+
+def test_river_trajectory_matches_oracle(built):
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: river spot (400x400), 100 iterations, check every 10")
+    spot = Spot(workloads.river_spot())
+    gpu, cpu = GpuSolver(spot), oracle.Oracle(spot)
+    pot = spot.starting_pot
+    for k in range(10):
+        gpu.iterate(10)
+        cpu.iterate(10)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {10 * (k + 1)}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) / pot < POT_TOL
+    rg, sg = gpu.download_arenas()
+    rc, sc = cpu.arenas()
+    _close_arenas(rg, rc, pot)
+    _close_arenas(sg, sc, 1.0)
+
+
+def test_turn_matches_oracle(built):
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: turn spot (1128x1128, 28509 nodes), 12 iterations")
+    spot = Spot(workloads.turn_spot())
+    gpu, cpu = GpuSolver(spot), oracle.Oracle(spot)
+    pot = spot.starting_pot
+    for n in (2, 10):
+        gpu.iterate(n)
+        cpu.iterate(n)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {gpu.iteration}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) / pot < POT_TOL
+    rg, sg = gpu.download_arenas()
+    rc, sc = cpu.arenas()
+    _close_arenas(rg, rc, pot)
+    _close_arenas(sg, sc, 1.0)
+
+
+# bench/new_f32_dcfr.log:2-4 of the reference: the CPU f32 engine on
+# bench_spot.json (Ks7h2d, 1,245,971 nodes), exploitability in chips.
+GOLDEN_F32 = {10: 33.7904, 20: 9.3381, 30: 4.8733}
+
+
+def test_bench_spot_reproduces_reference_trajectory(built):
+    from gtopen_amd import GpuSolver, Spot
+
+    _say("case: bench_spot.json (reference golden trajectory), 30 iterations")
+    spot = Spot(open(golden("bench_spot.json")).read())
+    gpu = GpuSolver(spot)
+    for it in (10, 20, 30):
+        gpu.iterate(it - gpu.iteration)
+        e = gpu.exploitability()
+        _say(f"  iter {it}: gpu {e:.4f} chips, reference {GOLDEN_F32[it]:.4f}")
+        assert abs(e - GOLDEN_F32[it]) / spot.starting_pot < POT_TOL
+
+
+def test_best_response_matches_oracle_on_gpu_arenas(built):
+    """GPU best response and average-strategy values on the same arenas as the
+    CPU (best_response.rs:19-223), per hand."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: turn spot, BR/avg root values on GPU-produced arenas")
+    spot = Spot(workloads.turn_spot())
+    gpu = GpuSolver(spot)
+    gpu.iterate(6)
+    r, s = gpu.download_arenas()
+    cpu = oracle.Oracle(spot)
+    cpu.set_arenas(r, s)
+    for p in range(2):
+        for mode in (0, 1):
+            vg = gpu.root_cfv(p, mode).astype(np.float64)
+            vc = cpu.root_cfv(p, mode)
+            err = np.abs(vg - vc).max()
+            _say(f"  p{p} mode{mode}: max |gpu-cpu| {err:.3e}")
+            assert err <= 1e-5 * spot.starting_pot + 1e-5 * np.abs(vc).max()
+    assert abs(gpu.exploitability() - cpu.exploitability()) / spot.starting_pot < 1e-6
+
+
+def test_graph_replay_matches_eager(built):
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: river spot, hipGraph replay vs eager launches, 40 iterations with checks between")
+    spot = Spot(workloads.river_spot(seed=11))
+    a = GpuSolver(spot, use_graphs=True)
+    b = GpuSolver(spot, use_graphs=False)
+    for _ in range(4):
+        a.iterate(10)
+        b.iterate(10)
+        assert a.exploitability() == b.exploitability()
+    ra, sa = a.download_arenas()
+    rb, sb = b.download_arenas()
+    for p in range(2):
+        assert np.array_equal(ra[p], rb[p]) and np.array_equal(sa[p], sb[p])
