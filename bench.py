@@ -32,6 +32,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-target", action="store_true")
+    ap.add_argument("--storage", default="f32", choices=["f32", "compressed"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -47,7 +48,7 @@ def main():
         f"{spot.num_action_nodes} action nodes, hands {spot.num_hands}")
     pot = spot.starting_pot
 
-    gpu = GpuSolver(spot, storage="f32")
+    gpu = GpuSolver(spot, storage=args.storage)
     D = gpu.reached_entries
     gpu.iterate(args.warmup)
     gpu.synchronize()
@@ -74,7 +75,8 @@ def main():
     # algorithmic bytes: regret/strategy arena entries D; the update kernel
     # reads+writes R and S (16 B per entry per iteration, both sweeps), the
     # down sweep reads R once more (4 B): 20*D per iteration in total.
-    alg = {"up_action": 16.0 * D, "down_action": 4.0 * D}
+    eb = 4.0 if args.storage == "f32" else 2.0  # bytes per arena entry
+    alg = {"up_action": 4.0 * eb * D, "down_action": eb * D}
     dom_bytes = alg.get(dom)
     roofline = None
     if dom_bytes is not None:
@@ -83,11 +85,11 @@ def main():
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                     "avg_launch_us": round(kinds[dom][0] * 1e3 / dom_launches, 2),
                     "algorithmic_bytes_per_iter": dom_bytes}
-    whole = 20.0 * D / (ms_per_step * 1e-3) / 1e9
+    whole = 5.0 * eb * D / (ms_per_step * 1e-3) / 1e9
 
     sec_to_target, iters_to_target, e_final = None, None, None
     if not args.no_target:
-        g2 = GpuSolver(spot, storage="f32")
+        g2 = GpuSolver(spot, storage=args.storage)
         t0 = time.perf_counter()
         while g2.iteration < 2000:
             g2.iterate(10)
@@ -131,7 +133,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.storage == "f32" else "i16/u16 arenas, f32 math",
         "data": "synthetic",
         "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
                    "hands": list(spot.num_hands), "arena_entries_D": D, "parallelism": "single GPU"},

@@ -69,8 +69,9 @@ void discounts(int algo, uint32_t it, float* out3) {
 class Engine {
    public:
     Engine(const gto_spot_view& v, const gto_hip_options& o) : opts_(o) {
-        if (o.storage != GTO_STORAGE_F32)
-            throw std::runtime_error("compressed device arenas are not enabled in this build");
+        if (o.storage != GTO_STORAGE_F32 && o.storage != GTO_STORAGE_COMPRESSED)
+            throw std::runtime_error("unknown storage mode");
+        compressed_ = o.storage == GTO_STORAGE_COMPRESSED;
         HIP_OK(hipSetDevice(o.device));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         plan_ = build_plan(v);
@@ -78,6 +79,10 @@ class Engine {
         data_size_[0] = v.data_size[0];
         data_size_[1] = v.data_size[1];
         rake_ = v.rake_pct;
+        node_kind_.assign(v.node_kind, v.node_kind + v.num_nodes);
+        node_player_.assign(v.node_player, v.node_player + v.num_nodes);
+        node_na_.assign(v.node_num_children, v.node_num_children + v.num_nodes);
+        node_off_.assign(v.node_data_offset, v.node_data_offset + v.num_nodes);
         for (int q = 0; q < 2; ++q) {
             weights_[q].assign(v.weights[q], v.weights[q] + v.num_hands[q]);
             c1_[q].assign(v.hand_c1[q], v.hand_c1[q] + v.num_hands[q]);
@@ -163,16 +168,88 @@ class Engine {
 
     void download(int which, int p, float* dst) {
         if (!dst || data_size_[p] == 0) return;
-        HIP_OK(hipMemcpyAsync(dst, which == 0 ? S_.R[p] : S_.S[p], sizeof(float) * data_size_[p],
+        if (!compressed_) {
+            HIP_OK(hipMemcpyAsync(dst, which == 0 ? S_.R[p] : S_.S[p], sizeof(float) * data_size_[p],
+                                  hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            return;
+        }
+        // decode per action node: v = q * scale / 32767 (i16) or / 65535 (u16), store.rs:153-166
+        std::vector<uint16_t> q(data_size_[p]);
+        std::vector<float> sc(num_nodes_);
+        HIP_OK(hipMemcpyAsync(q.data(), which == 0 ? (void*)S_.Rq[p] : (void*)S_.Sq[p],
+                              sizeof(uint16_t) * data_size_[p], hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipMemcpyAsync(sc.data(), which == 0 ? S_.Rs[p] : S_.Ss[p], sizeof(float) * num_nodes_,
                               hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (uint32_t n = 0; n < num_nodes_; ++n) {
+            if (node_kind_[n] != GTO_KIND_ACTION || node_player_[n] != p) continue;
+            size_t cnt = (size_t)node_na_[n] * plan_.nh[p];
+            uint64_t off = node_off_[n];
+            if (which == 0) {
+                float k = sc[n] / 32767.0f;
+                for (size_t e = 0; e < cnt; ++e) dst[off + e] = (float)(int16_t)q[off + e] * k;
+            } else {
+                float k = sc[n] / 65535.0f;
+                for (size_t e = 0; e < cnt; ++e) dst[off + e] = (float)q[off + e] * k;
+            }
+        }
+    }
+    void download_raw(int p, int16_t* rq, uint16_t* sq, float* rs, float* ss) {
+        if (!compressed_) throw std::runtime_error("engine was created with f32 arenas");
+        if (rq) HIP_OK(hipMemcpyAsync(rq, S_.Rq[p], sizeof(int16_t) * data_size_[p], hipMemcpyDeviceToHost, stream_));
+        if (sq) HIP_OK(hipMemcpyAsync(sq, S_.Sq[p], sizeof(uint16_t) * data_size_[p], hipMemcpyDeviceToHost, stream_));
+        if (rs) HIP_OK(hipMemcpyAsync(rs, S_.Rs[p], sizeof(float) * num_nodes_, hipMemcpyDeviceToHost, stream_));
+        if (ss) HIP_OK(hipMemcpyAsync(ss, S_.Ss[p], sizeof(float) * num_nodes_, hipMemcpyDeviceToHost, stream_));
         HIP_OK(hipStreamSynchronize(stream_));
     }
     void upload(int which, int p, const float* src) {
         if (!src || data_size_[p] == 0) return;
-        HIP_OK(hipMemcpyAsync(which == 0 ? S_.R[p] : S_.S[p], src, sizeof(float) * data_size_[p],
+        reach_valid_[0] = reach_valid_[1] = false;
+        if (!compressed_) {
+            HIP_OK(hipMemcpyAsync(which == 0 ? S_.R[p] : S_.S[p], src, sizeof(float) * data_size_[p],
+                                  hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            return;
+        }
+        // encode per action node against its block max (store.rs:199-215)
+        std::vector<uint16_t> q(data_size_[p], 0);
+        std::vector<float> sc(num_nodes_, 0.0f);
+        HIP_OK(hipMemcpyAsync(sc.data(), which == 0 ? S_.Rs[p] : S_.Ss[p], sizeof(float) * num_nodes_,
+                              hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (uint32_t n = 0; n < num_nodes_; ++n) {
+            if (node_kind_[n] != GTO_KIND_ACTION || node_player_[n] != p) continue;
+            size_t cnt = (size_t)node_na_[n] * plan_.nh[p];
+            uint64_t off = node_off_[n];
+            float m = 0.0f;
+            for (size_t e = 0; e < cnt; ++e) m = std::fmax(m, which == 0 ? std::fabs(src[off + e]) : src[off + e]);
+            if (!(m > 0.0f && std::isfinite(m))) {
+                sc[n] = 0.0f;
+                continue;
+            }
+            sc[n] = m;
+            if (which == 0) {
+                float k = 32767.0f / m;
+                for (size_t e = 0; e < cnt; ++e) {
+                    float r = std::nearbyint(src[off + e] * k);
+                    r = r != r ? 0.0f : std::fmin(std::fmax(r, -32768.0f), 32767.0f);
+                    q[off + e] = (uint16_t)(int16_t)r;
+                }
+            } else {
+                float k = 65535.0f / m;
+                for (size_t e = 0; e < cnt; ++e) {
+                    float r = std::nearbyint(std::fmax(src[off + e], 0.0f) * k);
+                    r = r != r ? 0.0f : std::fmin(std::fmax(r, 0.0f), 65535.0f);
+                    q[off + e] = (uint16_t)r;
+                }
+            }
+        }
+        HIP_OK(hipMemcpyAsync(which == 0 ? (void*)S_.Rq[p] : (void*)S_.Sq[p], q.data(),
+                              sizeof(uint16_t) * data_size_[p], hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipMemcpyAsync(which == 0 ? S_.Rs[p] : S_.Ss[p], sc.data(), sizeof(float) * num_nodes_,
                               hipMemcpyHostToDevice, stream_));
         HIP_OK(hipStreamSynchronize(stream_));
-        reach_valid_[0] = reach_valid_[1] = false;
     }
 
     void update_locks(const int64_t* lock_off, const float* sigma, uint64_t n_sigma) {
@@ -223,6 +300,9 @@ class Engine {
 
    private:
     gto_hip_options opts_;
+    bool compressed_ = false;
+    std::vector<uint8_t> node_kind_, node_player_, node_na_;
+    std::vector<uint64_t> node_off_;
     hipStream_t stream_ = nullptr;
     Plan plan_;
     uint32_t num_nodes_ = 0;
@@ -352,10 +432,21 @@ class Engine {
     void alloc_state() {
         for (int q = 0; q < 2; ++q) {
             size_t ds = std::max<uint64_t>(data_size_[q], 1);
-            S_.R[q] = (float*)dalloc(sizeof(float) * ds);
-            S_.S[q] = (float*)dalloc(sizeof(float) * ds);
-            HIP_OK(hipMemset(S_.R[q], 0, sizeof(float) * ds));
-            HIP_OK(hipMemset(S_.S[q], 0, sizeof(float) * ds));
+            if (compressed_) {
+                S_.Rq[q] = (int16_t*)dalloc(sizeof(int16_t) * ds);
+                S_.Sq[q] = (uint16_t*)dalloc(sizeof(uint16_t) * ds);
+                S_.Rs[q] = (float*)dalloc(sizeof(float) * num_nodes_);
+                S_.Ss[q] = (float*)dalloc(sizeof(float) * num_nodes_);
+                HIP_OK(hipMemset(S_.Rq[q], 0, sizeof(int16_t) * ds));
+                HIP_OK(hipMemset(S_.Sq[q], 0, sizeof(uint16_t) * ds));
+                HIP_OK(hipMemset(S_.Rs[q], 0, sizeof(float) * num_nodes_));
+                HIP_OK(hipMemset(S_.Ss[q], 0, sizeof(float) * num_nodes_));
+            } else {
+                S_.R[q] = (float*)dalloc(sizeof(float) * ds);
+                S_.S[q] = (float*)dalloc(sizeof(float) * ds);
+                HIP_OK(hipMemset(S_.R[q], 0, sizeof(float) * ds));
+                HIP_OK(hipMemset(S_.S[q], 0, sizeof(float) * ds));
+            }
             S_.reach[q] = (float*)dalloc(sizeof(float) * (size_t)plan_.num_slots[q] * plan_.nh[q]);
             S_.alive[q] = (uint8_t*)dalloc(plan_.num_slots[q]);
             HIP_OK(hipMemset(S_.alive[q], 0, plan_.num_slots[q]));
@@ -436,10 +527,18 @@ class Engine {
             const DevLevel& D = lv_[l];
             for (int q = 0; q < 2; ++q) {
                 if (!((mask >> q) & 1) || !D.n_act[q]) continue;
-                if (kind == kAverageF32)
-                    launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, D.n_act[q], 0, T_, S_, (const uint32_t*)D.act[q], q);
-                else
-                    launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, D.n_act[q], 0, T_, S_, (const uint32_t*)D.act[q], q);
+                const uint32_t* nodes = D.act[q];
+                if (kind == kAverageF32) {
+                    if (compressed_)
+                        launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, D.n_act[q], 0, T_, S_, nodes, q);
+                    else
+                        launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, D.n_act[q], 0, T_, S_, nodes, q);
+                } else {
+                    if (compressed_)
+                        launch(PK_DOWN_ACTION, k_down_action<kCurrentQ>, D.n_act[q], 0, T_, S_, nodes, q);
+                    else
+                        launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, D.n_act[q], 0, T_, S_, nodes, q);
+                }
             }
             if (D.n_edge)
                 launch(PK_DOWN_CHANCE, k_down_chance, D.n_edge, 0, T_, S_, (const uint32_t*)D.e_parent,
@@ -457,11 +556,20 @@ class Engine {
             for (int q = 0; q < 2; ++q) {
                 if (!D.n_act[q]) continue;
                 const uint32_t* nodes = D.act[q];
-                switch (mode) {
-                    case 0: launch(PK_UP_ACTION, k_up_action<0, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                    case 1: launch(PK_UP_ACTION, k_up_action<1, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                    case 2: launch(PK_UP_ACTION, k_up_action<2, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                    default: launch(PK_UP_ACTION, k_up_action<3, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                if (compressed_) {
+                    switch (mode) {
+                        case 0: launch(PK_UP_ACTION, k_up_action<0, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        case 1: launch(PK_UP_ACTION, k_up_action<1, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        case 2: launch(PK_UP_ACTION, k_up_action<2, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        default: launch(PK_UP_ACTION, k_up_action<3, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                    }
+                } else {
+                    switch (mode) {
+                        case 0: launch(PK_UP_ACTION, k_up_action<0, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        case 1: launch(PK_UP_ACTION, k_up_action<1, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        case 2: launch(PK_UP_ACTION, k_up_action<2, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        default: launch(PK_UP_ACTION, k_up_action<3, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                    }
                 }
             }
         }
@@ -621,9 +729,12 @@ int gto_hip_upload_arenas(gto_hip* h, const float* r0, const float* r1, const fl
     });
 }
 
-int gto_hip_download_compressed(gto_hip*, int, int16_t*, uint16_t*, float*, float*) {
-    set_last_error("engine was created with f32 arenas");
-    return GTO_ERR_INVALID;
+int gto_hip_download_compressed(gto_hip* h, int p, int16_t* rq, uint16_t* sq, float* rs, float* ss) {
+    if (p < 0 || p > 1) {
+        set_last_error("gto_hip_download_compressed: bad player");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] { h->e->download_raw(p, rq, sq, rs, ss); });
 }
 
 int gto_hip_update_locks(gto_hip* h, const int64_t* lock_off, const float* sigma, uint64_t n_sigma) {

@@ -24,6 +24,21 @@ __device__ __forceinline__ void for_actions(int na, F&& f) {
     for (int a = kMaxUnroll; a < na; ++a) f(a, std::false_type{});
 }
 
+// Round half to even, then saturate as Rust's `as i16` / `as u16` do
+// (NaN -> 0), store.rs:171-195.
+__device__ __forceinline__ int16_t quant_i16(float x) {
+    float r = rintf(x);
+    if (!(r == r)) return 0;
+    r = fminf(fmaxf(r, -32768.0f), 32767.0f);
+    return (int16_t)r;
+}
+__device__ __forceinline__ uint16_t quant_u16(float x) {
+    float r = rintf(x);
+    if (!(r == r)) return 0;
+    r = fminf(fmaxf(r, 0.0f), 65535.0f);
+    return (uint16_t)r;
+}
+
 // ---------------------------------------------------------------------------
 // Strategy at one hand of an action node.
 //   CURRENT: regret matching, sigma = max(R,0) * (1/sum), uniform if
@@ -588,6 +603,72 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                     Sm[idx] = Sm[idx] * ds + reach * sh.template sigma<C>(a);
                 });
                 out[i] = acc;
+            }
+        } else {
+            // Compressed arenas (cfr.rs:350-385): decode + discount + update
+            // with a block-wide max, then quantize against the new maxima.
+            // Pass 2 recomputes the same values (deterministically) instead
+            // of staging na*nh floats.
+            __shared__ float s_rmax[kBlock], s_smax[kBlock];
+            int16_t* Rq = S.Rq[p] + off;
+            uint16_t* Sq = S.Sq[p] + off;
+            const float kr = S.Rs[p][n] / 32767.0f;
+            const float kss = S.Ss[p][n] / 65535.0f * ds;
+            float rmax = 0.0f, smax = 0.0f;
+            auto update_vals = [&](uint32_t i, auto&& sink) {
+                SigmaHand<kCurrentQ> sh;
+                sh.init(Rq, nhp, i, na);
+                float v[kMaxUnroll];
+                float acc = 0.0f;
+                for_actions(na, [&](int a, auto c) {
+                    constexpr bool C = decltype(c)::value;
+                    float va = val(a, c, i);
+                    if constexpr (C) v[a] = va;
+                    acc += sh.template sigma<C>(a) * va;
+                });
+                const float reach = rp[i];
+                for_actions(na, [&](int a, auto c) {
+                    constexpr bool C = decltype(c)::value;
+                    const size_t idx = (size_t)a * nhp + i;
+                    const int qv = sh.template raw<C>(a);
+                    float va;
+                    if constexpr (C) va = v[a];
+                    else va = val(a, c, i);
+                    const float d = qv > 0 ? pos : neg;
+                    const float nr = (float)qv * kr * d + (va - acc);
+                    const float ns = (float)Sq[idx] * kss + reach * sh.template sigma<C>(a);
+                    sink(idx, nr, ns);
+                });
+                return acc;
+            };
+            for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
+                out[i] = update_vals(i, [&](size_t, float nr, float ns) {
+                    rmax = fmaxf(rmax, fabsf(nr));
+                    smax = fmaxf(smax, ns);
+                });
+            }
+            s_rmax[threadIdx.x] = rmax;
+            s_smax[threadIdx.x] = smax;
+            __syncthreads();
+            for (int st = kBlock / 2; st > 0; st >>= 1) {
+                if ((int)threadIdx.x < st) {
+                    s_rmax[threadIdx.x] = fmaxf(s_rmax[threadIdx.x], s_rmax[threadIdx.x + st]);
+                    s_smax[threadIdx.x] = fmaxf(s_smax[threadIdx.x], s_smax[threadIdx.x + st]);
+                }
+                __syncthreads();
+            }
+            const float RM = s_rmax[0], SM = s_smax[0];
+            const bool rok = RM > 0.0f && isfinite(RM), sok = SM > 0.0f && isfinite(SM);
+            const float rk = rok ? 32767.0f / RM : 0.0f, sk = sok ? 65535.0f / SM : 0.0f;
+            for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
+                update_vals(i, [&](size_t idx, float nr, float ns) {
+                    Rq[idx] = rok ? quant_i16(nr * rk) : (int16_t)0;
+                    Sq[idx] = sok ? quant_u16(fmaxf(ns, 0.0f) * sk) : (uint16_t)0;
+                });
+            }
+            if (threadIdx.x == 0) {
+                S.Rs[p][n] = rok ? RM : 0.0f;
+                S.Ss[p][n] = sok ? SM : 0.0f;
             }
         }
     }

@@ -121,3 +121,41 @@ def test_graph_replay_matches_eager(built):
     rb, sb = b.download_arenas()
     for p in range(2):
         assert np.array_equal(ra[p], rb[p]) and np.array_equal(sa[p], sb[p])
+
+
+# bench/new_i16_dcfr.log:2-3 of the reference: the CPU compressed (i16/u16)
+# engine on bench_spot.json.
+GOLDEN_I16 = {10: 33.5076, 20: 9.2809}
+
+
+def test_compressed_river_matches_oracle(built):
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: river spot, compressed i16/u16 arenas, 60 iterations")
+    spot = Spot(workloads.river_spot())
+    gpu, cpu = GpuSolver(spot, storage="compressed"), oracle.Oracle(spot, storage="compressed")
+    pot = spot.starting_pot
+    for _ in range(6):
+        gpu.iterate(10)
+        cpu.iterate(10)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {gpu.iteration}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) / pot < POT_TOL
+    rg, sg = gpu.download_arenas()
+    rc, sc = cpu.arenas()
+    _close_arenas(rg, rc, pot, rtol=2e-3)
+    _close_arenas(sg, sc, 1.0, rtol=2e-3)
+
+
+def test_compressed_bench_spot_reproduces_reference_trajectory(built):
+    from gtopen_amd import GpuSolver, Spot
+
+    _say("case: bench_spot.json, compressed arenas, 20 iterations")
+    spot = Spot(open(golden("bench_spot.json")).read())
+    gpu = GpuSolver(spot, storage="compressed")
+    for it in (10, 20):
+        gpu.iterate(it - gpu.iteration)
+        e = gpu.exploitability()
+        _say(f"  iter {it}: gpu {e:.4f} chips, reference {GOLDEN_I16[it]:.4f}")
+        assert abs(e - GOLDEN_I16[it]) / spot.starting_pot < POT_TOL
