@@ -37,19 +37,24 @@ const char* kProfNames[GTO_PROF_KINDS] = {"root", "down_action", "down_chance", 
                                           "up_show", "up_chance", "up_action", "other"};
 
 struct DevLevel {
-    uint32_t* act[2] = {nullptr, nullptr};
+    ActRec* act[2] = {nullptr, nullptr};
     uint32_t n_act[2] = {0, 0};
-    uint32_t* fold = nullptr;
+    FoldRec* fold = nullptr;
     uint32_t n_fold = 0;
-    uint32_t* show = nullptr;
+    ShowRec* show = nullptr;
     uint32_t n_show = 0;
-    uint32_t* chance = nullptr;
+    ChanceRec* chance = nullptr;
     uint32_t n_chance = 0;
-    uint32_t* e_parent = nullptr;
-    uint32_t* e_child = nullptr;
-    uint32_t* e_card = nullptr;
+    EdgeRec* edge = nullptr;
     uint32_t n_edge = 0;
 };
+
+// Workgroups for a list of `count` one-wavefront items, padded to a
+// multiple of the 8 XCDs (see xcd_slice).
+uint32_t wave_grid(uint32_t count, uint32_t waves = kWaves) {
+    uint32_t blocks = (count + waves - 1) / waves;
+    return (blocks + 7u) & ~7u;
+}
 
 // Discounts::for_iteration (cfr.rs:44-63), computed in f64 and stored as f32.
 void discounts(int algo, uint32_t it, float* out3) {
@@ -255,9 +260,6 @@ class Engine {
     void update_locks(const int64_t* lock_off, const float* sigma, uint64_t n_sigma) {
         HIP_OK(hipStreamSynchronize(stream_));
         drop_graph();
-        std::vector<int64_t> lo(num_nodes_, -1);
-        if (lock_off) std::copy(lock_off, lock_off + num_nodes_, lo.begin());
-        HIP_OK(hipMemcpy((void*)T_.lock_off, lo.data(), sizeof(int64_t) * num_nodes_, hipMemcpyHostToDevice));
         if (lock_sigma_buf_) {
             HIP_OK(hipFree(lock_sigma_buf_));
             allocs_.erase(std::find(allocs_.begin(), allocs_.end(), (void*)lock_sigma_buf_));
@@ -268,6 +270,19 @@ class Engine {
             HIP_OK(hipMemcpy(lock_sigma_buf_, sigma, sizeof(float) * n_sigma, hipMemcpyHostToDevice));
         }
         T_.lock_sigma = lock_sigma_buf_;
+        // patch the action records' lock offsets
+        for (size_t l = 0; l < plan_.levels.size(); ++l)
+            for (int q = 0; q < 2; ++q) {
+                std::vector<ActRec>& recs = plan_.levels[l].act_rec[q];
+                if (recs.empty()) continue;
+                for (ActRec& r : recs) {
+                    int64_t lo = lock_off ? lock_off[r.node] : -1;
+                    if (lo >= 0 && (uint64_t)lo + (uint64_t)r.na * plan_.nh[node_player_[r.node]] > n_sigma)
+                        throw std::runtime_error("lock sigma block out of range for node " + std::to_string(r.node));
+                    r.lock_off = lo;
+                }
+                HIP_OK(hipMemcpy(lv_[l].act[q], recs.data(), sizeof(ActRec) * recs.size(), hipMemcpyHostToDevice));
+            }
         reach_valid_[0] = reach_valid_[1] = false;
     }
 
@@ -321,6 +336,9 @@ class Engine {
     float* disc_buf_ = nullptr;
     float* lock_sigma_buf_ = nullptr;
     size_t show_lds_[2] = {0, 0};
+    uint32_t show_wave_[2] = {0, 0};
+    uint32_t show_waves_[2] = {kWaves, kWaves};
+    size_t fold_lds_[2] = {0, 0};
     hipGraphExec_t graph_ = nullptr;
     gto_hip_profile* prof_ = nullptr;
     std::vector<hipEvent_t> evs_;
@@ -347,32 +365,12 @@ class Engine {
     }
 
     void upload_tree(const gto_spot_view& v) {
-        const uint32_t N = v.num_nodes;
-        T_.player = upload_arr(v.node_player, N);
-        T_.num_children = upload_arr(v.node_num_children, N);
-        T_.street = upload_arr(v.node_street, N);
-        T_.children_start = upload_arr(v.node_children_start, N);
-        T_.data_offset = upload_arr(v.node_data_offset, N);
-        std::vector<float> tw(N), tl(N), tt(N);
-        for (uint32_t n = 0; n < N; ++n) {
-            tw[n] = (float)v.node_t_win[n];
-            tl[n] = (float)v.node_t_lose[n];
-            tt[n] = (float)v.node_t_tie[n];
-        }
-        T_.t_win = upload_vec(tw);
-        T_.t_lose = upload_vec(tl);
-        T_.t_tie = upload_vec(tt);
-        T_.children = upload_arr(v.children, v.num_children_total);
-        for (int q = 0; q < 2; ++q) T_.owner[q] = upload_vec(plan_.owner[q]);
-        T_.cfv_slot = upload_vec(plan_.cfv_slot);
-        T_.ch_off = upload_vec(plan_.ch_off);
-        T_.ch_cnt = upload_vec(plan_.ch_cnt);
-        T_.ch_child = upload_vec(plan_.ch_child);
-        T_.ch_card = upload_vec(plan_.ch_card);
-        T_.show_slot = upload_vec(plan_.show_slot);
-        std::vector<int64_t> lo(N, -1);
-        T_.lock_off = upload_vec(lo);
+        if (plan_.max_actions > (uint32_t)kMaxActions)
+            throw std::runtime_error("a node has more than 64 actions; the GPU engine supports at most 64");
         T_.lock_sigma = nullptr;
+        T_.kids = upload_vec(plan_.kids);
+        T_.chr_card = upload_vec(plan_.chr_card);
+        T_.chr_cfv = upload_vec(plan_.chr_cfv);
         T_.nh_max = plan_.nh_max;
         for (int q = 0; q < 2; ++q) {
             const uint32_t nh = v.num_hands[q];
@@ -383,49 +381,59 @@ class Engine {
             for (uint32_t i = 0; i < nh; ++i) m[i] = (1ull << v.hand_c1[q][i]) | (1ull << v.hand_c2[q][i]);
             T_.hmask[q] = upload_vec(m);
             T_.same[q] = upload_arr(v.same_combo[q], nh);
+            std::vector<uint32_t> hp(nh);
+            for (uint32_t i = 0; i < nh; ++i) {
+                uint32_t sm = v.same_combo[q][i];
+                if (sm != 0xFFFFFFFFu && sm >= 0xFFFFu) throw std::runtime_error("too many hands for the packed hand table");
+                hp[i] = v.hand_c1[q][i] | (v.hand_c2[q][i] << 8) | ((sm == 0xFFFFFFFFu ? 0xFFFFu : sm) << 16);
+            }
+            T_.hpack[q] = upload_vec(hp);
             T_.weights[q] = upload_arr(v.weights[q], nh);
             T_.hc_off[q] = upload_vec(plan_.hc_off[q]);
             T_.hc_idx[q] = upload_vec(plan_.hc_idx[q]);
             T_.sd_off[q] = upload_vec(plan_.sd_off[q]);
             T_.sd_cnt[q] = upload_vec(plan_.sd_cnt[q]);
             T_.sd_idx[q] = upload_vec(plan_.sd_idx[q]);
-            T_.sd_lbub[q] = upload_vec(plan_.sd_lbub[q]);
-            T_.sd_cnt4[q] = upload_vec(plan_.sd_cnt4[q]);
             T_.sd_coff[q] = upload_vec(plan_.sd_coff[q]);
             T_.sd_cpos_base[q] = upload_vec(plan_.sd_cpos_base[q]);
             T_.sd_cpos[q] = upload_vec(plan_.sd_cpos[q]);
+            T_.sd_tab[q] = (const uint4*)upload_vec(plan_.sd_tab[q]);
         }
-        // showdown LDS for traverser p depends on the opponent's largest board list
+        // LDS: showdown, per wavefront, for traverser p (opponent list length)
         for (int p = 0; p < 2; ++p) {
             size_t mo = plan_.max_board_cnt[1 - p];
-            show_lds_[p] = 16 * (mo + 1) + 16 * (2 * mo + 52) + 4 * mo + 16;
+            show_wave_[p] = (uint32_t)(((28 * mo + 424 + 16) + 15) / 16 * 16);
+            show_waves_[p] = (uint32_t)std::min<size_t>(kWaves, (160 * 1024) / show_wave_[p]);
+            if (show_waves_[p] == 0)
+                throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
+            show_lds_[p] = (size_t)show_wave_[p] * show_waves_[p];
+            size_t nho = plan_.nh[1 - p];
+            size_t per_wave = ((nho * 4 + 52 * 8 + 15) / 16) * 16;
+            fold_lds_[p] = per_wave * kWaves;
         }
-        size_t lds_max = std::max(show_lds_[0], show_lds_[1]);
-        if (lds_max > 160 * 1024 - 8 * 1024)
+        size_t lds_max = std::max(std::max(show_lds_[0], show_lds_[1]), std::max(fold_lds_[0], fold_lds_[1]));
+        if (lds_max > 160 * 1024)
             throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
         HIP_OK(hipFuncSetAttribute((const void*)k_up_show, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)std::max<size_t>(lds_max, 1024)));
-        // level lists
+                                   (int)std::max(show_lds_[0], show_lds_[1])));
+        HIP_OK(hipFuncSetAttribute((const void*)k_up_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)std::max(fold_lds_[0], fold_lds_[1])));
         lv_.resize(plan_.levels.size());
         for (size_t l = 0; l < plan_.levels.size(); ++l) {
             const LevelLists& L = plan_.levels[l];
             DevLevel& D = lv_[l];
             for (int q = 0; q < 2; ++q) {
-                D.n_act[q] = (uint32_t)L.act[q].size();
-                if (D.n_act[q]) D.act[q] = upload_vec(L.act[q]);
+                D.n_act[q] = (uint32_t)L.act_rec[q].size();
+                if (D.n_act[q]) D.act[q] = upload_vec(L.act_rec[q]);
             }
-            D.n_fold = (uint32_t)L.fold.size();
-            if (D.n_fold) D.fold = upload_vec(L.fold);
-            D.n_show = (uint32_t)L.show.size();
-            if (D.n_show) D.show = upload_vec(L.show);
-            D.n_chance = (uint32_t)L.chance.size();
-            if (D.n_chance) D.chance = upload_vec(L.chance);
-            D.n_edge = (uint32_t)L.edge_parent.size();
-            if (D.n_edge) {
-                D.e_parent = upload_vec(L.edge_parent);
-                D.e_child = upload_vec(L.edge_child);
-                D.e_card = upload_vec(L.edge_card);
-            }
+            D.n_fold = (uint32_t)L.fold_rec.size();
+            if (D.n_fold) D.fold = upload_vec(L.fold_rec);
+            D.n_show = (uint32_t)L.show_rec.size();
+            if (D.n_show) D.show = upload_vec(L.show_rec);
+            D.n_chance = (uint32_t)L.chance_rec.size();
+            if (D.n_chance) D.chance = upload_vec(L.chance_rec);
+            D.n_edge = (uint32_t)L.edge_rec.size();
+            if (D.n_edge) D.edge = upload_vec(L.edge_rec);
         }
     }
 
@@ -488,13 +496,17 @@ class Engine {
     // --- launch helpers with optional per-kind event timing ---------------
     template <class K, class... Args>
     void launch(int kind, K kernel, uint32_t grid, size_t lds, Args... args) {
+        launch_b(kind, kernel, grid, kBlock, lds, args...);
+    }
+    template <class K, class... Args>
+    void launch_b(int kind, K kernel, uint32_t grid, uint32_t block, size_t lds, Args... args) {
         if (grid == 0) return;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof_) {
             e0 = event();
             HIP_OK(hipEventRecord(e0, stream_));
         }
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, stream_, args...);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream_, args...);
         HIP_OK(hipGetLastError());
         if (prof_) {
             e1 = event();
@@ -527,22 +539,19 @@ class Engine {
             const DevLevel& D = lv_[l];
             for (int q = 0; q < 2; ++q) {
                 if (!((mask >> q) & 1) || !D.n_act[q]) continue;
-                const uint32_t* nodes = D.act[q];
+                const ActRec* recs = D.act[q];
+                const uint32_t n = D.n_act[q], g = wave_grid(n);
                 if (kind == kAverageF32) {
-                    if (compressed_)
-                        launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, D.n_act[q], 0, T_, S_, nodes, q);
-                    else
-                        launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, D.n_act[q], 0, T_, S_, nodes, q);
+                    if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, g, 0, T_, S_, recs, n, q);
+                    else launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, g, 0, T_, S_, recs, n, q);
                 } else {
-                    if (compressed_)
-                        launch(PK_DOWN_ACTION, k_down_action<kCurrentQ>, D.n_act[q], 0, T_, S_, nodes, q);
-                    else
-                        launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, D.n_act[q], 0, T_, S_, nodes, q);
+                    if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kCurrentQ>, g, 0, T_, S_, recs, n, q);
+                    else launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, g, 0, T_, S_, recs, n, q);
                 }
             }
             if (D.n_edge)
-                launch(PK_DOWN_CHANCE, k_down_chance, D.n_edge, 0, T_, S_, (const uint32_t*)D.e_parent,
-                       (const uint32_t*)D.e_child, (const uint32_t*)D.e_card, mask);
+                launch(PK_DOWN_CHANCE, k_down_chance, wave_grid(D.n_edge), 0, T_, S_, (const EdgeRec*)D.edge,
+                       D.n_edge, mask);
         }
     }
 
@@ -550,25 +559,32 @@ class Engine {
     void launch_up(int p, int mode) {
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
-            launch(PK_UP_FOLD, k_up_fold, D.n_fold, 0, T_, S_, (const uint32_t*)D.fold, p);
-            launch(PK_UP_SHOW, k_up_show, D.n_show, show_lds_[p], T_, S_, (const uint32_t*)D.show, p);
-            launch(PK_UP_CHANCE, k_up_chance, D.n_chance, 0, T_, S_, (const uint32_t*)D.chance, p);
+            if (D.n_fold)
+                launch(PK_UP_FOLD, k_up_fold, wave_grid(D.n_fold), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
+                       D.n_fold, p);
+            if (D.n_show)
+                launch_b(PK_UP_SHOW, k_up_show, wave_grid(D.n_show, show_waves_[p]), 64 * show_waves_[p],
+                         show_lds_[p], T_, S_, (const ShowRec*)D.show, D.n_show, p, show_wave_[p]);
+            if (D.n_chance)
+                launch(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), 0, T_, S_, (const ChanceRec*)D.chance,
+                       D.n_chance, p);
             for (int q = 0; q < 2; ++q) {
                 if (!D.n_act[q]) continue;
-                const uint32_t* nodes = D.act[q];
+                const ActRec* recs = D.act[q];
+                const uint32_t n = D.n_act[q], g = wave_grid(n);
                 if (compressed_) {
                     switch (mode) {
-                        case 0: launch(PK_UP_ACTION, k_up_action<0, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                        case 1: launch(PK_UP_ACTION, k_up_action<1, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                        case 2: launch(PK_UP_ACTION, k_up_action<2, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                        default: launch(PK_UP_ACTION, k_up_action<3, true>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        case 0: launch(PK_UP_ACTION, k_up_action<0, true>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 1: launch(PK_UP_ACTION, k_up_action<1, true>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 2: launch(PK_UP_ACTION, k_up_action<2, true>, g, 0, T_, S_, recs, n, p, q); break;
+                        default: launch(PK_UP_ACTION, k_up_action<3, true>, g, 0, T_, S_, recs, n, p, q); break;
                     }
                 } else {
                     switch (mode) {
-                        case 0: launch(PK_UP_ACTION, k_up_action<0, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                        case 1: launch(PK_UP_ACTION, k_up_action<1, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                        case 2: launch(PK_UP_ACTION, k_up_action<2, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
-                        default: launch(PK_UP_ACTION, k_up_action<3, false>, D.n_act[q], 0, T_, S_, nodes, p, q); break;
+                        case 0: launch(PK_UP_ACTION, k_up_action<0, false>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 1: launch(PK_UP_ACTION, k_up_action<1, false>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 2: launch(PK_UP_ACTION, k_up_action<2, false>, g, 0, T_, S_, recs, n, p, q); break;
+                        default: launch(PK_UP_ACTION, k_up_action<3, false>, g, 0, T_, S_, recs, n, p, q); break;
                     }
                 }
             }

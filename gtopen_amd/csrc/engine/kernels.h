@@ -4,51 +4,40 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "records.h"
+
 namespace gto {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wavefronts of 64)
+constexpr int kWave = 64;         // CDNA wavefront
+constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxUnroll = 8;     // actions cached in registers per hand
-constexpr int kMaxActions = 256;  // tree builder caps action counts at 250
+constexpr int kMaxActions = 64;   // engine limit on actions per node (one ballot bit each)
 
-// Static tree, hand and table data (read-only for the whole solve).
+// Static hand and table data (read-only for the whole solve).
 struct DevTree {
-    const uint8_t* player;
-    const uint8_t* num_children;
-    const uint8_t* street;
-    const uint32_t* children_start;
-    const uint64_t* data_offset;
-    const float* t_win;
-    const float* t_lose;
-    const float* t_tie;
-    const uint32_t* children;
-    const uint32_t* owner[2];
-    const uint32_t* cfv_slot;
-    const uint32_t* ch_off;
-    const uint32_t* ch_cnt;
-    const uint32_t* ch_child;
-    const uint8_t* ch_card;
-    const uint32_t* show_slot;
-    const int64_t* lock_off;
     const float* lock_sigma;
-    // hands
+    const uint32_t* kids;      // ActRec children: value rows, then reach slots
+    const uint8_t* chr_card;   // ChanceRec children: dealt card
+    const uint32_t* chr_cfv;   // ChanceRec children: value row
     uint32_t nh[2];
     uint32_t nh_max;
     const uint8_t* c1[2];
     const uint8_t* c2[2];
     const uint64_t* hmask[2];
     const uint32_t* same[2];
+    const uint32_t* hpack[2];    // c1 | c2 << 8 | identical opponent combo << 16 (0xFFFF: none)
     const float* weights[2];
-    const uint32_t* hc_off[2];
+    const uint32_t* hc_off[2];   // per-card hand lists (53 offsets)
     const uint16_t* hc_idx[2];
-    // showdown tables
-    const uint32_t* sd_off[2];
-    const uint32_t* sd_cnt[2];
-    const uint16_t* sd_idx[2];
-    const uint32_t* sd_lbub[2];
-    const uint32_t* sd_cnt4[2];
-    const uint32_t* sd_coff[2];
-    const uint32_t* sd_cpos_base[2];
-    const uint16_t* sd_cpos[2];
+    // showdown tables, per river-board slot
+    const uint32_t* sd_off[2];        // start of q's sorted list
+    const uint32_t* sd_cnt[2];        // its length
+    const uint16_t* sd_idx[2];        // hand index, ascending strength
+    const uint32_t* sd_coff[2];       // 53 per board: per-card list offsets
+    const uint32_t* sd_cpos_base[2];  // per board: base of the per-card lists
+    const uint16_t* sd_cpos[2];       // sorted positions holding card c
+    const uint4* sd_tab[2];           // [board][hand]: packed rank/segment table
 };
 
 // Mutable solver state.

@@ -1,8 +1,12 @@
 // Level-synchronous DCFR sweep kernels for CDNA4 (gfx950).
 //
-// One workgroup of 256 threads per tree node of a depth level; threads
-// stride over the hands of the player whose vector the node produces, so
-// every arena, reach and value access is a contiguous 256-wide run.
+// One wavefront (64 lanes) per tree node of a depth level, four nodes per
+// workgroup; lanes stride over the hands of the player whose vector the
+// node produces, so arena, reach and value accesses are contiguous 64-wide
+// runs, and reductions are wavefront-level (ballots, shuffles) with no
+// workgroup barriers. Each wavefront finds its node's rows through one
+// work record (records.h). Workgroups are mapped to XCD-contiguous slices
+// of the work list.
 // Arithmetic mirrors the CPU engine (crates/solver/src/cfr.rs,
 // best_response.rs, game.rs) operation for operation in f32, with f64
 // accumulation where the CPU accumulates in f64 (fold and showdown sums,
@@ -16,13 +20,24 @@ namespace gto {
 
 // Visit actions 0..na-1; the first kMaxUnroll visits get a compile-time
 // action index (cached values live in registers), the rest re-read memory.
+template <int A, class F>
+__device__ __forceinline__ void for_cached(int na, F& f) {
+    if constexpr (A < kMaxUnroll) {
+        if (A < na) {
+            f(A, std::true_type{});
+            for_cached<A + 1>(na, f);
+        }
+    }
+}
 template <class F>
 __device__ __forceinline__ void for_actions(int na, F&& f) {
-#pragma unroll
-    for (int a = 0; a < kMaxUnroll; ++a)
-        if (a < na) f(a, std::true_type{});
+    for_cached<0>(na, f);
     for (int a = kMaxUnroll; a < na; ++a) f(a, std::false_type{});
 }
+
+// Element q (0 or 1) of a two-entry record field without dynamic indexing
+// (which would place the record in scratch memory).
+__device__ __forceinline__ uint32_t pick(const uint32_t (&v)[2], int q) { return q ? v[1] : v[0]; }
 
 // Round half to even, then saturate as Rust's `as i16` / `as u16` do
 // (NaN -> 0), store.rs:171-195.
@@ -180,7 +195,184 @@ struct SigmaHand<kAverageQ> {
 };
 
 // ---------------------------------------------------------------------------
-// Root: reach = range weights (gpu/mod.rs:640 in the reference's terms).
+// Strategy of one hand at a node with a compile-time action count NA: all
+// NA arena values are loaded before any arithmetic (no per-action
+// branches), then the same regret-matching / averaging arithmetic as
+// SigmaHand, with selects instead of branches.
+// ---------------------------------------------------------------------------
+template <int NA, int KIND>
+struct HandSigma;
+
+// fetch(): issue the NA arena loads; finish(): strategy from them; load():
+// both. Splitting lets a caller issue loads for several hands first.
+template <int NA>
+struct HandSigma<NA, kCurrentF32> {
+    float raw[NA], sig[NA];
+    __device__ __forceinline__ void fetch(const float* __restrict__ b, uint32_t nh, uint32_t i) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) raw[a] = b[(size_t)a * nh + i];
+    }
+    __device__ __forceinline__ void finish() {
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sum += fmaxf(raw[a], 0.0f);
+        const bool uni = !(sum > 1e-12f);
+        const float inv = 1.0f / sum;
+        const float u = 1.0f / (float)NA;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sig[a] = uni ? u : fmaxf(raw[a], 0.0f) * inv;
+    }
+    __device__ __forceinline__ void load(const float* __restrict__ b, uint32_t nh, uint32_t i) {
+        fetch(b, nh, i);
+        finish();
+    }
+};
+
+template <int NA>
+struct HandSigma<NA, kAverageF32> {
+    float raw[NA], sig[NA];
+    __device__ __forceinline__ void fetch(const float* __restrict__ b, uint32_t nh, uint32_t i) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) raw[a] = b[(size_t)a * nh + i];
+    }
+    __device__ __forceinline__ void finish() {
+        double sum = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sum += (double)fmaxf(raw[a], 0.0f);
+        const bool uni = !(sum > 1e-12);
+        const float u = 1.0f / (float)NA;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sig[a] = uni ? u : (float)((double)fmaxf(raw[a], 0.0f) / sum);
+    }
+    __device__ __forceinline__ void load(const float* __restrict__ b, uint32_t nh, uint32_t i) {
+        fetch(b, nh, i);
+        finish();
+    }
+};
+
+template <int NA>
+struct HandSigma<NA, kCurrentQ> {
+    int raw[NA];
+    float sig[NA];
+    __device__ __forceinline__ void fetch(const int16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) raw[a] = b[(size_t)a * nh + i];
+    }
+    __device__ __forceinline__ void finish() {
+        int sum = 0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sum += max(raw[a], 0);
+        const bool uni = !(sum > 0);
+        const float inv = 1.0f / (float)sum;
+        const float u = 1.0f / (float)NA;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sig[a] = uni ? u : (float)max(raw[a], 0) * inv;
+    }
+    __device__ __forceinline__ void load(const int16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+        fetch(b, nh, i);
+        finish();
+    }
+};
+
+template <int NA>
+struct HandSigma<NA, kAverageQ> {
+    uint32_t raw[NA];
+    float sig[NA];
+    __device__ __forceinline__ void fetch(const uint16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) raw[a] = b[(size_t)a * nh + i];
+    }
+    __device__ __forceinline__ void finish() {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sum += raw[a];
+        const bool uni = sum == 0;
+        const float inv = 1.0f / (float)sum;
+        const float u = 1.0f / (float)NA;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sig[a] = uni ? u : (float)raw[a] * inv;
+    }
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+        fetch(b, nh, i);
+        finish();
+    }
+};
+
+// Calls f(std::integral_constant<int, NA>) for na in 1..8, f(0) otherwise
+// (runtime action count).
+template <class F>
+__device__ __forceinline__ void dispatch_na(int na, F&& f) {
+    switch (na) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 5: f(std::integral_constant<int, 5>{}); break;
+        case 6: f(std::integral_constant<int, 6>{}); break;
+        case 7: f(std::integral_constant<int, 7>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: f(std::integral_constant<int, 0>{}); break;
+    }
+}
+
+template <int KIND>
+struct ArenaOf;
+template <>
+struct ArenaOf<kCurrentF32> {
+    static __device__ const float* get(const DevState& S, int q) { return S.R[q]; }
+};
+template <>
+struct ArenaOf<kAverageF32> {
+    static __device__ const float* get(const DevState& S, int q) { return S.S[q]; }
+};
+template <>
+struct ArenaOf<kCurrentQ> {
+    static __device__ const int16_t* get(const DevState& S, int q) { return S.Rq[q]; }
+};
+template <>
+struct ArenaOf<kAverageQ> {
+    static __device__ const uint16_t* get(const DevState& S, int q) { return S.Sq[q]; }
+};
+
+// ---------------------------------------------------------------------------
+// Launch-shape helpers.
+// ---------------------------------------------------------------------------
+// The dispatcher hands workgroup b to XCD b % 8; the host pads the grid to a
+// multiple of 8 and this gives each XCD a contiguous slice of the list, so
+// nodes sharing a river board's tables share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_slice(uint32_t b, uint32_t grid) {
+    const uint32_t per = grid >> 3;
+    return (b & 7u) * per + (b >> 3);
+}
+
+__device__ __forceinline__ uint32_t wave_item() {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    return xcd_slice(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;
+}
+
+// Hands per lane whose loads are issued together in the hand loops (a
+// load's first use otherwise waits for it before the next one is issued).
+constexpr int kBatch = 4;
+
+// Orders this wavefront's LDS writes before its later LDS reads.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ float wave_max_f32(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Root: reach = range weights.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_root(DevTree T, DevState S, int mask) {
     for (int q = 0; q < 2; ++q) {
@@ -207,276 +399,389 @@ __global__ void k_begin_iteration(DevState S) {
 // the zero-reach pruning test of cfr.rs:424-430 for the subtree below.
 // ---------------------------------------------------------------------------
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, const uint32_t* nodes, int q) {
-    const uint32_t n = nodes[blockIdx.x];
+__global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
+                                                        uint32_t count, int q) {
+    const uint32_t w = wave_item();
+    if (w >= count) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const ActRec r = recs[w];
     const uint32_t nh = T.nh[q];
-    const int na = T.num_children[n];
-    const uint32_t cs = T.children_start[n];
-    const uint32_t src = T.owner[q][n];
-    __shared__ int s_any[kMaxActions];
-    for (int a = threadIdx.x; a < na; a += kBlock) s_any[a] = 0;
-    __syncthreads();
-    float* reach = S.reach[q];
+    const int na = r.na;
+    const uint32_t* kid_own = T.kids + r.kid + na;
+    float* __restrict__ reach = S.reach[q];
+    const uint32_t src = pick(r.own, q);
     if (!S.alive[q][src]) {
         for (int a = 0; a < na; ++a) {
-            float* dst = reach + (size_t)T.owner[q][T.children[cs + a]] * nh;
-            for (uint32_t i = threadIdx.x; i < nh; i += kBlock) dst[i] = 0.0f;
+            float* dst = reach + (size_t)kid_own[a] * nh;
+            for (uint32_t i = lane; i < nh; i += kWave) dst[i] = 0.0f;
         }
-        for (int a = threadIdx.x; a < na; a += kBlock) S.alive[q][T.owner[q][T.children[cs + a]]] = 0;
+        for (int a = lane; a < na; a += kWave) S.alive[q][kid_own[a]] = 0;
         return;
     }
-    const float* pr = reach + (size_t)src * nh;
-    const int64_t loff = T.lock_off[n];
-    const uint64_t off = T.data_offset[n];
-    float* dst_c[kMaxUnroll];
-    for_actions(na, [&](int a, auto cached) {
-        if constexpr (decltype(cached)::value) dst_c[a] = reach + (size_t)T.owner[q][T.children[cs + a]] * nh;
-    });
-    auto dst = [&](int a, auto cached) -> float* {
-        if constexpr (decltype(cached)::value) return dst_c[a];
-        else return reach + (size_t)T.owner[q][T.children[cs + a]] * nh;
-    };
-    for (uint32_t i = threadIdx.x; i < nh; i += kBlock) {
-        const float p = pr[i];
-        if (loff >= 0) {
-            const float* sig = T.lock_sigma + loff;
-            for_actions(na, [&](int a, auto cached) {
-                float v = p * sig[(size_t)a * nh + i];
-                dst(a, cached)[i] = v;
-                if (v > 0.0f) s_any[a] = 1;
-            });
-            continue;
+    const float* __restrict__ pr = reach + (size_t)src * nh;
+    uint64_t anym = 0;
+    if (r.lock_off >= 0) {
+        const float* sig = T.lock_sigma + r.lock_off;
+        for (uint32_t i = lane; i < nh; i += kWave) {
+            const float pv = pr[i];
+            for (int a = 0; a < na; ++a) {
+                float v = pv * sig[(size_t)a * nh + i];
+                reach[(size_t)kid_own[a] * nh + i] = v;
+                anym |= (uint64_t)(v > 0.0f) << a;
+            }
         }
-        SigmaHand<KIND> sh;
-        if constexpr (KIND == kCurrentF32) sh.init(S.R[q] + off, nh, i, na);
-        else if constexpr (KIND == kAverageF32) sh.init(S.S[q] + off, nh, i, na);
-        else if constexpr (KIND == kCurrentQ) sh.init(S.Rq[q] + off, nh, i, na);
-        else sh.init(S.Sq[q] + off, nh, i, na);
-        for_actions(na, [&](int a, auto cached) {
-            float v = p * sh.template sigma<decltype(cached)::value>(a);
-            dst(a, cached)[i] = v;
-            if (v > 0.0f) s_any[a] = 1;
+    } else {
+        const auto* arena = ArenaOf<KIND>::get(S, q) + r.doff;
+        dispatch_na(na, [&](auto NAc) {
+            constexpr int NA = decltype(NAc)::value;
+            if constexpr (NA > 0) {
+                float* dst[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) dst[a] = reach + (size_t)kid_own[a] * nh;
+                for (uint32_t i0 = lane; i0 < nh; i0 += 2 * kWave) {
+                    HandSigma<NA, KIND> hs[2];
+                    float pv[2];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const uint32_t ii = min(i0 + h * kWave, nh - 1);
+                        hs[h].fetch(arena, nh, ii);
+                        pv[h] = pr[ii];
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const uint32_t i = i0 + h * kWave;
+                        if (i >= nh) break;
+                        hs[h].finish();
+#pragma unroll
+                        for (int a = 0; a < NA; ++a) {
+                            float v = pv[h] * hs[h].sig[a];
+                            dst[a][i] = v;
+                            anym |= (uint64_t)(v > 0.0f) << a;
+                        }
+                    }
+                }
+            } else {
+                for (uint32_t i = lane; i < nh; i += kWave) {
+                    SigmaHand<KIND> sh;
+                    sh.init(arena, nh, i, na);
+                    const float pv = pr[i];
+                    for (int a = 0; a < na; ++a) {
+                        float v = pv * sh.template sigma<false>(a);
+                        reach[(size_t)kid_own[a] * nh + i] = v;
+                        anym |= (uint64_t)(v > 0.0f) << a;
+                    }
+                }
+            }
         });
     }
-    __syncthreads();
-    for (int a = threadIdx.x; a < na; a += kBlock) S.alive[q][T.owner[q][T.children[cs + a]]] = s_any[a] ? 1 : 0;
+    for (int a = 0; a < na; ++a) {
+        const bool any = __ballot((anym >> a) & 1ull) != 0ull;
+        if (lane == 0) S.alive[q][kid_own[a]] = any ? 1 : 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Down sweep, chance edges: child reach = parent reach with the hands that
 // hold the dealt card zeroed (cfr.rs:615-629), for the players in `mask`.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, const uint32_t* parents,
-                                                        const uint32_t* childs, const uint32_t* cards, int mask) {
-    const uint32_t e = blockIdx.x;
-    const uint32_t pn = parents[e], cn = childs[e];
-    const uint64_t cm = 1ull << cards[e];
+__global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, const EdgeRec* __restrict__ recs,
+                                                        uint32_t count, int mask) {
+    const uint32_t w = wave_item();
+    if (w >= count) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const EdgeRec r = recs[w];
+    const uint64_t cm = 1ull << r.card;
     for (int q = 0; q < 2; ++q) {
         if (!((mask >> q) & 1)) continue;
         const uint32_t nh = T.nh[q];
-        const uint32_t src = T.owner[q][pn], dst = T.owner[q][cn];
-        float* out = S.reach[q] + (size_t)dst * nh;
-        const bool alive = S.alive[q][src];
-        const float* in = S.reach[q] + (size_t)src * nh;
+        float* out = S.reach[q] + (size_t)pick(r.dst, q) * nh;
+        const bool alive = S.alive[q][pick(r.src, q)];
+        const float* in = S.reach[q] + (size_t)pick(r.src, q) * nh;
         bool any = false;
-        for (uint32_t i = threadIdx.x; i < nh; i += kBlock) {
-            float v = (!alive || (T.hmask[q][i] & cm)) ? 0.0f : in[i];
-            out[i] = v;
-            any |= v > 0.0f;
+        for (uint32_t i0 = lane; i0 < nh; i0 += kBatch * kWave) {
+            float v[kBatch];
+            uint64_t m[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t ii = min(i0 + u * kWave, nh - 1);
+                v[u] = in[ii];
+                m[u] = T.hmask[q][ii];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t i = i0 + u * kWave;
+                if (i >= nh) break;
+                const float x = (!alive || (m[u] & cm)) ? 0.0f : v[u];
+                out[i] = x;
+                any |= x > 0.0f;
+            }
         }
-        any = __syncthreads_or(any);
-        if (threadIdx.x == 0) S.alive[q][dst] = any ? 1 : 0;
+        any = __ballot(any) != 0ull;
+        if (lane == 0) S.alive[q][pick(r.dst, q)] = any ? 1 : 0;
     }
-}
-
-// ---------------------------------------------------------------------------
-// Block-wide f64 reduction helper (deterministic tree order).
-// ---------------------------------------------------------------------------
-__device__ inline double block_sum_f64(double v, double* sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    double r = sh[0];
-    __syncthreads();
-    return r;
 }
 
 // ---------------------------------------------------------------------------
 // Up sweep, fold terminals (game.rs:470-496): value = amount * (opponent
-// reach compatible with the hand), with f64 totals. Per-card sums run
-// serially in hand order, as the CPU accumulates them.
+// reach compatible with the hand), with f64 totals. The opponent's row is
+// staged in LDS; per-card sums run serially in hand order (the CPU's
+// order), one lane per card, over the card's hand list fetched into
+// registers first. LDS per wavefront: nh_o floats + 52 doubles.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const uint32_t* nodes, int p) {
+constexpr int kMaxHandList = 51;  // hands holding one card: at most 51
+
+__global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
+                                                    uint32_t count, int p) {
+    extern __shared__ double lds[];
     const int o = 1 - p;
-    const uint32_t n = nodes[blockIdx.x];
-    const uint32_t nhp = T.nh[p], nho = T.nh[o];
-    float* out = S.cfv + (size_t)T.cfv_slot[n] * T.nh_max;
-    const uint32_t osrc = T.owner[o][n];
-    if (!S.alive[o][osrc]) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) out[i] = 0.0f;
+    const uint32_t nho = T.nh[o], nhp = T.nh[p];
+    const uint32_t w = wave_item();
+    if (w >= count) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const FoldRec r = recs[w];
+    float* out = S.cfv + (size_t)r.cfv * T.nh_max;
+    const uint32_t oslot = pick(r.own, o);
+    if (!S.alive[o][oslot]) {
+        for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
         return;
     }
-    const float* ro = S.reach[o] + (size_t)osrc * nho;
-    __shared__ double s_card[52];
-    __shared__ double s_red[kBlock];
-    if (threadIdx.x < 52) {
-        const uint32_t c = threadIdx.x;
-        double acc = 0.0;
-        for (uint32_t k = T.hc_off[o][c]; k < T.hc_off[o][c + 1]; ++k) acc += (double)ro[T.hc_idx[o][k]];
-        s_card[c] = acc;
+    const size_t per_wave = ((nho * 4 + 52 * 8 + 15) / 16) * 16;
+    char* base = (char*)lds + wave * per_wave;
+    double* s_card = (double*)base;
+    float* rs = (float*)(base + 52 * 8);
+    const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
+    // (unconditional loads at clamped indices: no per-entry branches)
+    uint32_t len = 0, k0 = 0;
+    uint32_t idx[kMaxHandList];
+    if (lane < 52) {
+        k0 = T.hc_off[o][lane];
+        len = T.hc_off[o][lane + 1] - k0;
     }
+    const uint32_t last = len ? len - 1 : 0;
+    const uint16_t* hsrc = T.hc_idx[o] + (len ? k0 : 0);
+#pragma unroll
+    for (int t = 0; t < kMaxHandList; ++t) idx[t] = hsrc[min((uint32_t)t, last)];
     double part = 0.0;
-    for (uint32_t j = threadIdx.x; j < nho; j += kBlock) part += (double)ro[j];
-    const double tot = block_sum_f64(part, s_red);  // includes the barrier that publishes s_card
-    const float amount = T.player[n] == p ? T.t_lose[n] : T.t_win[n];
-    for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-        uint32_t sm = T.same[p][i];
-        float same_r = sm != 0xFFFFFFFFu ? ro[sm] : 0.0f;
-        float valid = (float)(tot - s_card[T.c1[p][i]] - s_card[T.c2[p][i]]) + same_r;
-        out[i] = amount * valid;
+    for (uint32_t j0 = lane; j0 < nho; j0 += kBatch * kWave) {
+        float v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) v[u] = ro[min(j0 + u * kWave, nho - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (j0 + u * kWave < nho) {
+                rs[j0 + u * kWave] = v[u];
+                part += (double)v[u];
+            }
+    }
+    const double tot = wave_sum_f64(part);
+    wave_sync();
+    if (lane < 52) {
+        float v[kMaxHandList];
+#pragma unroll
+        for (int t = 0; t < kMaxHandList; ++t) v[t] = rs[idx[t]];
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < kMaxHandList; ++t) acc += t < (int)len ? (double)v[t] : 0.0;
+        s_card[lane] = acc;
+    }
+    wave_sync();
+    const float amount = r.folder == (uint32_t)p ? r.t_lose : r.t_win;
+    const uint32_t* __restrict__ hp = T.hpack[p];
+    for (uint32_t i0 = lane; i0 < nhp; i0 += kBatch * kWave) {
+        uint32_t h[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) h[u] = hp[min(i0 + u * kWave, nhp - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const uint32_t i = i0 + u * kWave;
+            if (i >= nhp) break;
+            const uint32_t sm = h[u] >> 16;
+            const float same_r = sm != 0xFFFFu ? rs[sm] : 0.0f;
+            const float valid = (float)(tot - s_card[h[u] & 0xFFu] - s_card[(h[u] >> 8) & 0xFFu]) + same_r;
+            out[i] = amount * valid;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Up sweep, showdown terminals (game.rs:501-577). The opponent's reach is
-// staged in LDS in strength order; f64 exclusive prefix and suffix sums
-// give the strictly-weaker / strictly-stronger totals at each hand's static
-// rank (lb, ub); per-card prefix and suffix sums (serial, in the CPU's
-// accumulation order) remove the hands that share a card with ours.
-// LDS: m_o floats + 2(m_o+1) doubles + 2(E+52) doubles, E = 2 m_o.
+// Up sweep, showdown terminals (game.rs:501-577), one wavefront per node.
+// The opponent's reach is staged in LDS in strength order; an f64 exclusive
+// prefix P over it gives the strictly-weaker total at each hand's static
+// rank lb (and the strictly-stronger total from ub); per-card prefix sums
+// (serial in strength order, one lane per card, the CPU's accumulation
+// order) remove the opponent hands sharing a card with ours. Hands are
+// written in hand order (coalesced) from one 16-byte static table entry
+// each. LDS per wavefront: 8 (m+1) + 8 (2m+52) + 4 m bytes, m = opponent
+// hands on the board.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const uint32_t* nodes, int p) {
+constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
+
+__global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
+                                                    uint32_t count, int p, uint32_t per_wave) {
     extern __shared__ double lds[];
-    __shared__ double s_part[kBlock];
-    __shared__ double s_scan[kBlock];
+    const uint32_t w = wave_item();
+    if (w >= count) return;
     const int o = 1 - p;
-    const uint32_t n = nodes[blockIdx.x];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const ShowRec r = recs[w];
     const uint32_t nhp = T.nh[p], nho = T.nh[o];
-    float* out = S.cfv + (size_t)T.cfv_slot[n] * T.nh_max;
-    const uint32_t osrc = T.owner[o][n];
-    for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) out[i] = 0.0f;
-    if (!S.alive[o][osrc]) return;
-    const float* ro = S.reach[o] + (size_t)osrc * nho;
-    const uint32_t b = T.show_slot[n];
-    const uint32_t mo = T.sd_cnt[o][b], mp = T.sd_cnt[p][b];
-    const uint16_t* oidx = T.sd_idx[o] + T.sd_off[o][b];
-    const uint32_t* coff = T.sd_coff[o] + (size_t)b * 53;
-    const uint16_t* cpos = T.sd_cpos[o] + T.sd_cpos_base[o][b];
-    const uint32_t E = coff[52];
-    double* P = lds;                  // mo + 1
-    double* Q = P + (mo + 1);         // mo + 1
-    double* cpre = Q + (mo + 1);      // E + 52
-    double* csuf = cpre + (E + 52);   // E + 52
-    float* rs = (float*)(csuf + (E + 52));  // mo
-    for (uint32_t k = threadIdx.x; k < mo; k += kBlock) rs[k] = ro[oidx[k]];
-    __syncthreads();
-    // chunked scan: thread t owns [t*ch, min((t+1)*ch, mo))
-    const uint32_t ch = (mo + kBlock - 1) / kBlock;
-    const uint32_t lo = min(threadIdx.x * ch, mo), hi = min(lo + ch, mo);
-    double acc = 0.0;
-    for (uint32_t k = lo; k < hi; ++k) acc += (double)rs[k];
-    s_part[threadIdx.x] = acc;
-    s_scan[threadIdx.x] = acc;
-    __syncthreads();
-    // inclusive Hillis-Steele scan of the chunk totals
-    for (int s = 1; s < kBlock; s <<= 1) {
-        double add = (int)threadIdx.x >= s ? s_scan[threadIdx.x - s] : 0.0;
-        __syncthreads();
-        s_scan[threadIdx.x] += add;
-        __syncthreads();
+    float* out = S.cfv + (size_t)r.cfv * T.nh_max;
+    const uint32_t oslot = pick(r.own, o);
+    const uint32_t b = r.board;
+    const uint32_t mo = T.sd_cnt[o][b];
+    if (!S.alive[o][oslot] || mo == 0) {  // no opponent hand can be on this board: all values 0
+        for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
+        return;
     }
-    const double total = s_scan[kBlock - 1];
-    {
-        double run = s_scan[threadIdx.x] - s_part[threadIdx.x];  // exclusive base
-        for (uint32_t k = lo; k < hi; ++k) {
-            P[k] = run;
-            run += (double)rs[k];
-        }
-        if (threadIdx.x == kBlock - 1) P[mo] = total;
-        double suf = total - s_scan[threadIdx.x];  // sum of later chunks
-        // suffix within the chunk, accumulated from its end
-        double runs = 0.0;
-        for (uint32_t k = hi; k-- > lo;) {
-            runs += (double)rs[k];
-            Q[k] = runs + suf;
-        }
-        if (threadIdx.x == 0) Q[mo] = 0.0;
+    const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
+    const uint16_t* __restrict__ oidx = T.sd_idx[o] + T.sd_off[o][b];
+    const uint32_t* __restrict__ coff = T.sd_coff[o] + (size_t)b * 53;
+    const uint16_t* __restrict__ cposg = T.sd_cpos[o] + T.sd_cpos_base[o][b];
+    const uint32_t E = 2 * mo;
+    char* base = (char*)lds + (size_t)wave * per_wave;
+    double* P = (double*)base;              // mo + 1
+    double* cpre = P + (mo + 1);            // E + 52
+    float* rs = (float*)(cpre + (E + 52));  // mo
+    for (uint32_t k0 = lane; k0 < mo; k0 += kBatch * kWave) {
+        uint32_t id[kBatch];
+        float v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) id[u] = oidx[min(k0 + u * kWave, mo - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) v[u] = ro[id[u]];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (k0 + u * kWave < mo) rs[k0 + u * kWave] = v[u];
     }
-    if (threadIdx.x < 52) {
-        const uint32_t c = threadIdx.x;
-        const uint32_t s0 = coff[c], s1 = coff[c + 1];
-        const uint32_t base = s0 + c;
+    // this lane's card list positions, fetched while the row lands in LDS
+    // (unconditional loads at clamped indices: no per-entry branches, so
+    // all of them are in flight together)
+    uint32_t s0 = 0, len = 0;
+    uint32_t pos[kMaxCardList];
+    if (lane < 52) {
+        s0 = coff[lane];
+        len = coff[lane + 1] - s0;
+    }
+    const uint32_t last = len ? len - 1 : 0;
+    const uint16_t* cp_src = cposg + (len ? s0 : 0);
+#pragma unroll
+    for (int t = 0; t < kMaxCardList; ++t) pos[t] = cp_src[min((uint32_t)t, last)];
+    wave_sync();
+    // exclusive prefix over strength order: lane-contiguous chunks + a
+    // wavefront scan of the chunk totals
+    const uint32_t ch = (mo + kWave - 1) / kWave;
+    const uint32_t lo = min(lane * ch, mo), hi = min(lo + ch, mo);
+    double part = 0.0;
+    for (uint32_t k = lo; k < hi; ++k) part += (double)rs[k];
+    double incl = part;
+    for (int off = 1; off < kWave; off <<= 1) {
+        double y = __shfl_up(incl, off);
+        if ((int)lane >= off) incl += y;
+    }
+    double run = __shfl_up(incl, 1);
+    if (lane == 0) run = 0.0;
+    const double total = __shfl(incl, kWave - 1);
+    for (uint32_t k = lo; k < hi; ++k) {
+        P[k] = run;
+        run += (double)rs[k];
+    }
+    if (lane == 0) P[mo] = total;
+    // per-card exclusive prefix in strength order, one lane per card
+    if (lane < 52) {
+        double* cp = cpre + s0 + lane;
+        float v[kMaxCardList];
+#pragma unroll
+        for (int t = 0; t < kMaxCardList; ++t) v[t] = mo ? rs[pos[t]] : 0.0f;
         double a = 0.0;
-        for (uint32_t t = s0; t < s1; ++t) {
-            cpre[base + (t - s0)] = a;
-            a += (double)rs[cpos[t]];
+#pragma unroll
+        for (int t = 0; t < kMaxCardList; ++t) {
+            if (t < (int)len) cp[t] = a;
+            a += t < (int)len ? (double)v[t] : 0.0;
         }
-        cpre[base + (s1 - s0)] = a;
-        double d = 0.0;
-        csuf[base + (s1 - s0)] = 0.0;
-        for (uint32_t t = s1; t-- > s0;) {
-            d += (double)rs[cpos[t]];
-            csuf[base + (t - s0)] = d;
-        }
+        cp[len] = a;
     }
-    __syncthreads();
-    const float win = T.t_win[n], lose = T.t_lose[n], tie = T.t_tie[n];
-    const uint16_t* pidx = T.sd_idx[p] + T.sd_off[p][b];
-    const uint32_t* lbub = T.sd_lbub[p] + T.sd_off[p][b];
-    const uint32_t* cnt4 = T.sd_cnt4[p] + T.sd_off[p][b];
-    for (uint32_t k = threadIdx.x; k < mp; k += kBlock) {
-        const uint32_t i = pidx[k];
-        const uint32_t lb = lbub[k] & 0xFFFFu, ub = lbub[k] >> 16;
-        const uint32_t c4 = cnt4[k];
-        const uint32_t a1 = T.c1[p][i], a2 = T.c2[p][i];
-        const uint32_t b1 = coff[a1] + a1, b2 = coff[a2] + a2;
-        const uint32_t l1 = coff[a1 + 1] - coff[a1], l2 = coff[a2 + 1] - coff[a2];
-        const float lower = (float)(P[lb] - cpre[b1 + (c4 & 0xFF)] - cpre[b2 + ((c4 >> 8) & 0xFF)]);
-        const float higher = (float)(Q[ub] - csuf[b1 + ((c4 >> 16) & 0xFF)] - csuf[b2 + (c4 >> 24)]);
-        const uint32_t sm = T.same[p][i];
-        const float same_r = sm != 0xFFFFFFFFu ? ro[sm] : 0.0f;
-        const float valid = (float)(total - cpre[b1 + l1] - cpre[b2 + l2]) + same_r;
-        const float tied = valid - lower - higher;
-        out[i] = win * lower + lose * higher + tie * tied;
+    wave_sync();
+    const float win = r.win, lose = r.lose, tie = r.tie;
+    const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
+    for (uint32_t i0 = lane; i0 < nhp; i0 += kBatch * kWave) {
+        uint4 ent[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) ent[u] = tab[min(i0 + u * kWave, nhp - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const uint32_t i = i0 + u * kWave;
+            if (i >= nhp) break;
+            const uint4 e = ent[u];
+            if (e.x == 0xFFFFFFFFu) {
+                out[i] = 0.0f;
+                continue;
+            }
+            const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
+            const double* cp1 = cpre + (e.y & 0xFFFFu);
+            const double* cp2 = cpre + (e.y >> 16);
+            const double S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
+            const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
+            const float higher =
+                (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
+            const uint32_t sp = e.w >> 16;
+            const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
+            const float valid = (float)(total - S1 - S2) + same_r;
+            const float tied = valid - lower - higher;
+            out[i] = win * lower + lose * higher + tie * tied;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Up sweep, chance nodes (cfr.rs:637-663): sum the children's values over
 // the dealt cards in card order, skipping hands that hold the card, then
-// scale by 1/(46 - street).
+// scale by 1/(46 - street). One workgroup per node (a node reads up to 49
+// child rows); loads are issued in batches of 8 children ahead of the
+// in-order adds.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, const uint32_t* nodes, int p) {
+__global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, const ChanceRec* __restrict__ recs,
+                                                      uint32_t count, int p) {
+    const uint32_t w = xcd_slice(blockIdx.x, gridDim.x);
+    if (w >= count) return;
     const int o = 1 - p;
-    const uint32_t n = nodes[blockIdx.x];
+    const ChanceRec r = recs[w];
     const uint32_t nhp = T.nh[p];
-    float* out = S.cfv + (size_t)T.cfv_slot[n] * T.nh_max;
-    if (!S.alive[o][T.owner[o][n]]) {
+    float* out = S.cfv + (size_t)r.cfv * T.nh_max;
+    if (!S.alive[o][pick(r.own, o)]) {
         for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) out[i] = 0.0f;
         return;
     }
-    const uint32_t off = T.ch_off[n], cnt = T.ch_cnt[n];
-    const float inv = 1.0f / (float)(46 - (int)T.street[n]);
+    const uint8_t* __restrict__ cards = T.chr_card + r.off;
+    const uint32_t* __restrict__ rows = T.chr_cfv + r.off;
+    const float* __restrict__ cfv = S.cfv;
     for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
         const uint64_t hm = T.hmask[p][i];
         float acc = 0.0f;
-        for (uint32_t t = 0; t < cnt; ++t) {
-            if ((hm >> T.ch_card[off + t]) & 1ull) continue;
-            acc += S.cfv[(size_t)T.cfv_slot[T.ch_child[off + t]] * T.nh_max + i];
+        uint32_t t = 0;
+        for (; t + 8 <= r.cnt; t += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = cfv[(size_t)rows[t + u] * T.nh_max + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (!((hm >> cards[t + u]) & 1ull)) acc += v[u];
         }
-        out[i] = acc * inv;
+        for (; t < r.cnt; ++t)
+            if (!((hm >> cards[t]) & 1ull)) acc += cfv[(size_t)rows[t] * T.nh_max + i];
+        out[i] = acc * r.inv;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Up sweep, action nodes.
-//   MODE 0 (DCFR/CFR+ iteration, f32 arenas): at the traverser's nodes
+//   MODE 0 (DCFR/CFR+ iteration): at the traverser's nodes
 //     out = sum_a sigma_a v_a, then R <- R*(R>0 ? pos : neg) + (v_a - out),
-//     S <- S*strat + reach*sigma_a (cfr.rs:297-336, 493-502); locked nodes
-//     play the lock without updating. Opponent nodes sum their children.
+//     S <- S*strat + reach*sigma_a (cfr.rs:297-336, 493-502); compressed
+//     arenas decode, update, take the node's block max and re-quantize
+//     (cfr.rs:350-385). Locked nodes play the lock without updating.
+//     Opponent nodes sum their children.
 //   MODE 1 (best response): max over children at the traverser's nodes,
 //     lock-weighted where locked (best_response.rs:67-95).
 //   MODE 2 (average strategy values): average-strategy weighting
@@ -485,190 +790,247 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
 // Subtrees whose opponent reach is all zero produce zeros and no update.
 // ---------------------------------------------------------------------------
 template <int MODE, bool COMPRESSED>
-__global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, const uint32_t* nodes, int p, int q) {
+__global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
+                                                      uint32_t count, int p, int q) {
+    const uint32_t w = wave_item();
+    if (w >= count) return;
     const int o = 1 - p;
-    const uint32_t n = nodes[blockIdx.x];
+    const uint32_t lane = threadIdx.x & 63;
+    const ActRec r = recs[w];
     const uint32_t nhp = T.nh[p];
     const uint32_t nh_max = T.nh_max;
-    const int na = T.num_children[n];
-    const uint32_t cs = T.children_start[n];
-    float* out = S.cfv + (size_t)T.cfv_slot[n] * nh_max;
-    if (!S.alive[o][T.owner[o][n]]) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) out[i] = 0.0f;
+    const int na = r.na;
+    float* __restrict__ out = S.cfv + (size_t)r.cfv * nh_max;
+    if (!S.alive[o][pick(r.own, o)]) {
+        for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
         return;
     }
-    const float* vb[kMaxUnroll];
-    for_actions(na, [&](int a, auto cached) {
-        if constexpr (decltype(cached)::value) vb[a] = S.cfv + (size_t)T.cfv_slot[T.children[cs + a]] * nh_max;
-    });
-    auto val = [&](int a, auto cached, uint32_t i) -> float {
-        if constexpr (decltype(cached)::value) return vb[a][i];
-        else return S.cfv[(size_t)T.cfv_slot[T.children[cs + a]] * nh_max + i];
-    };
-    if (q != p) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-            float acc = 0.0f;
-            for_actions(na, [&](int a, auto c) { acc += val(a, c, i); });
-            out[i] = acc;
-        }
-        return;
-    }
-    const int64_t loff = T.lock_off[n];
-    if (loff >= 0) {
-        const float* sig = T.lock_sigma + loff;
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-            float acc = 0.0f;
-            for_actions(na, [&](int a, auto c) { acc += sig[(size_t)a * nhp + i] * val(a, c, i); });
-            out[i] = acc;
-        }
-        return;
-    }
-    const uint64_t off = T.data_offset[n];
-    if constexpr (MODE == 1) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-            float best = 0.0f;
-            for_actions(na, [&](int a, auto c) {
-                float v = val(a, c, i);
-                if (a == 0 || v > best) best = v;
-            });
-            out[i] = best;
-        }
-        return;
-    } else if constexpr (MODE == 2) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-            float acc = 0.0f;
-            if constexpr (COMPRESSED) {
-                SigmaHand<kAverageQ> sh;
-                sh.init(S.Sq[p] + off, nhp, i, na);
-                for_actions(na, [&](int a, auto c) {
-                    acc += sh.template sigma<decltype(c)::value>(a) * val(a, c, i);
-                });
+    const uint32_t* kid_cfv = T.kids + r.kid;
+    const float* __restrict__ cfv = S.cfv;
+    auto row = [&](int a) -> const float* { return cfv + (size_t)kid_cfv[a] * nh_max; };
+    if (q != p || r.lock_off >= 0 || MODE == 1) {
+        // opponent node: sum; locked node: lock-weighted; best response: max
+        const float* sig = r.lock_off >= 0 ? T.lock_sigma + r.lock_off : nullptr;
+        const int kind = q != p ? 0 : (sig ? 1 : 2);
+        dispatch_na(na, [&](auto NAc) {
+            constexpr int NA = decltype(NAc)::value;
+            if constexpr (NA > 0) {
+                const float* rw[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) rw[a] = row(a);
+                for (uint32_t i0 = lane; i0 < nhp; i0 += 2 * kWave) {
+                    float v[2][NA];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const uint32_t ii = min(i0 + h * kWave, nhp - 1);
+#pragma unroll
+                        for (int a = 0; a < NA; ++a) v[h][a] = rw[a][ii];
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const uint32_t i = i0 + h * kWave;
+                        if (i >= nhp) break;
+                        float acc = 0.0f;
+                        if (kind == 0) {
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) acc += v[h][a];
+                        } else if (kind == 1) {
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) acc += sig[(size_t)a * nhp + i] * v[h][a];
+                        } else {
+                            acc = v[h][0];
+#pragma unroll
+                            for (int a = 1; a < NA; ++a) acc = v[h][a] > acc ? v[h][a] : acc;
+                        }
+                        out[i] = acc;
+                    }
+                }
             } else {
-                SigmaHand<kAverageF32> sh;
-                sh.init(S.S[p] + off, nhp, i, na);
-                for_actions(na, [&](int a, auto c) {
-                    acc += sh.template sigma<decltype(c)::value>(a) * val(a, c, i);
-                });
+                for (uint32_t i = lane; i < nhp; i += kWave) {
+                    float acc = kind == 2 ? row(0)[i] : 0.0f;
+                    for (int a = 0; a < na; ++a) {
+                        const float v = row(a)[i];
+                        if (kind == 0) acc += v;
+                        else if (kind == 1) acc += sig[(size_t)a * nhp + i] * v;
+                        else if (a > 0 && v > acc) acc = v;
+                    }
+                    out[i] = acc;
+                }
             }
-            out[i] = acc;
-        }
+        });
         return;
-    } else if constexpr (MODE == 3) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-            float acc = 0.0f;
-            if constexpr (COMPRESSED) {
-                SigmaHand<kCurrentQ> sh;
-                sh.init(S.Rq[p] + off, nhp, i, na);
-                for_actions(na, [&](int a, auto c) {
-                    acc += sh.template sigma<decltype(c)::value>(a) * val(a, c, i);
-                });
+    }
+    const uint64_t off = r.doff;
+    if constexpr (MODE == 2 || MODE == 3) {
+        // value of the average (2) or current (3) strategy, no update
+        constexpr int KIND = MODE == 2 ? (COMPRESSED ? kAverageQ : kAverageF32) : (COMPRESSED ? kCurrentQ : kCurrentF32);
+        const auto* arena = ArenaOf<KIND>::get(S, p) + off;
+        dispatch_na(na, [&](auto NAc) {
+            constexpr int NA = decltype(NAc)::value;
+            if constexpr (NA > 0) {
+                for (uint32_t i = lane; i < nhp; i += kWave) {
+                    HandSigma<NA, KIND> hs;
+                    hs.load(arena, nhp, i);
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) acc += hs.sig[a] * row(a)[i];
+                    out[i] = acc;
+                }
             } else {
-                SigmaHand<kCurrentF32> sh;
-                sh.init(S.R[p] + off, nhp, i, na);
-                for_actions(na, [&](int a, auto c) {
-                    acc += sh.template sigma<decltype(c)::value>(a) * val(a, c, i);
-                });
+                for (uint32_t i = lane; i < nhp; i += kWave) {
+                    SigmaHand<KIND> sh;
+                    sh.init(arena, nhp, i, na);
+                    float acc = 0.0f;
+                    for (int a = 0; a < na; ++a) acc += sh.template sigma<false>(a) * row(a)[i];
+                    out[i] = acc;
+                }
             }
-            out[i] = acc;
-        }
+        });
         return;
     } else {
         const uint32_t t = *S.iter;
         const float pos = S.disc[3 * t + 0], neg = S.disc[3 * t + 1], ds = S.disc[3 * t + 2];
-        const float* rp = S.reach[p] + (size_t)T.owner[p][n] * nhp;
+        const float* __restrict__ rp = S.reach[p] + (size_t)pick(r.own, p) * nhp;
         if constexpr (!COMPRESSED) {
-            float* R = S.R[p] + off;
-            float* Sm = S.S[p] + off;
-            for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-                SigmaHand<kCurrentF32> sh;
-                sh.init(R, nhp, i, na);
-                float v[kMaxUnroll];
-                float acc = 0.0f;
-                for_actions(na, [&](int a, auto c) {
-                    constexpr bool C = decltype(c)::value;
-                    float va = val(a, c, i);
-                    if constexpr (C) v[a] = va;
-                    acc += sh.template sigma<C>(a) * va;
-                });
-                const float reach = rp[i];
-                for_actions(na, [&](int a, auto c) {
-                    constexpr bool C = decltype(c)::value;
-                    const size_t idx = (size_t)a * nhp + i;
-                    const float r = sh.template raw<C>(a);
-                    float va;
-                    if constexpr (C) va = v[a];
-                    else va = val(a, c, i);
-                    const float d = r > 0.0f ? pos : neg;
-                    R[idx] = r * d + (va - acc);
-                    Sm[idx] = Sm[idx] * ds + reach * sh.template sigma<C>(a);
-                });
-                out[i] = acc;
-            }
-        } else {
-            // Compressed arenas (cfr.rs:350-385): decode + discount + update
-            // with a block-wide max, then quantize against the new maxima.
-            // Pass 2 recomputes the same values (deterministically) instead
-            // of staging na*nh floats.
-            __shared__ float s_rmax[kBlock], s_smax[kBlock];
-            int16_t* Rq = S.Rq[p] + off;
-            uint16_t* Sq = S.Sq[p] + off;
-            const float kr = S.Rs[p][n] / 32767.0f;
-            const float kss = S.Ss[p][n] / 65535.0f * ds;
-            float rmax = 0.0f, smax = 0.0f;
-            auto update_vals = [&](uint32_t i, auto&& sink) {
-                SigmaHand<kCurrentQ> sh;
-                sh.init(Rq, nhp, i, na);
-                float v[kMaxUnroll];
-                float acc = 0.0f;
-                for_actions(na, [&](int a, auto c) {
-                    constexpr bool C = decltype(c)::value;
-                    float va = val(a, c, i);
-                    if constexpr (C) v[a] = va;
-                    acc += sh.template sigma<C>(a) * va;
-                });
-                const float reach = rp[i];
-                for_actions(na, [&](int a, auto c) {
-                    constexpr bool C = decltype(c)::value;
-                    const size_t idx = (size_t)a * nhp + i;
-                    const int qv = sh.template raw<C>(a);
-                    float va;
-                    if constexpr (C) va = v[a];
-                    else va = val(a, c, i);
-                    const float d = qv > 0 ? pos : neg;
-                    const float nr = (float)qv * kr * d + (va - acc);
-                    const float ns = (float)Sq[idx] * kss + reach * sh.template sigma<C>(a);
-                    sink(idx, nr, ns);
-                });
-                return acc;
-            };
-            for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-                out[i] = update_vals(i, [&](size_t, float nr, float ns) {
-                    rmax = fmaxf(rmax, fabsf(nr));
-                    smax = fmaxf(smax, ns);
-                });
-            }
-            s_rmax[threadIdx.x] = rmax;
-            s_smax[threadIdx.x] = smax;
-            __syncthreads();
-            for (int st = kBlock / 2; st > 0; st >>= 1) {
-                if ((int)threadIdx.x < st) {
-                    s_rmax[threadIdx.x] = fmaxf(s_rmax[threadIdx.x], s_rmax[threadIdx.x + st]);
-                    s_smax[threadIdx.x] = fmaxf(s_smax[threadIdx.x], s_smax[threadIdx.x + st]);
+            float* __restrict__ R = S.R[p] + off;
+            float* __restrict__ Sm = S.S[p] + off;
+            dispatch_na(na, [&](auto NAc) {
+                constexpr int NA = decltype(NAc)::value;
+                if constexpr (NA > 0) {
+                    const float* rw[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) rw[a] = row(a);
+                    // two hands per step: every load of both is issued before
+                    // the first store
+                    for (uint32_t i0 = lane; i0 < nhp; i0 += 2 * kWave) {
+                        HandSigma<NA, kCurrentF32> hs[2];
+                        float v[2][NA], sm[2][NA], reach[2];
+                        bool ok[2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const uint32_t i = i0 + h * kWave;
+                            ok[h] = i < nhp;
+                            const uint32_t ii = ok[h] ? i : i0;
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) {
+                                hs[h].raw[a] = R[(size_t)a * nhp + ii];
+                                v[h][a] = rw[a][ii];
+                                sm[h][a] = Sm[(size_t)a * nhp + ii];
+                            }
+                            reach[h] = rp[ii];
+                        }
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            if (!ok[h]) continue;
+                            const uint32_t i = i0 + h * kWave;
+                            hs[h].finish();
+                            float acc = 0.0f;
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) acc += hs[h].sig[a] * v[h][a];
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) {
+                                const float rv = hs[h].raw[a];
+                                const float d = rv > 0.0f ? pos : neg;
+                                R[(size_t)a * nhp + i] = rv * d + (v[h][a] - acc);
+                                Sm[(size_t)a * nhp + i] = sm[h][a] * ds + reach[h] * hs[h].sig[a];
+                            }
+                            out[i] = acc;
+                        }
+                    }
+                } else {
+                    for (uint32_t i = lane; i < nhp; i += kWave) {
+                        SigmaHand<kCurrentF32> sh;
+                        sh.init(R, nhp, i, na);
+                        float acc = 0.0f;
+                        for (int a = 0; a < na; ++a) acc += sh.template sigma<false>(a) * row(a)[i];
+                        const float reach = rp[i];
+                        for (int a = 0; a < na; ++a) {
+                            const size_t idx = (size_t)a * nhp + i;
+                            const float rv = R[idx];
+                            const float sg = sh.template sigma<false>(a);
+                            const float d = rv > 0.0f ? pos : neg;
+                            R[idx] = rv * d + (row(a)[i] - acc);
+                            Sm[idx] = Sm[idx] * ds + reach * sg;
+                        }
+                        out[i] = acc;
+                    }
                 }
-                __syncthreads();
-            }
-            const float RM = s_rmax[0], SM = s_smax[0];
-            const bool rok = RM > 0.0f && isfinite(RM), sok = SM > 0.0f && isfinite(SM);
-            const float rk = rok ? 32767.0f / RM : 0.0f, sk = sok ? 65535.0f / SM : 0.0f;
-            for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
-                update_vals(i, [&](size_t idx, float nr, float ns) {
-                    Rq[idx] = rok ? quant_i16(nr * rk) : (int16_t)0;
-                    Sq[idx] = sok ? quant_u16(fmaxf(ns, 0.0f) * sk) : (uint16_t)0;
-                });
-            }
-            if (threadIdx.x == 0) {
-                S.Rs[p][n] = rok ? RM : 0.0f;
-                S.Ss[p][n] = sok ? SM : 0.0f;
+            });
+        } else {
+            // Compressed arenas (cfr.rs:350-385). Pass 1 finds the node's new
+            // block maxima; pass 2 recomputes the same values
+            // (deterministically) and quantizes them.
+            int16_t* __restrict__ Rq = S.Rq[p] + off;
+            uint16_t* __restrict__ Sq = S.Sq[p] + off;
+            const float kr = S.Rs[p][r.node] / 32767.0f;
+            const float kss = S.Ss[p][r.node] / 65535.0f * ds;
+            float rmax = 0.0f, smax = 0.0f;
+            float RM = 0.0f, SM = 0.0f, rk = 0.0f, sk = 0.0f;
+            bool rok = false, sok = false;
+            dispatch_na(na, [&](auto NAc) {
+                constexpr int NA = decltype(NAc)::value;
+                auto hand = [&](uint32_t i, auto&& sink) -> float {
+                    if constexpr (NA > 0) {
+                        HandSigma<NA, kCurrentQ> hs;
+                        hs.load(Rq, nhp, i);
+                        float v[NA], sq[NA];
+#pragma unroll
+                        for (int a = 0; a < NA; ++a) {
+                            v[a] = row(a)[i];
+                            sq[a] = (float)Sq[(size_t)a * nhp + i];
+                        }
+                        const float reach = rp[i];
+                        float acc = 0.0f;
+#pragma unroll
+                        for (int a = 0; a < NA; ++a) acc += hs.sig[a] * v[a];
+#pragma unroll
+                        for (int a = 0; a < NA; ++a) {
+                            const int qv = hs.raw[a];
+                            const float d = qv > 0 ? pos : neg;
+                            const float nr = (float)qv * kr * d + (v[a] - acc);
+                            const float ns = sq[a] * kss + reach * hs.sig[a];
+                            sink((size_t)a * nhp + i, nr, ns);
+                        }
+                        return acc;
+                    } else {
+                        SigmaHand<kCurrentQ> sh;
+                        sh.init(Rq, nhp, i, na);
+                        float acc = 0.0f;
+                        for (int a = 0; a < na; ++a) acc += sh.template sigma<false>(a) * row(a)[i];
+                        const float reach = rp[i];
+                        for (int a = 0; a < na; ++a) {
+                            const size_t idx = (size_t)a * nhp + i;
+                            const int qv = Rq[idx];
+                            const float d = qv > 0 ? pos : neg;
+                            const float nr = (float)qv * kr * d + (row(a)[i] - acc);
+                            const float ns = (float)Sq[idx] * kss + reach * sh.template sigma<false>(a);
+                            sink(idx, nr, ns);
+                        }
+                        return acc;
+                    }
+                };
+                for (uint32_t i = lane; i < nhp; i += kWave) {
+                    out[i] = hand(i, [&](size_t, float nr, float ns) {
+                        rmax = fmaxf(rmax, fabsf(nr));
+                        smax = fmaxf(smax, ns);
+                    });
+                }
+                RM = wave_max_f32(rmax);
+                SM = wave_max_f32(smax);
+                rok = RM > 0.0f && isfinite(RM);
+                sok = SM > 0.0f && isfinite(SM);
+                rk = rok ? 32767.0f / RM : 0.0f;
+                sk = sok ? 65535.0f / SM : 0.0f;
+                for (uint32_t i = lane; i < nhp; i += kWave) {
+                    hand(i, [&](size_t idx, float nr, float ns) {
+                        Rq[idx] = rok ? quant_i16(nr * rk) : (int16_t)0;
+                        Sq[idx] = sok ? quant_u16(fmaxf(ns, 0.0f) * sk) : (uint16_t)0;
+                    });
+                }
+            });
+            if (lane == 0) {
+                S.Rs[p][r.node] = rok ? RM : 0.0f;
+                S.Ss[p][r.node] = sok ? SM : 0.0f;
             }
         }
     }

@@ -223,6 +223,108 @@ Plan build_plan(const gto_spot_view& v) {
             }
         }
     }
+    // hand-ordered showdown tables: 16 bytes per (board, hand of p)
+    //   x = lb | ub << 16        (0xFFFFFFFF: the hand conflicts with the board)
+    //   y = seg(c1) | seg(c2) << 16   start of each card's list in the
+    //       opponent's per-card arrays (the card offset plus the card index)
+    //   z = entries before lb (c1, c2), before ub (c1, c2), one byte each
+    //   w = list length (c1) | (c2) << 8 | opponent sorted position of the
+    //       identical combo << 16 (0xFFFF: none)
+    for (int p = 0; p < 2; ++p) {
+        const int o = 1 - p;
+        const size_t nhp = v.num_hands[p];
+        pl.sd_tab[p].assign((size_t)pl.num_boards * nhp * 4, 0u);
+        for (size_t e = 0; e < (size_t)pl.num_boards * nhp; ++e) pl.sd_tab[p][4 * e] = 0xFFFFFFFFu;
+        for (uint32_t b = 0; b < pl.num_boards; ++b) {
+            const uint32_t* coff = &pl.sd_coff[o][(size_t)b * 53];
+            std::vector<int> pos_of(v.num_hands[o], -1);
+            for (uint32_t t = 0; t < pl.sd_cnt[o][b]; ++t) pos_of[pl.sd_idx[o][pl.sd_off[o][b] + t]] = (int)t;
+            for (uint32_t t = 0; t < pl.sd_cnt[p][b]; ++t) {
+                uint32_t k = pl.sd_off[p][b] + t;
+                uint16_t h = pl.sd_idx[p][k];
+                uint32_t a1 = v.hand_c1[p][h], a2 = v.hand_c2[p][h];
+                uint32_t l1 = coff[a1 + 1] - coff[a1], l2 = coff[a2 + 1] - coff[a2];
+                uint32_t sp = 0xFFFFu;
+                uint32_t sm = v.same_combo[p][h];
+                if (sm != kSent) {
+                    if (pos_of[sm] < 0) throw std::runtime_error("identical combo missing from a board list");
+                    sp = (uint32_t)pos_of[sm];
+                }
+                if (coff[52] + 52 > 0xFFFFu || l1 > 255 || l2 > 255) throw std::runtime_error("showdown table overflow");
+                uint32_t* e = &pl.sd_tab[p][4 * ((size_t)b * nhp + h)];
+                e[0] = pl.sd_lbub[p][k];
+                e[1] = (coff[a1] + a1) | ((coff[a2] + a2) << 16);
+                e[2] = pl.sd_cnt4[p][k];
+                e[3] = l1 | (l2 << 8) | (sp << 16);
+            }
+        }
+    }
+    // work records
+    for (uint32_t l = 0; l < L; ++l) {
+        LevelLists& ll = pl.levels[l];
+        for (int q = 0; q < 2; ++q)
+            for (uint32_t n : ll.act[q]) {
+                ActRec r{};
+                r.doff = v.node_data_offset[n];
+                r.lock_off = -1;
+                r.node = n;
+                r.own[0] = pl.owner[0][n];
+                r.own[1] = pl.owner[1][n];
+                r.cfv = pl.cfv_slot[n];
+                r.na = v.node_num_children[n];
+                r.kid = (uint32_t)pl.kids.size();
+                pl.max_actions = std::max(pl.max_actions, r.na);
+                const uint32_t cs = v.node_children_start[n];
+                for (uint32_t a = 0; a < r.na; ++a) pl.kids.push_back(pl.cfv_slot[v.children[cs + a]]);
+                for (uint32_t a = 0; a < r.na; ++a) pl.kids.push_back(pl.owner[q][v.children[cs + a]]);
+                ll.act_rec[q].push_back(r);
+            }
+        for (uint32_t n : ll.fold) {
+            FoldRec r{};
+            r.cfv = pl.cfv_slot[n];
+            r.own[0] = pl.owner[0][n];
+            r.own[1] = pl.owner[1][n];
+            r.folder = v.node_player[n];
+            r.t_win = (float)v.node_t_win[n];
+            r.t_lose = (float)v.node_t_lose[n];
+            ll.fold_rec.push_back(r);
+        }
+        for (uint32_t n : ll.show) {
+            ShowRec r{};
+            r.cfv = pl.cfv_slot[n];
+            r.own[0] = pl.owner[0][n];
+            r.own[1] = pl.owner[1][n];
+            r.board = pl.show_slot[n];
+            r.win = (float)v.node_t_win[n];
+            r.lose = (float)v.node_t_lose[n];
+            r.tie = (float)v.node_t_tie[n];
+            ll.show_rec.push_back(r);
+        }
+        for (uint32_t n : ll.chance) {
+            ChanceRec r{};
+            r.cfv = pl.cfv_slot[n];
+            r.own[0] = pl.owner[0][n];
+            r.own[1] = pl.owner[1][n];
+            r.off = (uint32_t)pl.chr_cfv.size();
+            r.cnt = pl.ch_cnt[n];
+            r.inv = 1.0f / (float)(46 - (int)v.node_street[n]);
+            for (uint32_t t = 0; t < pl.ch_cnt[n]; ++t) {
+                pl.chr_card.push_back(pl.ch_card[pl.ch_off[n] + t]);
+                pl.chr_cfv.push_back(pl.cfv_slot[pl.ch_child[pl.ch_off[n] + t]]);
+            }
+            ll.chance_rec.push_back(r);
+        }
+        for (size_t e = 0; e < ll.edge_parent.size(); ++e) {
+            EdgeRec r{};
+            for (int q = 0; q < 2; ++q) {
+                r.src[q] = pl.owner[q][ll.edge_parent[e]];
+                r.dst[q] = pl.owner[q][ll.edge_child[e]];
+            }
+            r.card = ll.edge_card[e];
+            ll.edge_rec.push_back(r);
+        }
+    }
+
     // per-card hand lists (fold terminals)
     for (int q = 0; q < 2; ++q) {
         std::vector<std::vector<uint16_t>> by_card(52);

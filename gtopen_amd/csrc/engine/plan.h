@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../../include/gtopen_amd.h"
+#include "records.h"
 
 namespace gto {
 
@@ -15,6 +16,12 @@ struct LevelLists {
     std::vector<uint32_t> act[2], fold, show, chance;
     // chance edges whose parent sits at this depth: (parent, child, card)
     std::vector<uint32_t> edge_parent, edge_child, edge_card;
+    // work records, in the same order as the lists above
+    std::vector<ActRec> act_rec[2];
+    std::vector<FoldRec> fold_rec;
+    std::vector<ShowRec> show_rec;
+    std::vector<ChanceRec> chance_rec;
+    std::vector<EdgeRec> edge_rec;
 };
 
 struct Plan {
@@ -60,6 +67,17 @@ struct Plan {
     // per player: hand indices holding card c, ascending (fold card sums)
     std::vector<uint32_t> hc_off[2];  // 53 entries
     std::vector<uint16_t> hc_idx[2];
+
+    // children of action nodes referenced by ActRec::kid
+    std::vector<uint32_t> kids;
+    // chance children referenced by ChanceRec: dealt card and value row
+    std::vector<uint8_t> chr_card;
+    std::vector<uint32_t> chr_cfv;
+    // per board slot and hand of q (hand order), four words: rank bounds,
+    // per-card list segments, per-card counts, list lengths + identical
+    // combo position (layout documented in build_plan)
+    std::vector<uint32_t> sd_tab[2];
+    uint32_t max_actions = 0;
 
     // algorithmic arena entries over reached action nodes (sum na*nh_actor)
     uint64_t reached_entries = 0;

@@ -159,3 +159,28 @@ def test_compressed_bench_spot_reproduces_reference_trajectory(built):
         e = gpu.exploitability()
         _say(f"  iter {it}: gpu {e:.4f} chips, reference {GOLDEN_I16[it]:.4f}")
         assert abs(e - GOLDEN_I16[it]) / spot.starting_pot < POT_TOL
+
+
+def test_full_range_river_matches_oracle(built):
+    """All 1081 combos per player on a river board (the largest showdown
+    lists: the showdown kernel runs with fewer wavefronts per workgroup)."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, SpotConfig, StreetSizing, TreeConfig
+
+    _say("case: river, full 1081x1081 ranges, 30 iterations")
+    s = [StreetSizing(), StreetSizing(), StreetSizing.of("50 a", "2.5x")]
+    tree = TreeConfig(starting_pot=10.0, effective_stack=40.0, oop=s, ip=s, max_raises=1)
+    spot = Spot(SpotConfig("Kd8s5h2c2d", "22+,A2+,K2+,Q2+,J2+,T2+,92+,82+,72+,62+,52+,42+,32",
+                           "22+,A2+,K2+,Q2+,J2+,T2+,92+,82+,72+,62+,52+,42+,32", tree))
+    assert spot.num_hands == (1081, 1081)
+    gpu, cpu = GpuSolver(spot), oracle.Oracle(spot)
+    for _ in range(3):
+        gpu.iterate(10)
+        cpu.iterate(10)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {gpu.iteration}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) / spot.starting_pot < POT_TOL
+    rg, sg = gpu.download_arenas()
+    rc, sc = cpu.arenas()
+    _close_arenas(rg, rc, spot.starting_pot)
+    _close_arenas(sg, sc, 1.0)
