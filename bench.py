@@ -22,6 +22,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(args, kind):
+    """HBM bytes per iteration of the kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (profiles/), if any."""
+    path = os.path.join(ROOT, "profiles", f"{args.config}_{args.storage}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    prof = json.load(open(path))
+    key = {"up_action": "k_up_action<0, %s>" % ("true" if args.storage == "compressed" else "false"),
+           "down_action": "k_down_action<%d>" % (2 if args.storage == "compressed" else 0)}.get(kind)
+    k = prof["kernels"].get(key)
+    if not k:
+        return None
+    return round(k["read_bytes_per_iter"] + k["write_bytes_per_iter"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -33,7 +48,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--storage", default="f32", choices=["f32", "compressed"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB mebibytes (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe << 18  # float32 elements
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"bench: copy probe {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(the tree would shard by turn card across GPUs; not built)", file=sys.stderr)
@@ -82,7 +110,9 @@ def main():
     if dom_bytes is not None:
         achieved = dom_bytes / (dom_ms_iter * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(args, dom),
+                    "basis": "bytes and time summed over one iteration's launches of the kernel",
                     "avg_launch_us": round(kinds[dom][0] * 1e3 / dom_launches, 2),
                     "algorithmic_bytes_per_iter": dom_bytes}
     whole = 5.0 * eb * D / (ms_per_step * 1e-3) / 1e9

@@ -391,6 +391,8 @@ class Engine {
             T_.weights[q] = upload_arr(v.weights[q], nh);
             T_.hc_off[q] = upload_vec(plan_.hc_off[q]);
             T_.hc_idx[q] = upload_vec(plan_.hc_idx[q]);
+            T_.hc_t[q] = upload_vec(plan_.hc_t[q]);
+            T_.sd_cpos_t[q] = upload_vec(plan_.sd_cpos_t[q]);
             T_.sd_off[q] = upload_vec(plan_.sd_off[q]);
             T_.sd_cnt[q] = upload_vec(plan_.sd_cnt[q]);
             T_.sd_idx[q] = upload_vec(plan_.sd_idx[q]);
@@ -402,7 +404,9 @@ class Engine {
         // LDS: showdown, per wavefront, for traverser p (opponent list length)
         for (int p = 0; p < 2; ++p) {
             size_t mo = plan_.max_board_cnt[1 - p];
-            show_wave_[p] = (uint32_t)(((28 * mo + 424 + 16) + 15) / 16 * 16);
+            const size_t nh_o = plan_.nh[1 - p];
+            const size_t creg = std::max<size_t>(2 * mo + 52, (nh_o + 1) / 2);
+            show_wave_[p] = (uint32_t)((8 * (mo + 1) + 8 * creg + 4 * mo + 16 + 15) / 16 * 16);
             show_waves_[p] = (uint32_t)std::min<size_t>(kWaves, (160 * 1024) / show_wave_[p]);
             if (show_waves_[p] == 0)
                 throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");

@@ -528,7 +528,28 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
 // order), one lane per card, over the card's hand list fetched into
 // registers first. LDS per wavefront: nh_o floats + 52 doubles.
 // ---------------------------------------------------------------------------
-constexpr int kMaxHandList = 51;  // hands holding one card: at most 51
+constexpr int kMaxHandList = 56;  // hands holding one card: at most 51 (a multiple of 8)
+constexpr int kMaxH = 21;         // hand rows per lane: 21 * 64 >= 1326
+
+// In-order f64 sum of the first `len` of this lane's LDS values vals[idx[t]]
+// (idx packed two per word), reading 8 at a time ahead of the adds.
+template <int N>  // N: a multiple of 8
+__device__ __forceinline__ double ordered_sum(const float* vals, const uint32_t (&idx2)[N / 2], uint32_t len) {
+    static_assert(N % 8 == 0, "ordered_sum reads 8 entries at a time");
+    double acc = 0.0;
+#pragma unroll
+    for (int t0 = 0; t0 < N; t0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t w2 = idx2[(t0 + u) / 2];
+            v[u] = vals[((t0 + u) & 1) ? (w2 >> 16) : (w2 & 0xFFFFu)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += (t0 + u) < (int)len ? (double)v[u] : 0.0;
+    }
+    return acc;
+}
 
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
                                                     uint32_t count, int p) {
@@ -539,9 +560,27 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     if (w >= count) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
+    // static per-card hand lists and packed hand words: independent of the
+    // node, issued together with the record load
+    uint32_t len = 0, k0 = 0;
+    if (lane < 52) {
+        k0 = T.hc_off[o][lane];
+        len = T.hc_off[o][lane + 1] - k0;
+    }
     const FoldRec r = recs[w];
     float* out = S.cfv + (size_t)r.cfv * T.nh_max;
     const uint32_t oslot = pick(r.own, o);
+    const uint32_t* __restrict__ hp = T.hpack[p];
+    uint32_t hw[kMaxH];
+#pragma unroll
+    for (int u = 0; u < kMaxH; ++u)
+        if (u * kWave < (int)nhp) hw[u] = hp[min(lane + u * kWave, nhp - 1)];
+    // entry-major lists: entry t of every card is one contiguous 52-wide row
+    const uint16_t* hsrc = T.hc_t[o] + min(lane, 51u);
+    uint32_t idx2[kMaxHandList / 2];
+#pragma unroll
+    for (int t = 0; t < kMaxHandList; t += 2)
+        idx2[t / 2] = (uint32_t)hsrc[t * 52] | ((uint32_t)hsrc[(t + 1) * 52] << 16);
     if (!S.alive[o][oslot]) {
         for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
         return;
@@ -551,54 +590,31 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     double* s_card = (double*)base;
     float* rs = (float*)(base + 52 * 8);
     const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
-    // (unconditional loads at clamped indices: no per-entry branches)
-    uint32_t len = 0, k0 = 0;
-    uint32_t idx[kMaxHandList];
-    if (lane < 52) {
-        k0 = T.hc_off[o][lane];
-        len = T.hc_off[o][lane + 1] - k0;
-    }
-    const uint32_t last = len ? len - 1 : 0;
-    const uint16_t* hsrc = T.hc_idx[o] + (len ? k0 : 0);
+    float rh[kMaxH];
 #pragma unroll
-    for (int t = 0; t < kMaxHandList; ++t) idx[t] = hsrc[min((uint32_t)t, last)];
+    for (int u = 0; u < kMaxH; ++u)
+        if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
     double part = 0.0;
-    for (uint32_t j0 = lane; j0 < nho; j0 += kBatch * kWave) {
-        float v[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) v[u] = ro[min(j0 + u * kWave, nho - 1)];
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u)
-            if (j0 + u * kWave < nho) {
-                rs[j0 + u * kWave] = v[u];
-                part += (double)v[u];
-            }
-    }
+    for (int u = 0; u < kMaxH; ++u)
+        if (lane + u * kWave < nho) {
+            rs[lane + u * kWave] = rh[u];
+            part += (double)rh[u];
+        }
     const double tot = wave_sum_f64(part);
     wave_sync();
-    if (lane < 52) {
-        float v[kMaxHandList];
-#pragma unroll
-        for (int t = 0; t < kMaxHandList; ++t) v[t] = rs[idx[t]];
-        double acc = 0.0;
-#pragma unroll
-        for (int t = 0; t < kMaxHandList; ++t) acc += t < (int)len ? (double)v[t] : 0.0;
-        s_card[lane] = acc;
-    }
+    if (lane < 52) s_card[lane] = ordered_sum<kMaxHandList>(rs, idx2, len);
     wave_sync();
     const float amount = r.folder == (uint32_t)p ? r.t_lose : r.t_win;
-    const uint32_t* __restrict__ hp = T.hpack[p];
-    for (uint32_t i0 = lane; i0 < nhp; i0 += kBatch * kWave) {
-        uint32_t h[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) h[u] = hp[min(i0 + u * kWave, nhp - 1)];
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const uint32_t i = i0 + u * kWave;
-            if (i >= nhp) break;
-            const uint32_t sm = h[u] >> 16;
+    for (int u = 0; u < kMaxH; ++u) {
+        const uint32_t i = lane + u * kWave;
+        if (u * kWave >= (int)nhp) break;
+        if (i < nhp) {
+            const uint32_t h = hw[u];
+            const uint32_t sm = h >> 16;
             const float same_r = sm != 0xFFFFu ? rs[sm] : 0.0f;
-            const float valid = (float)(tot - s_card[h[u] & 0xFFu] - s_card[(h[u] >> 8) & 0xFFu]) + same_r;
+            const float valid = (float)(tot - s_card[h & 0xFFu] - s_card[(h >> 8) & 0xFFu]) + same_r;
             out[i] = amount * valid;
         }
     }
@@ -616,6 +632,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 // hands on the board.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
+static_assert(kMaxCardList % 8 == 0, "card lists are read 8 entries at a time");
 
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     uint32_t count, int p, uint32_t per_wave) {
@@ -630,44 +647,54 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     float* out = S.cfv + (size_t)r.cfv * T.nh_max;
     const uint32_t oslot = pick(r.own, o);
     const uint32_t b = r.board;
+    // node and board scalars (one round trip)
+    const bool alive = S.alive[o][oslot];
     const uint32_t mo = T.sd_cnt[o][b];
-    if (!S.alive[o][oslot] || mo == 0) {  // no opponent hand can be on this board: all values 0
+    const uint32_t ooff = T.sd_off[o][b];
+    const uint32_t* __restrict__ coff = T.sd_coff[o] + (size_t)b * 53;
+    uint32_t c0 = 0, c1 = 0;
+    if (lane < 52) {
+        c0 = coff[lane];
+        c1 = coff[lane + 1];
+    }
+    if (!alive || mo == 0) {  // no reach, or no opponent hand on this board: all values 0
         for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
         return;
     }
+    // the opponent row in hand order, the board's strength order and this
+    // lane's card list, all in flight together
     const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
-    const uint16_t* __restrict__ oidx = T.sd_idx[o] + T.sd_off[o][b];
-    const uint32_t* __restrict__ coff = T.sd_coff[o] + (size_t)b * 53;
-    const uint16_t* __restrict__ cposg = T.sd_cpos[o] + T.sd_cpos_base[o][b];
-    const uint32_t E = 2 * mo;
+    const uint16_t* __restrict__ oidx = T.sd_idx[o] + ooff;
+    float rh[kMaxH];
+#pragma unroll
+    for (int u = 0; u < kMaxH; ++u)
+        if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
+    uint32_t oi[kMaxH];
+#pragma unroll
+    for (int u = 0; u < kMaxH; ++u)
+        if (u * kWave < (int)mo) oi[u] = oidx[min(lane + u * kWave, mo - 1)];
+    const uint32_t len = c1 - c0;
+    // entry-major lists: entry t of every card is one contiguous 52-wide row
+    const uint16_t* cp_src = T.sd_cpos_t[o] + (size_t)b * (kMaxCardList * 52) + min(lane, 51u);
+    uint32_t pos2[kMaxCardList / 2];
+#pragma unroll
+    for (int t = 0; t < kMaxCardList; t += 2)
+        pos2[t / 2] = (uint32_t)cp_src[t * 52] | ((uint32_t)cp_src[(t + 1) * 52] << 16);
+    // LDS: P (mo+1 doubles) | card prefixes (max(2mo+52, nho/2) doubles; the
+    // hand-order row is staged there first) | rs (mo floats)
+    const uint32_t creg = max(2 * mo + 52, (nho + 1) / 2);
     char* base = (char*)lds + (size_t)wave * per_wave;
-    double* P = (double*)base;              // mo + 1
-    double* cpre = P + (mo + 1);            // E + 52
-    float* rs = (float*)(cpre + (E + 52));  // mo
-    for (uint32_t k0 = lane; k0 < mo; k0 += kBatch * kWave) {
-        uint32_t id[kBatch];
-        float v[kBatch];
+    double* P = (double*)base;
+    double* cpre = P + (mo + 1);
+    float* rs = (float*)(cpre + creg);
+    float* rhs = (float*)cpre;
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) id[u] = oidx[min(k0 + u * kWave, mo - 1)];
+    for (int u = 0; u < kMaxH; ++u)
+        if (lane + u * kWave < nho) rhs[lane + u * kWave] = rh[u];
+    wave_sync();
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) v[u] = ro[id[u]];
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u)
-            if (k0 + u * kWave < mo) rs[k0 + u * kWave] = v[u];
-    }
-    // this lane's card list positions, fetched while the row lands in LDS
-    // (unconditional loads at clamped indices: no per-entry branches, so
-    // all of them are in flight together)
-    uint32_t s0 = 0, len = 0;
-    uint32_t pos[kMaxCardList];
-    if (lane < 52) {
-        s0 = coff[lane];
-        len = coff[lane + 1] - s0;
-    }
-    const uint32_t last = len ? len - 1 : 0;
-    const uint16_t* cp_src = cposg + (len ? s0 : 0);
-#pragma unroll
-    for (int t = 0; t < kMaxCardList; ++t) pos[t] = cp_src[min((uint32_t)t, last)];
+    for (int u = 0; u < kMaxH; ++u)
+        if (lane + u * kWave < mo) rs[lane + u * kWave] = rhs[oi[u]];
     wave_sync();
     // exclusive prefix over strength order: lane-contiguous chunks + a
     // wavefront scan of the chunk totals
@@ -688,29 +715,36 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         run += (double)rs[k];
     }
     if (lane == 0) P[mo] = total;
-    // per-card exclusive prefix in strength order, one lane per card
+    // per-card exclusive prefix in strength order (the CPU's order), one
+    // lane per card
     if (lane < 52) {
-        double* cp = cpre + s0 + lane;
-        float v[kMaxCardList];
-#pragma unroll
-        for (int t = 0; t < kMaxCardList; ++t) v[t] = mo ? rs[pos[t]] : 0.0f;
+        double* cp = cpre + c0 + lane;
         double a = 0.0;
 #pragma unroll
-        for (int t = 0; t < kMaxCardList; ++t) {
-            if (t < (int)len) cp[t] = a;
-            a += t < (int)len ? (double)v[t] : 0.0;
+        for (int t0 = 0; t0 < kMaxCardList; t0 += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t w2 = pos2[(t0 + u) / 2];
+                v[u] = rs[((t0 + u) & 1) ? (w2 >> 16) : (w2 & 0xFFFFu)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (t0 + u < (int)len) cp[t0 + u] = a;
+                a += (t0 + u) < (int)len ? (double)v[u] : 0.0;
+            }
         }
         cp[len] = a;
     }
     wave_sync();
     const float win = r.win, lose = r.lose, tie = r.tie;
     const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
-    for (uint32_t i0 = lane; i0 < nhp; i0 += kBatch * kWave) {
-        uint4 ent[kBatch];
+    for (uint32_t i0 = lane; i0 < nhp; i0 += 8 * kWave) {
+        uint4 ent[8];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) ent[u] = tab[min(i0 + u * kWave, nhp - 1)];
+        for (int u = 0; u < 8; ++u) ent[u] = tab[min(i0 + u * kWave, nhp - 1)];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const uint32_t i = i0 + u * kWave;
             if (i >= nhp) break;
             const uint4 e = ent[u];

@@ -11,6 +11,9 @@
 
 namespace gto {
 
+constexpr uint32_t kHandListCap = 56;  // hands holding one card: <= 51
+constexpr uint32_t kCardListCap = 48;  // one card's hands on a river board: <= 46
+
 struct LevelLists {
     // action nodes split by actor, fold, showdown and chance nodes at this depth
     std::vector<uint32_t> act[2], fold, show, chance;
@@ -67,6 +70,13 @@ struct Plan {
     // per player: hand indices holding card c, ascending (fold card sums)
     std::vector<uint32_t> hc_off[2];  // 53 entries
     std::vector<uint16_t> hc_idx[2];
+    // the same lists entry-major: [t * 52 + c] = t-th hand holding card c
+    // (0 past the list's end), kHandListCap rows, so the lanes of a
+    // wavefront (one per card) read one contiguous row per entry
+    std::vector<uint16_t> hc_t[2];
+    // per board, q's per-card position lists entry-major likewise:
+    // [board][t * 52 + c], kCardListCap rows
+    std::vector<uint16_t> sd_cpos_t[2];
 
     // children of action nodes referenced by ActRec::kid
     std::vector<uint32_t> kids;
