@@ -411,9 +411,7 @@ class Engine {
             if (show_waves_[p] == 0)
                 throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
             show_lds_[p] = (size_t)show_wave_[p] * show_waves_[p];
-            size_t nho = plan_.nh[1 - p];
-            size_t per_wave = ((nho * 4 + 52 * 8 + 15) / 16) * 16;
-            fold_lds_[p] = per_wave * kWaves;
+            fold_lds_[p] = kFoldWaveBytes * kWaves;
         }
         size_t lds_max = std::max(std::max(show_lds_[0], show_lds_[1]), std::max(fold_lds_[0], fold_lds_[1]));
         if (lds_max > 160 * 1024)
@@ -567,7 +565,8 @@ class Engine {
                 launch(PK_UP_FOLD, k_up_fold, wave_grid(D.n_fold), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
                        D.n_fold, p);
             if (D.n_show)
-                launch_b(PK_UP_SHOW, k_up_show, wave_grid(D.n_show, show_waves_[p]), 64 * show_waves_[p],
+                launch_b(PK_UP_SHOW, k_up_show,
+                         wave_grid((D.n_show + kNodesPerWave - 1) / kNodesPerWave, show_waves_[p]), 64 * show_waves_[p],
                          show_lds_[p], T_, S_, (const ShowRec*)D.show, D.n_show, p, show_wave_[p]);
             if (D.n_chance)
                 launch(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), 0, T_, S_, (const ChanceRec*)D.chance,

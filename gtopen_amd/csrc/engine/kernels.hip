@@ -353,6 +353,9 @@ __device__ __forceinline__ uint32_t wave_item() {
 // Hands per lane whose loads are issued together in the hand loops (a
 // load's first use otherwise waits for it before the next one is issued).
 constexpr int kBatch = 4;
+// Terminal nodes per wavefront in the fold and showdown kernels: static
+// per-player / per-board tables stay in registers across the run.
+constexpr uint32_t kNodesPerWave = 8;
 
 // Orders this wavefront's LDS writes before its later LDS reads.
 __device__ __forceinline__ void wave_sync() {
@@ -528,28 +531,11 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
 // order), one lane per card, over the card's hand list fetched into
 // registers first. LDS per wavefront: nh_o floats + 52 doubles.
 // ---------------------------------------------------------------------------
-constexpr int kMaxHandList = 56;  // hands holding one card: at most 51 (a multiple of 8)
 constexpr int kMaxH = 21;         // hand rows per lane: 21 * 64 >= 1326
+constexpr size_t kFoldWaveBytes = 52 * 8 + 1328 * 4;  // per-card sums + combo-indexed row
 
-// In-order f64 sum of the first `len` of this lane's LDS values vals[idx[t]]
-// (idx packed two per word), reading 8 at a time ahead of the adds.
-template <int N>  // N: a multiple of 8
-__device__ __forceinline__ double ordered_sum(const float* vals, const uint32_t (&idx2)[N / 2], uint32_t len) {
-    static_assert(N % 8 == 0, "ordered_sum reads 8 entries at a time");
-    double acc = 0.0;
-#pragma unroll
-    for (int t0 = 0; t0 < N; t0 += 8) {
-        float v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const uint32_t w2 = idx2[(t0 + u) / 2];
-            v[u] = vals[((t0 + u) & 1) ? (w2 >> 16) : (w2 & 0xFFFFu)];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc += (t0 + u) < (int)len ? (double)v[u] : 0.0;
-    }
-    return acc;
-}
+// combo_index (cards.rs:100-103) of two distinct cards, hi > lo
+__device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return hi * (hi - 1) / 2 + lo; }
 
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
                                                     uint32_t count, int p) {
@@ -560,50 +546,65 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     if (w >= count) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
-    // static per-card hand lists and packed hand words: independent of the
-    // node, issued together with the record load
-    uint32_t len = 0, k0 = 0;
-    if (lane < 52) {
-        k0 = T.hc_off[o][lane];
-        len = T.hc_off[o][lane + 1] - k0;
+    // packed hand words (cards, identical combo) of both players: static
+    const uint32_t* __restrict__ hpp = T.hpack[p];
+    const uint32_t* __restrict__ hpo = T.hpack[o];
+    uint32_t hw[kMaxH], ho[kMaxH];
+#pragma unroll
+    for (int u = 0; u < kMaxH; ++u) {
+        if (u * kWave < (int)nhp) hw[u] = hpp[min(lane + u * kWave, nhp - 1)];
+        if (u * kWave < (int)nho) ho[u] = hpo[min(lane + u * kWave, nho - 1)];
     }
     const FoldRec r = recs[w];
     float* out = S.cfv + (size_t)r.cfv * T.nh_max;
     const uint32_t oslot = pick(r.own, o);
-    const uint32_t* __restrict__ hp = T.hpack[p];
-    uint32_t hw[kMaxH];
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u)
-        if (u * kWave < (int)nhp) hw[u] = hp[min(lane + u * kWave, nhp - 1)];
-    // entry-major lists: entry t of every card is one contiguous 52-wide row
-    const uint16_t* hsrc = T.hc_t[o] + min(lane, 51u);
-    uint32_t idx2[kMaxHandList / 2];
-#pragma unroll
-    for (int t = 0; t < kMaxHandList; t += 2)
-        idx2[t / 2] = (uint32_t)hsrc[t * 52] | ((uint32_t)hsrc[(t + 1) * 52] << 16);
     if (!S.alive[o][oslot]) {
         for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
         return;
     }
-    const size_t per_wave = ((nho * 4 + 52 * 8 + 15) / 16) * 16;
-    char* base = (char*)lds + wave * per_wave;
-    double* s_card = (double*)base;
-    float* rs = (float*)(base + 52 * 8);
     const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
     float rh[kMaxH];
 #pragma unroll
     for (int u = 0; u < kMaxH; ++u)
         if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
+    // LDS per wavefront: the opponent row indexed by combo (0 where the
+    // opponent has no such hand) and the 52 per-card sums
+    const size_t per_wave = kFoldWaveBytes;
+    char* base = (char*)lds + wave * per_wave;
+    double* s_card = (double*)base;
+    float* rsc = (float*)(base + 52 * 8);
+    for (uint32_t k = lane; k < 1326; k += kWave) rsc[k] = 0.0f;
+    wave_sync();
     double part = 0.0;
 #pragma unroll
     for (int u = 0; u < kMaxH; ++u)
         if (lane + u * kWave < nho) {
-            rs[lane + u * kWave] = rh[u];
+            const uint32_t h = ho[u];
+            rsc[combo_of(h & 0xFFu, (h >> 8) & 0xFFu)] = rh[u];
             part += (double)rh[u];
         }
     const double tot = wave_sum_f64(part);
     wave_sync();
-    if (lane < 52) s_card[lane] = ordered_sum<kMaxHandList>(rs, idx2, len);
+    // card c's hands in hand (combo) order: (c, x) for x < c, then (x, c)
+    // for x > c -- the order the CPU accumulates them in (game.rs:478-485);
+    // absent combos contribute +0.0, which leaves an f64 sum unchanged
+    if (lane < 52) {
+        const uint32_t c = lane;
+        double acc = 0.0;
+#pragma unroll
+        for (int t0 = 0; t0 < 56; t0 += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t t = min((uint32_t)(t0 + u), 50u);
+                const uint32_t x = t < c ? t : t + 1;
+                v[u] = rsc[x < c ? c * (c - 1) / 2 + x : x * (x - 1) / 2 + c];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += (t0 + u) < 51 ? (double)v[u] : 0.0;
+        }
+        s_card[c] = acc;
+    }
     wave_sync();
     const float amount = r.folder == (uint32_t)p ? r.t_lose : r.t_win;
 #pragma unroll
@@ -612,9 +613,9 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         if (u * kWave >= (int)nhp) break;
         if (i < nhp) {
             const uint32_t h = hw[u];
-            const uint32_t sm = h >> 16;
-            const float same_r = sm != 0xFFFFu ? rs[sm] : 0.0f;
-            const float valid = (float)(tot - s_card[h & 0xFFu] - s_card[(h >> 8) & 0xFFu]) + same_r;
+            const uint32_t a1 = h & 0xFFu, a2 = (h >> 8) & 0xFFu;
+            const float same_r = rsc[combo_of(a1, a2)];
+            const float valid = (float)(tot - s_card[a1] - s_card[a2]) + same_r;
             out[i] = amount * valid;
         }
     }
@@ -632,139 +633,153 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 // hands on the board.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
+constexpr int kShowBatch = 8;     // per-hand table entries in flight per lane
 static_assert(kMaxCardList % 8 == 0, "card lists are read 8 entries at a time");
 
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     uint32_t count, int p, uint32_t per_wave) {
     extern __shared__ double lds[];
-    const uint32_t w = wave_item();
-    if (w >= count) return;
+    const uint32_t first = wave_item() * kNodesPerWave;
+    if (first >= count) return;
+    const uint32_t end = min(first + kNodesPerWave, count);
     const int o = 1 - p;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
-    const ShowRec r = recs[w];
     const uint32_t nhp = T.nh[p], nho = T.nh[o];
-    float* out = S.cfv + (size_t)r.cfv * T.nh_max;
-    const uint32_t oslot = pick(r.own, o);
-    const uint32_t b = r.board;
-    // node and board scalars (one round trip)
-    const bool alive = S.alive[o][oslot];
-    const uint32_t mo = T.sd_cnt[o][b];
-    const uint32_t ooff = T.sd_off[o][b];
-    const uint32_t* __restrict__ coff = T.sd_coff[o] + (size_t)b * 53;
-    uint32_t c0 = 0, c1 = 0;
-    if (lane < 52) {
-        c0 = coff[lane];
-        c1 = coff[lane + 1];
-    }
-    if (!alive || mo == 0) {  // no reach, or no opponent hand on this board: all values 0
-        for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
-        return;
-    }
-    // the opponent row in hand order, the board's strength order and this
-    // lane's card list, all in flight together
-    const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
-    const uint16_t* __restrict__ oidx = T.sd_idx[o] + ooff;
-    float rh[kMaxH];
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u)
-        if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
-    uint32_t oi[kMaxH];
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u)
-        if (u * kWave < (int)mo) oi[u] = oidx[min(lane + u * kWave, mo - 1)];
-    const uint32_t len = c1 - c0;
-    // entry-major lists: entry t of every card is one contiguous 52-wide row
-    const uint16_t* cp_src = T.sd_cpos_t[o] + (size_t)b * (kMaxCardList * 52) + min(lane, 51u);
-    uint32_t pos2[kMaxCardList / 2];
-#pragma unroll
-    for (int t = 0; t < kMaxCardList; t += 2)
-        pos2[t / 2] = (uint32_t)cp_src[t * 52] | ((uint32_t)cp_src[(t + 1) * 52] << 16);
-    // LDS: P (mo+1 doubles) | card prefixes (max(2mo+52, nho/2) doubles; the
-    // hand-order row is staged there first) | rs (mo floats)
-    const uint32_t creg = max(2 * mo + 52, (nho + 1) / 2);
     char* base = (char*)lds + (size_t)wave * per_wave;
-    double* P = (double*)base;
-    double* cpre = P + (mo + 1);
-    float* rs = (float*)(cpre + creg);
-    float* rhs = (float*)cpre;
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u)
-        if (lane + u * kWave < nho) rhs[lane + u * kWave] = rh[u];
-    wave_sync();
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u)
-        if (lane + u * kWave < mo) rs[lane + u * kWave] = rhs[oi[u]];
-    wave_sync();
-    // exclusive prefix over strength order: lane-contiguous chunks + a
-    // wavefront scan of the chunk totals
-    const uint32_t ch = (mo + kWave - 1) / kWave;
-    const uint32_t lo = min(lane * ch, mo), hi = min(lo + ch, mo);
-    double part = 0.0;
-    for (uint32_t k = lo; k < hi; ++k) part += (double)rs[k];
-    double incl = part;
-    for (int off = 1; off < kWave; off <<= 1) {
-        double y = __shfl_up(incl, off);
-        if ((int)lane >= off) incl += y;
-    }
-    double run = __shfl_up(incl, 1);
-    if (lane == 0) run = 0.0;
-    const double total = __shfl(incl, kWave - 1);
-    for (uint32_t k = lo; k < hi; ++k) {
-        P[k] = run;
-        run += (double)rs[k];
-    }
-    if (lane == 0) P[mo] = total;
-    // per-card exclusive prefix in strength order (the CPU's order), one
-    // lane per card
-    if (lane < 52) {
-        double* cp = cpre + c0 + lane;
-        double a = 0.0;
-#pragma unroll
-        for (int t0 = 0; t0 < kMaxCardList; t0 += 8) {
-            float v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const uint32_t w2 = pos2[(t0 + u) / 2];
-                v[u] = rs[((t0 + u) & 1) ? (w2 >> 16) : (w2 & 0xFFFFu)];
+    // per-board static data, kept in registers across this wavefront's run
+    // of nodes (showdown lists are sorted by board) and reloaded only when
+    // the board changes; the 16-byte per-hand entries are loaded per node
+    uint32_t cur_b = 0xFFFFFFFFu, mo = 0, c0 = 0, len = 0;
+    uint32_t oi[kMaxH];
+    uint32_t pos2[kMaxCardList / 2];
+#pragma unroll 1
+    for (uint32_t w = first; w < end; ++w) {
+        const ShowRec r = recs[w];
+        float* out = S.cfv + (size_t)r.cfv * T.nh_max;
+        const uint32_t oslot = pick(r.own, o);
+        const uint32_t b = r.board;
+        if (b != cur_b) {
+            cur_b = b;
+            mo = T.sd_cnt[o][b];
+            const uint16_t* __restrict__ oidx = T.sd_idx[o] + T.sd_off[o][b];
+            const uint32_t* __restrict__ coff = T.sd_coff[o] + (size_t)b * 53;
+            c0 = 0;
+            len = 0;
+            if (lane < 52) {
+                c0 = coff[lane];
+                len = coff[lane + 1] - c0;
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (t0 + u < (int)len) cp[t0 + u] = a;
-                a += (t0 + u) < (int)len ? (double)v[u] : 0.0;
-            }
+            for (int u = 0; u < kMaxH; ++u)
+                if (u * kWave < (int)mo) oi[u] = oidx[min(lane + u * kWave, mo - 1)];
+            const uint16_t* cp_src = T.sd_cpos_t[o] + (size_t)b * (kMaxCardList * 52) + min(lane, 51u);
+#pragma unroll
+            for (int t = 0; t < kMaxCardList; t += 2)
+                pos2[t / 2] = (uint32_t)cp_src[t * 52] | ((uint32_t)cp_src[(t + 1) * 52] << 16);
         }
-        cp[len] = a;
-    }
-    wave_sync();
-    const float win = r.win, lose = r.lose, tie = r.tie;
-    const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
-    for (uint32_t i0 = lane; i0 < nhp; i0 += 8 * kWave) {
-        uint4 ent[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) ent[u] = tab[min(i0 + u * kWave, nhp - 1)];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const uint32_t i = i0 + u * kWave;
-            if (i >= nhp) break;
-            const uint4 e = ent[u];
-            if (e.x == 0xFFFFFFFFu) {
-                out[i] = 0.0f;
-                continue;
-            }
-            const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
-            const double* cp1 = cpre + (e.y & 0xFFFFu);
-            const double* cp2 = cpre + (e.y >> 16);
-            const double S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
-            const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
-            const float higher =
-                (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
-            const uint32_t sp = e.w >> 16;
-            const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
-            const float valid = (float)(total - S1 - S2) + same_r;
-            const float tied = valid - lower - higher;
-            out[i] = win * lower + lose * higher + tie * tied;
+        if (!S.alive[o][oslot] || mo == 0) {  // no reach, or no opponent hand on this board: all 0
+            for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
+            continue;
         }
+        const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
+        float rh[kMaxH];
+#pragma unroll
+        for (int u = 0; u < kMaxH; ++u)
+            if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
+        // LDS: P (mo+1 doubles) | card prefixes (max(2mo+52, nho/2) doubles;
+        // the hand-order row is staged there first) | rs (mo floats)
+        const uint32_t creg = max(2 * mo + 52, (nho + 1) / 2);
+        double* P = (double*)base;
+        double* cpre = P + (mo + 1);
+        float* rs = (float*)(cpre + creg);
+        float* rhs = (float*)cpre;
+#pragma unroll
+        for (int u = 0; u < kMaxH; ++u)
+            if (lane + u * kWave < nho) rhs[lane + u * kWave] = rh[u];
+        // the first batch of this node's per-hand entries: in flight during
+        // the LDS phases
+        const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
+        uint4 ent[kShowBatch];
+#pragma unroll
+        for (int u = 0; u < kShowBatch; ++u) ent[u] = tab[min(lane + u * kWave, nhp - 1)];
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < kMaxH; ++u)
+            if (lane + u * kWave < mo) rs[lane + u * kWave] = rhs[oi[u]];
+        wave_sync();
+        // exclusive prefix over strength order: lane-contiguous chunks + a
+        // wavefront scan of the chunk totals
+        const uint32_t ch = (mo + kWave - 1) / kWave;
+        const uint32_t lo = min(lane * ch, mo), hi = min(lo + ch, mo);
+        double part = 0.0;
+        for (uint32_t k = lo; k < hi; ++k) part += (double)rs[k];
+        double incl = part;
+        for (int off = 1; off < kWave; off <<= 1) {
+            double y = __shfl_up(incl, off);
+            if ((int)lane >= off) incl += y;
+        }
+        double run = __shfl_up(incl, 1);
+        if (lane == 0) run = 0.0;
+        const double total = __shfl(incl, kWave - 1);
+        for (uint32_t k = lo; k < hi; ++k) {
+            P[k] = run;
+            run += (double)rs[k];
+        }
+        if (lane == 0) P[mo] = total;
+        // per-card exclusive prefix in strength order (the CPU's order), one
+        // lane per card
+        if (lane < 52) {
+            double* cp = cpre + c0 + lane;
+            double a = 0.0;
+#pragma unroll
+            for (int t0 = 0; t0 < kMaxCardList; t0 += 8) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t w2 = pos2[(t0 + u) / 2];
+                    v[u] = rs[((t0 + u) & 1) ? (w2 >> 16) : (w2 & 0xFFFFu)];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (t0 + u < (int)len) cp[t0 + u] = a;
+                    a += (t0 + u) < (int)len ? (double)v[u] : 0.0;
+                }
+            }
+            cp[len] = a;
+        }
+        wave_sync();
+        const float win = r.win, lose = r.lose, tie = r.tie;
+#pragma unroll 1
+        for (uint32_t i0 = lane;; i0 += kShowBatch * kWave) {
+#pragma unroll
+            for (int u = 0; u < kShowBatch; ++u) {
+                const uint32_t i = i0 + u * kWave;
+                if (i >= nhp) break;
+                const uint4 e = ent[u];
+                if (e.x == 0xFFFFFFFFu) {
+                    out[i] = 0.0f;
+                    continue;
+                }
+                const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
+                const double* cp1 = cpre + (e.y & 0xFFFFu);
+                const double* cp2 = cpre + (e.y >> 16);
+                const double S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
+                const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
+                const float higher =
+                    (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
+                const uint32_t sp = e.w >> 16;
+                const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
+                const float valid = (float)(total - S1 - S2) + same_r;
+                const float tied = valid - lower - higher;
+                out[i] = win * lower + lose * higher + tie * tied;
+            }
+            const uint32_t n0 = i0 + kShowBatch * kWave;
+            if (n0 - lane >= nhp) break;
+#pragma unroll
+            for (int u = 0; u < kShowBatch; ++u) ent[u] = tab[min(n0 + u * kWave, nhp - 1)];
+        }
+        wave_sync();  // this node's LDS reads complete before the next node's writes
     }
 }
 
