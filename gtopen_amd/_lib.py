@@ -101,6 +101,8 @@ SIGNATURES = [
     ("gto_hip_upload_arenas", ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p, c_f32p, c_f32p]),
     ("gto_hip_download_compressed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int16), c_u16p, c_f32p, c_f32p]),
     ("gto_hip_update_locks", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_f32p, ctypes.c_uint64]),
+    ("gto_hip_set_weights", ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p]),
+    ("gto_hip_game_value", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f64p, c_f32p, c_f32p, c_f32p, c_f32p]),
     ("gto_hip_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("gto_hip_profile_kind_name", ctypes.c_char_p, [ctypes.c_int]),
     ("gto_hip_profile_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(HipProfile)]),

@@ -171,6 +171,60 @@ class Engine {
         HIP_OK(hipStreamSynchronize(stream_));
     }
 
+    // Range weights (the root reach); the spot's hands stay as built, and a
+    // zero weight leaves its hand with zero reach.
+    void set_weights(int q, const float* w) {
+        if (!w) return;
+        const uint32_t nh = plan_.nh[q];
+        for (uint32_t i = 0; i < nh; ++i)
+            if (!(w[i] >= 0.0f) || !std::isfinite(w[i]))
+                throw std::runtime_error("range weights must be finite and >= 0");
+        HIP_OK(hipStreamSynchronize(stream_));
+        HIP_OK(hipMemcpy((void*)T_.weights[q], w, sizeof(float) * nh, hipMemcpyHostToDevice));
+        weights_[q].assign(w, w + nh);
+        reach_valid_[0] = reach_valid_[1] = false;
+    }
+
+    // Game value of player p under the behavioural strategy held in the
+    // strategy arenas (sigma as stored, no normalisation):
+    //   U_p = sum_i w_p[i] * cfv_p(root)[i]   (best_response.rs:264-269)
+    // and its gradient. U_p is linear in each range and in each node's
+    // strategy, so dU/dw_p = cfv_p(root), dU/dw_o = the root values of o's
+    // hands under p's payoffs, and dU/dsigma_q(n, a, i) = reach_q(n, i) *
+    // chance weight * value of child a (k_grad_sigma).
+    void game_value(int p, double* U, float* gw[2], float* gs[2]) {
+        if (compressed_) throw std::runtime_error("game values need an engine with f32 arenas");
+        const int o = 1 - p;
+        launch_root(3);
+        launch_down(3, kRawF32);
+        reach_valid_[0] = reach_valid_[1] = false;
+        std::vector<float> c;
+        for (int pass = 0; pass < 2; ++pass) {
+            const int t = pass == 0 ? p : o;  // traverser of this sweep
+            if (pass == 1 && !gw[o] && !gs[o]) break;
+            launch_up(t, 4, pass);
+            c.resize(plan_.nh[t]);
+            HIP_OK(hipMemcpyAsync(c.data(), S_.cfv + (size_t)plan_.cfv_slot[0] * plan_.nh_max,
+                                  sizeof(float) * plan_.nh[t], hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            if (pass == 0 && U) *U = dot(p, c);
+            if (gw[t]) std::copy(c.begin(), c.end(), gw[t]);
+            if (gs[t] && data_size_[t]) {
+                if (!grad_buf_) grad_buf_ = (float*)dalloc(sizeof(float) * std::max(data_size_[0], data_size_[1]));
+                HIP_OK(hipMemsetAsync(grad_buf_, 0, sizeof(float) * data_size_[t], stream_));
+                for (size_t l = 0; l < lv_.size(); ++l) {
+                    const DevLevel& D = lv_[l];
+                    if (!D.n_act[t]) continue;
+                    launch(PK_OTHER, k_grad_sigma, wave_grid(D.n_act[t]), 0, T_, S_, (const ActRec*)D.act[t],
+                           D.n_act[t], t, grad_buf_);
+                }
+                HIP_OK(hipMemcpyAsync(gs[t], grad_buf_, sizeof(float) * data_size_[t], hipMemcpyDeviceToHost,
+                                      stream_));
+                HIP_OK(hipStreamSynchronize(stream_));
+            }
+        }
+    }
+
     void download(int which, int p, float* dst) {
         if (!dst || data_size_[p] == 0) return;
         if (!compressed_) {
@@ -335,6 +389,7 @@ class Engine {
     uint32_t disc_cap_ = 0;
     float* disc_buf_ = nullptr;
     float* lock_sigma_buf_ = nullptr;
+    float* grad_buf_ = nullptr;
     size_t show_lds_[2] = {0, 0};
     uint32_t show_wave_[2] = {0, 0};
     uint32_t show_waves_[2] = {kWaves, kWaves};
@@ -543,7 +598,9 @@ class Engine {
                 if (!((mask >> q) & 1) || !D.n_act[q]) continue;
                 const ActRec* recs = D.act[q];
                 const uint32_t n = D.n_act[q], g = wave_grid(n);
-                if (kind == kAverageF32) {
+                if (kind == kRawF32) {
+                    launch(PK_DOWN_ACTION, k_down_action<kRawF32>, g, 0, T_, S_, recs, n, q);
+                } else if (kind == kAverageF32) {
                     if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, g, 0, T_, S_, recs, n, q);
                     else launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, g, 0, T_, S_, recs, n, q);
                 } else {
@@ -557,17 +614,19 @@ class Engine {
         }
     }
 
-    // mode 0 = DCFR update, 1 = best response, 2 = average values, 3 = current values
-    void launch_up(int p, int mode) {
+    // mode 0 = DCFR update, 1 = best response, 2 = average values, 3 = current
+    // values, 4 = values of the strategy arena as given (f32 only); swap:
+    // terminal payoffs of the opponent instead of the traverser
+    void launch_up(int p, int mode, int swap = 0) {
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
             if (D.n_fold)
                 launch(PK_UP_FOLD, k_up_fold, wave_grid(D.n_fold), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
-                       D.n_fold, p);
+                       D.n_fold, p, swap);
             if (D.n_show)
                 launch_b(PK_UP_SHOW, k_up_show,
                          wave_grid((D.n_show + kNodesPerWave - 1) / kNodesPerWave, show_waves_[p]), 64 * show_waves_[p],
-                         show_lds_[p], T_, S_, (const ShowRec*)D.show, D.n_show, p, show_wave_[p]);
+                         show_lds_[p], T_, S_, (const ShowRec*)D.show, D.n_show, p, show_wave_[p], swap);
             if (D.n_chance)
                 launch(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), 0, T_, S_, (const ChanceRec*)D.chance,
                        D.n_chance, p);
@@ -587,6 +646,7 @@ class Engine {
                         case 0: launch(PK_UP_ACTION, k_up_action<0, false>, g, 0, T_, S_, recs, n, p, q); break;
                         case 1: launch(PK_UP_ACTION, k_up_action<1, false>, g, 0, T_, S_, recs, n, p, q); break;
                         case 2: launch(PK_UP_ACTION, k_up_action<2, false>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 4: launch(PK_UP_ACTION, k_up_action<4, false>, g, 0, T_, S_, recs, n, p, q); break;
                         default: launch(PK_UP_ACTION, k_up_action<3, false>, g, 0, T_, S_, recs, n, p, q); break;
                     }
                 }
@@ -727,6 +787,30 @@ int gto_hip_root_cfv(gto_hip* h, int p, int mode, float* out) {
         std::vector<float> c;
         h->e->root_cfv(p, mode, c);
         std::copy(c.begin(), c.end(), out);
+    });
+}
+
+int gto_hip_set_weights(gto_hip* h, const float* w0, const float* w1) {
+    if (!h) {
+        set_last_error("gto_hip_set_weights: null handle");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        h->e->set_weights(0, w0);
+        h->e->set_weights(1, w1);
+    });
+}
+
+int gto_hip_game_value(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1, float* grad_s0,
+                       float* grad_s1) {
+    if (!h || player < 0 || player > 1) {
+        set_last_error("gto_hip_game_value: null handle or bad player");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        float* gw[2] = {grad_w0, grad_w1};
+        float* gs[2] = {grad_s0, grad_s1};
+        h->e->game_value(player, value, gw, gs);
     });
 }
 

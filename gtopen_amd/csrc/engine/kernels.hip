@@ -64,7 +64,9 @@ __device__ __forceinline__ uint16_t quant_u16(float x) {
 //            (cfr.rs:704-722).
 // Values of the first kMaxUnroll actions are kept in registers.
 // ---------------------------------------------------------------------------
-enum SigmaKind { kCurrentF32 = 0, kAverageF32 = 1, kCurrentQ = 2, kAverageQ = 3 };
+//   RAW:     the strategy arena read as a behavioural strategy, sigma = S
+//            as stored (the game-value pass, whose strategy is an input).
+enum SigmaKind { kCurrentF32 = 0, kAverageF32 = 1, kCurrentQ = 2, kAverageQ = 3, kRawF32 = 4 };
 
 template <int KIND>
 struct SigmaHand;
@@ -194,6 +196,19 @@ struct SigmaHand<kAverageQ> {
     __device__ float sigma(int a) const { return uniform ? u : (float)raw<C>(a) * inv; }
 };
 
+template <>
+struct SigmaHand<kRawF32> {
+    const float* base;
+    uint32_t nh, i;
+    __device__ void init(const float* b, uint32_t nh_, uint32_t i_, int) {
+        base = b;
+        nh = nh_;
+        i = i_;
+    }
+    template <bool C>
+    __device__ float sigma(int a) const { return base[(size_t)a * nh + i]; }
+};
+
 // ---------------------------------------------------------------------------
 // Strategy of one hand at a node with a compile-time action count NA: all
 // NA arena values are loaded before any arithmetic (no per-action
@@ -298,6 +313,17 @@ struct HandSigma<NA, kAverageQ> {
     }
 };
 
+template <int NA>
+struct HandSigma<NA, kRawF32> {
+    float sig[NA];
+    __device__ __forceinline__ void fetch(const float* __restrict__ b, uint32_t nh, uint32_t i) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sig[a] = b[(size_t)a * nh + i];
+    }
+    __device__ __forceinline__ void finish() {}
+    __device__ __forceinline__ void load(const float* __restrict__ b, uint32_t nh, uint32_t i) { fetch(b, nh, i); }
+};
+
 // Calls f(std::integral_constant<int, NA>) for na in 1..8, f(0) otherwise
 // (runtime action count).
 template <class F>
@@ -332,6 +358,10 @@ struct ArenaOf<kCurrentQ> {
 template <>
 struct ArenaOf<kAverageQ> {
     static __device__ const uint16_t* get(const DevState& S, int q) { return S.Sq[q]; }
+};
+template <>
+struct ArenaOf<kRawF32> {
+    static __device__ const float* get(const DevState& S, int q) { return S.S[q]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -538,7 +568,7 @@ constexpr size_t kFoldWaveBytes = 52 * 8 + 1328 * 4;  // per-card sums + combo-i
 __device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return hi * (hi - 1) / 2 + lo; }
 
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
-                                                    uint32_t count, int p) {
+                                                    uint32_t count, int p, int swap) {
     extern __shared__ double lds[];
     const int o = 1 - p;
     const uint32_t nho = T.nh[o], nhp = T.nh[p];
@@ -606,7 +636,9 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         s_card[c] = acc;
     }
     wave_sync();
-    const float amount = r.folder == (uint32_t)p ? r.t_lose : r.t_win;
+    // swap: the opponent's payoff (win and lose exchanged), for game-value
+    // gradients with respect to the opponent's range and strategy
+    const float amount = (r.folder == (uint32_t)p) != (swap != 0) ? r.t_lose : r.t_win;
 #pragma unroll
     for (int u = 0; u < kMaxH; ++u) {
         const uint32_t i = lane + u * kWave;
@@ -637,7 +669,7 @@ constexpr int kShowBatch = 8;     // per-hand table entries in flight per lane
 static_assert(kMaxCardList % 8 == 0, "card lists are read 8 entries at a time");
 
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
-                                                    uint32_t count, int p, uint32_t per_wave) {
+                                                    uint32_t count, int p, uint32_t per_wave, int swap) {
     extern __shared__ double lds[];
     const uint32_t first = wave_item() * kNodesPerWave;
     if (first >= count) return;
@@ -749,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             cp[len] = a;
         }
         wave_sync();
-        const float win = r.win, lose = r.lose, tie = r.tie;
+        const float win = swap ? r.lose : r.win, lose = swap ? r.win : r.lose, tie = r.tie;
 #pragma unroll 1
         for (uint32_t i0 = lane;; i0 += kShowBatch * kWave) {
 #pragma unroll
@@ -836,6 +868,8 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
 //   MODE 2 (average strategy values): average-strategy weighting
 //     (best_response.rs:180-195).
 //   MODE 3 (current strategy values): regret-matched weighting, no update.
+//   MODE 4 (game value): the strategy arena taken as the behavioural
+//     strategy itself (kRawF32), no update.
 // Subtrees whose opponent reach is all zero produce zeros and no update.
 // ---------------------------------------------------------------------------
 template <int MODE, bool COMPRESSED>
@@ -910,9 +944,12 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
         return;
     }
     const uint64_t off = r.doff;
-    if constexpr (MODE == 2 || MODE == 3) {
-        // value of the average (2) or current (3) strategy, no update
-        constexpr int KIND = MODE == 2 ? (COMPRESSED ? kAverageQ : kAverageF32) : (COMPRESSED ? kCurrentQ : kCurrentF32);
+    if constexpr (MODE >= 2) {
+        // value of the average (2), current (3) or given (4) strategy, no update
+        static_assert(MODE != 4 || !COMPRESSED, "game values read an f32 strategy arena");
+        constexpr int KIND = MODE == 4   ? kRawF32
+                             : MODE == 2 ? (COMPRESSED ? kAverageQ : kAverageF32)
+                                         : (COMPRESSED ? kCurrentQ : kCurrentF32);
         const auto* arena = ArenaOf<KIND>::get(S, p) + off;
         dispatch_na(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
@@ -1082,6 +1119,31 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                 S.Ss[p][r.node] = sok ? SM : 0.0f;
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Game-value gradient with respect to actor q's behavioural strategy:
+// dU/dsigma(node, a, i) = reach_q(node, i) * (chance weight of the path) *
+// value of child a for hand i, read from the value rows of the sweep whose
+// traverser is q (payoffs: the game value's player). Locked nodes play a
+// fixed strategy: 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_grad_sigma(DevTree T, DevState S, const ActRec* __restrict__ recs,
+                                                       uint32_t count, int q, float* __restrict__ grad) {
+    const uint32_t w = wave_item();
+    if (w >= count) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const ActRec r = recs[w];
+    const uint32_t nh = T.nh[q];
+    const int na = r.na;
+    float* __restrict__ g = grad + r.doff;
+    const float* __restrict__ pr = S.reach[q] + (size_t)pick(r.own, q) * nh;
+    const uint32_t* kid_cfv = T.kids + r.kid;
+    const float k = r.lock_off >= 0 ? 0.0f : r.cw;
+    for (int a = 0; a < na; ++a) {
+        const float* __restrict__ v = S.cfv + (size_t)kid_cfv[a] * T.nh_max;
+        for (uint32_t i = lane; i < nh; i += kWave) g[(size_t)a * nh + i] = pr[i] * k * v[i];
     }
 }
 

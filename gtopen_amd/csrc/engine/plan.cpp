@@ -19,6 +19,7 @@ struct Item {
     uint8_t dealt[2];
     uint8_t ndealt;
     uint32_t src[2];
+    double cw;  // product of the chance probabilities on the path from the root
 };
 
 uint32_t river_key_of(const gto_spot_view& v, const Item& it) {
@@ -44,12 +45,13 @@ Plan build_plan(const gto_spot_view& v) {
     pl.show_slot.assign(N, kSent);
     std::vector<uint32_t> level(N, kSent);
     std::vector<uint32_t> key(N, kSent);
+    std::vector<float> cw(N, 0.0f);
 
     // depth-first walk over the reachable tree (chance nodes skip cards
     // already dealt, as cfr.rs:556-565 does)
     std::vector<Item> stack;
     pl.num_slots[0] = pl.num_slots[1] = 1;  // slot 0 = root reach
-    Item root{0, 0, {0, 0}, 0, {0, 0}};
+    Item root{0, 0, {0, 0}, 0, {0, 0}, 1.0};
     stack.push_back(root);
     uint32_t max_level = 0;
     std::vector<std::vector<uint32_t>> edges_p, edges_c, edges_card;
@@ -60,6 +62,7 @@ Plan build_plan(const gto_spot_view& v) {
         level[n] = it.level;
         pl.owner[0][n] = it.src[0];
         pl.owner[1][n] = it.src[1];
+        cw[n] = (float)it.cw;
         max_level = std::max(max_level, it.level);
         const uint8_t kind = v.node_kind[n];
         const uint32_t cs = v.node_children_start[n];
@@ -92,6 +95,7 @@ Plan build_plan(const gto_spot_view& v) {
                 k.dealt[k.ndealt++] = (uint8_t)c;
                 k.src[0] = pl.num_slots[0]++;
                 k.src[1] = pl.num_slots[1]++;
+                k.cw = it.cw / (double)(46 - (int)v.node_street[n]);
                 kids.push_back(k);
                 if (edges_p.size() <= it.level) {
                     edges_p.resize(it.level + 1);
@@ -273,6 +277,7 @@ Plan build_plan(const gto_spot_view& v) {
                 r.cfv = pl.cfv_slot[n];
                 r.na = v.node_num_children[n];
                 r.kid = (uint32_t)pl.kids.size();
+                r.cw = cw[n];
                 pl.max_actions = std::max(pl.max_actions, r.na);
                 const uint32_t cs = v.node_children_start[n];
                 for (uint32_t a = 0; a < r.na; ++a) pl.kids.push_back(pl.cfv_slot[v.children[cs + a]]);

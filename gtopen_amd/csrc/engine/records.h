@@ -18,7 +18,8 @@ struct ActRec {
     uint32_t cfv;       // value row of the node
     uint32_t na;        // number of actions
     uint32_t kid;       // index into the kids array
-    uint32_t pad[2];
+    float cw;           // product of the chance probabilities above the node
+    uint32_t pad;
 };
 static_assert(sizeof(ActRec) == 48, "ActRec layout");
 

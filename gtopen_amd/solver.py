@@ -74,10 +74,16 @@ class GpuSolver:
         return r, s
 
     def upload_arenas(self, regrets, strat):
-        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (regrets[0], regrets[1], strat[0], strat[1])]
-        L.check(
-            L.lib().gto_hip_upload_arenas(self._h, *[a.ctypes.data_as(L.c_f32p) for a in arrs]), "upload arenas"
-        )
+        """f32 arenas (None for an arena, or for a pair, keeps it)."""
+        regrets = regrets if regrets is not None else (None, None)
+        strat = strat if strat is not None else (None, None)
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+                for a in (regrets[0], regrets[1], strat[0], strat[1])]
+        for k, a in enumerate(arrs):
+            if a is not None and a.size != self.spot.data_size[k % 2]:
+                raise ValueError(f"arena {k} has {a.size} entries, expected {self.spot.data_size[k % 2]}")
+        ptrs = [None if a is None else a.ctypes.data_as(L.c_f32p) for a in arrs]
+        L.check(L.lib().gto_hip_upload_arenas(self._h, *ptrs), "upload arenas")
 
     def update_locks(self, locks):
         """locks: {node_index: sigma (na*nh float array)} (gpu/mod.rs:894-905)."""

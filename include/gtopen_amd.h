@@ -188,6 +188,31 @@ int gto_hip_upload_arenas(gto_hip* h, const float* regrets0, const float* regret
 int gto_hip_download_compressed(gto_hip* h, int p, int16_t* regrets_q, uint16_t* strat_q,
                                 float* regret_scale, float* strat_scale);
 
+/* Range weights per hand (the root reach), replacing the spot's; NULL keeps
+ * a player's weights. The hand lists stay as built: a weight of 0 leaves the
+ * hand in place with zero reach (the reference's builder would drop it,
+ * game.rs:177-180, which changes no value). Weights must be finite, >= 0. */
+int gto_hip_set_weights(gto_hip* h, const float* w0, const float* w1);
+
+/* Differentiable game value (the forward pass is traverse_avg,
+ * best_response.rs:128-223, with the strategy given rather than averaged):
+ * the strategy arenas (upload them with gto_hip_upload_arenas(h, NULL, NULL,
+ * s0, s1)) are read as the behavioural strategy sigma itself, without
+ * normalisation, and
+ *   value = U_player = sum_i w_player[i] * cfv_player(root)[i]
+ * (best_response.rs:264-269; chips, not divided by the pair weight sum).
+ * Gradients (any pointer may be NULL to skip it):
+ *   grad_w0/grad_w1 [num_hands[q]]: dU/dw_q; for q = player this is the
+ *     root counterfactual value cfv_player(root), for the opponent it is the
+ *     root value of the opponent's hands under the player's payoffs;
+ *   grad_s0/grad_s1 [data_size[q]] (arena layout): dU/dsigma_q(node,a,i) =
+ *     reach_q(node,i) * chance weight of the path * value of child a; 0 at
+ *     locked nodes and nodes outside the reachable tree.
+ * The reference has no backward; its CFR values are the reference gradient.
+ * F32 storage only. */
+int gto_hip_game_value(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
+                       float* grad_s0, float* grad_s1);
+
 /* Node locks: lock_off[num_nodes] (-1 = unlocked, else offset into sigma),
  * sigma = concatenated na*nh_actor blocks. Replaces update_locks /
  * build_lock_table (gpu/mod.rs:894-905, 961-973). n_sigma may be 0. */

@@ -77,10 +77,39 @@ class Oracle:
     storage 'f32' or 'compressed' (i16/u16 arenas, precision 32 only).
     """
 
-    def __init__(self, spot, precision=32, storage="f32", algo="dcfr"):
+    def __init__(self, spot, precision=32, storage="f32", algo="dcfr", weights=None):
+        """weights=(w0, w1): per-hand range weights replacing the spot's (the
+        engine reads them through a private copy of the spot's view, so
+        set_weights can change them later)."""
         self.spot = spot
-        self._h = lib().oracle_create(spot.view_ptr, precision, 0 if storage == "f32" else 1,
+        view_ptr = spot.view_ptr
+        if weights is not None:
+            from gtopen_amd import _lib as G
+
+            self._w = [np.array(weights[q], dtype=np.float32).reshape(spot.num_hands[q]).copy() for q in range(2)]
+            self._view = G.SpotView.from_buffer_copy(spot.view_ptr.contents)
+            for q in range(2):
+                self._view.weights[q] = self._w[q].ctypes.data_as(G.c_f32p)
+            view_ptr = ctypes.pointer(self._view)
+        self._h = lib().oracle_create(view_ptr, precision, 0 if storage == "f32" else 1,
                                       0 if algo == "dcfr" else 1)
+
+    def set_weights(self, w0, w1):
+        if not hasattr(self, "_w"):
+            raise RuntimeError("create the oracle with weights=... to change them")
+        self._w[0][:] = w0
+        self._w[1][:] = w1
+
+    def game_value(self, p, sigma):
+        """U_p = sum_i w_p[i] cfv_p(root)[i] with the strategy arenas set to
+        sigma (per-player arena arrays, already normalised), average-strategy
+        traversal (best_response.rs:128-223, 264-269). Overwrites the
+        arenas."""
+        zeros = [np.zeros(self.spot.data_size[q], dtype=np.float32) for q in range(2)]
+        self.set_arenas(zeros, sigma)
+        c = self.root_cfv(p, mode=1)
+        w = self._w[p] if hasattr(self, "_w") else self.spot.weights[p]
+        return float(np.dot(w.astype(np.float64), c))
 
     def __del__(self):
         h = getattr(self, "_h", None)
