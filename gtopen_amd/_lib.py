@@ -102,6 +102,20 @@ SIGNATURES = [
     ("gto_hip_download_compressed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int16), c_u16p, c_f32p, c_f32p]),
     ("gto_hip_update_locks", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_f32p, ctypes.c_uint64]),
     ("gto_hip_set_weights", ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p]),
+    ("gto_save_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint32, c_f32p, c_f32p, c_f32p, c_f32p,
+                                      ctypes.c_uint32, c_u32p, c_u64p, c_u64p, c_f32p, ctypes.c_uint32, c_u32p,
+                                      ctypes.POINTER(ctypes.c_char_p)]),
+    ("gto_save_open", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("gto_save_close", None, [ctypes.c_void_p]),
+    ("gto_save_config_json", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("gto_save_iteration", ctypes.c_uint32, [ctypes.c_void_p]),
+    ("gto_save_num_locks", ctypes.c_uint32, [ctypes.c_void_p]),
+    ("gto_save_lock", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.POINTER(c_f32p), c_u64p]),
+    ("gto_save_num_labels", ctypes.c_uint32, [ctypes.c_void_p]),
+    ("gto_save_label", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.POINTER(ctypes.c_char_p)]),
+    ("gto_save_validate_locks", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("gto_save_read_arenas", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_f32p, c_f32p, c_f32p, c_f32p]),
+    ("gto_validate_locks", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_u64p, c_u64p, c_f32p]),
     ("gto_hip_game_value", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f64p, c_f32p, c_f32p, c_f32p, c_f32p]),
     ("gto_hip_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("gto_hip_profile_kind_name", ctypes.c_char_p, [ctypes.c_int]),

@@ -120,4 +120,17 @@ struct Spot {
 };
 std::unique_ptr<Spot> build_spot(const SpotConfig& cfg, uint64_t max_nodes, Strictness strictness);
 
+// ---- save.rs -------------------------------------------------------------
+struct SaveHeader {  // save.rs:17-28
+    SpotConfig config;
+    uint32_t iteration = 0;
+    std::vector<std::pair<uint32_t, std::vector<float>>> locks;
+    std::vector<std::pair<uint32_t, std::string>> lock_labels;
+};
+std::string spot_config_json(const SpotConfig& c);  // serde layout, every field
+std::string save_header_json(const SaveHeader& h);
+SaveHeader parse_save_header(const std::string& line);
+// save.rs:42-70; throws Error with the reference's messages
+void validate_locks(const Spot& spot, const std::vector<std::pair<uint32_t, std::vector<float>>>& locks);
+
 }  // namespace gto

@@ -146,6 +146,19 @@ class Parser {
                         if (i_ + 4 > s_.size()) fail("bad \\u escape");
                         unsigned cp = (unsigned)std::strtoul(s_.substr(i_, 4).c_str(), nullptr, 16);
                         i_ += 4;
+                        if (cp >= 0xD800 && cp < 0xDC00 && i_ + 6 <= s_.size() && s_[i_] == '\\' &&
+                            s_[i_ + 1] == 'u') {
+                            unsigned lo = (unsigned)std::strtoul(s_.substr(i_ + 2, 4).c_str(), nullptr, 16);
+                            if (lo >= 0xDC00 && lo < 0xE000) {
+                                i_ += 6;
+                                cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+                                out.push_back((char)(0xF0 | (cp >> 18)));
+                                out.push_back((char)(0x80 | ((cp >> 12) & 0x3F)));
+                                out.push_back((char)(0x80 | ((cp >> 6) & 0x3F)));
+                                out.push_back((char)(0x80 | (cp & 0x3F)));
+                                break;
+                            }
+                        }
                         if (cp < 0x80) out.push_back((char)cp);
                         else if (cp < 0x800) {
                             out.push_back((char)(0xC0 | (cp >> 6)));
@@ -213,10 +226,8 @@ void street_sizings(const JVal* v, StreetSizing out[3], const char* name) {
     }
 }
 
-}  // namespace
 
-SpotConfig parse_spot_config(const std::string& json) {
-    JVal root = Parser(json).parse();
+SpotConfig spot_config_of(const JVal& root) {
     if (root.type != JVal::Obj) throw Error("invalid config JSON: expected an object");
     SpotConfig c;
     c.board = str_req(root, "board");
@@ -240,6 +251,175 @@ SpotConfig parse_spot_config(const std::string& json) {
     street_sizings(t->get("oop"), tc.oop, "oop");
     street_sizings(t->get("ip"), tc.ip, "ip");
     return c;
+}
+
+// ---- writer ----------------------------------------------------------------
+// Numbers are written in their shortest round-trip decimal form (what
+// serde_json prints for f64/f32): integral f64 values keep a ".0", exponents
+// carry no '+' or leading zeros.
+template <class T>
+std::string shortest(T x) {
+    if (!std::isfinite((double)x)) throw Error("cannot write a non-finite number to JSON");
+    const int maxp = sizeof(T) == 4 ? 9 : 17;
+    char buf[40];
+    for (int p = 1; p <= maxp; ++p) {
+        std::snprintf(buf, sizeof buf, "%.*g", p, (double)x);
+        T back = sizeof(T) == 4 ? (T)std::strtof(buf, nullptr) : (T)std::strtod(buf, nullptr);
+        if (back == x) break;
+    }
+    std::string s(buf), mant = s, ex;
+    size_t e = s.find('e');
+    if (e != std::string::npos) {
+        mant = s.substr(0, e);
+        std::string d = s.substr(e + 1);
+        bool neg = d[0] == '-';
+        size_t k = (d[0] == '-' || d[0] == '+') ? 1 : 0;
+        while (k + 1 < d.size() && d[k] == '0') ++k;
+        ex = std::string("e") + (neg ? "-" : "") + d.substr(k);
+    }
+    if (ex.empty() && mant.find('.') == std::string::npos) mant += ".0";
+    return mant + ex;
+}
+
+void put_str(std::string& o, const std::string& s) {
+    o.push_back('"');
+    for (unsigned char c : s) {
+        switch (c) {
+            case '"': o += "\\\""; break;
+            case '\\': o += "\\\\"; break;
+            case '\n': o += "\\n"; break;
+            case '\r': o += "\\r"; break;
+            case '\t': o += "\\t"; break;
+            case '\b': o += "\\b"; break;
+            case '\f': o += "\\f"; break;
+            default:
+                if (c < 0x20) {
+                    char b[8];
+                    std::snprintf(b, sizeof b, "\\u%04x", c);
+                    o += b;
+                } else {
+                    o.push_back((char)c);
+                }
+        }
+    }
+    o.push_back('"');
+}
+
+void put_sizes(std::string& o, const std::vector<BetSize>& v) {
+    o.push_back('[');
+    for (size_t k = 0; k < v.size(); ++k) {
+        if (k) o.push_back(',');
+        switch (v[k].kind) {
+            case SizeKind::AllIn: o += "\"AllIn\""; break;
+            case SizeKind::PotPct: o += "{\"PotPct\":" + shortest(v[k].value) + "}"; break;
+            default: o += "{\"PrevMult\":" + shortest(v[k].value) + "}"; break;
+        }
+    }
+    o.push_back(']');
+}
+
+void put_streets(std::string& o, const StreetSizing (&s)[3]) {
+    o.push_back('[');
+    for (int k = 0; k < 3; ++k) {
+        if (k) o.push_back(',');
+        o += "{\"bet\":";
+        put_sizes(o, s[k].bet);
+        o += ",\"raise\":";
+        put_sizes(o, s[k].raise);
+        o += ",\"donk\":";
+        put_sizes(o, s[k].donk);
+        o.push_back('}');
+    }
+    o.push_back(']');
+}
+
+uint32_t u32_of(const JVal& v, const char* what) {
+    if (v.type != JVal::Num || !(v.num >= 0.0) || v.num > 4294967295.0 || v.num != std::floor(v.num))
+        throw Error(std::string(what) + " must be an unsigned 32-bit integer");
+    return (uint32_t)v.num;
+}
+
+}  // namespace
+
+SpotConfig parse_spot_config(const std::string& json) { return spot_config_of(Parser(json).parse()); }
+
+std::string spot_config_json(const SpotConfig& c) {
+    std::string o = "{\"board\":";
+    put_str(o, c.board);
+    o += ",\"range_oop\":";
+    put_str(o, c.range_oop);
+    o += ",\"range_ip\":";
+    put_str(o, c.range_ip);
+    const TreeConfig& t = c.tree;
+    o += ",\"tree\":{\"starting_pot\":" + shortest(t.starting_pot) + ",\"effective_stack\":" +
+         shortest(t.effective_stack) + ",\"rake_pct\":" + shortest(t.rake_pct) + ",\"rake_cap\":" +
+         shortest(t.rake_cap) + ",\"oop\":";
+    put_streets(o, t.oop);
+    o += ",\"ip\":";
+    put_streets(o, t.ip);
+    o += ",\"allin_threshold\":" + shortest(t.allin_threshold) + ",\"add_allin\":" +
+         (t.add_allin ? "true" : "false") + ",\"max_raises\":" + std::to_string((unsigned)t.max_raises) + "}}";
+    return o;
+}
+
+std::string save_header_json(const SaveHeader& h) {
+    std::string o = "{\"config\":" + spot_config_json(h.config) + ",\"iteration\":" + std::to_string(h.iteration) +
+                    ",\"locks\":[";
+    for (size_t k = 0; k < h.locks.size(); ++k) {
+        if (k) o.push_back(',');
+        o += "[" + std::to_string(h.locks[k].first) + ",[";
+        const std::vector<float>& sg = h.locks[k].second;
+        for (size_t j = 0; j < sg.size(); ++j) {
+            if (j) o.push_back(',');
+            o += shortest(sg[j]);
+        }
+        o += "]]";
+    }
+    o += "],\"lock_labels\":[";
+    for (size_t k = 0; k < h.lock_labels.size(); ++k) {
+        if (k) o.push_back(',');
+        o += "[" + std::to_string(h.lock_labels[k].first) + ",";
+        put_str(o, h.lock_labels[k].second);
+        o.push_back(']');
+    }
+    o += "]}";
+    return o;
+}
+
+SaveHeader parse_save_header(const std::string& line) {
+    JVal root = Parser(line).parse();
+    if (root.type != JVal::Obj) throw Error("expected a JSON object");
+    SaveHeader h;
+    const JVal* c = root.get("config");
+    if (!c) throw Error("missing field `config`");
+    h.config = spot_config_of(*c);
+    const JVal* it = root.get("iteration");
+    if (!it) throw Error("missing field `iteration`");
+    h.iteration = u32_of(*it, "iteration");
+    // serde defaults: saves written before locks existed carry neither field
+    if (const JVal* l = root.get("locks"); l && l->type != JVal::Null) {
+        if (l->type != JVal::Arr) throw Error("locks must be an array");
+        for (const JVal& e : l->arr) {
+            if (e.type != JVal::Arr || e.arr.size() != 2 || e.arr[1].type != JVal::Arr)
+                throw Error("lock entries must be [node, [frequencies]]");
+            std::vector<float> sg;
+            sg.reserve(e.arr[1].arr.size());
+            for (const JVal& x : e.arr[1].arr) {
+                if (x.type != JVal::Num) throw Error("lock frequencies must be numbers");
+                sg.push_back((float)x.num);
+            }
+            h.locks.emplace_back(u32_of(e.arr[0], "lock node"), std::move(sg));
+        }
+    }
+    if (const JVal* l = root.get("lock_labels"); l && l->type != JVal::Null) {
+        if (l->type != JVal::Arr) throw Error("lock_labels must be an array");
+        for (const JVal& e : l->arr) {
+            if (e.type != JVal::Arr || e.arr.size() != 2 || e.arr[1].type != JVal::Str)
+                throw Error("lock label entries must be [node, \"label\"]");
+            h.lock_labels.emplace_back(u32_of(e.arr[0], "lock label node"), e.arr[1].str);
+        }
+    }
+    return h;
 }
 
 }  // namespace gto

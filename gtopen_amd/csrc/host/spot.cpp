@@ -9,8 +9,7 @@
 #include <thread>
 #include <unordered_map>
 
-#include "../../../include/gtopen_amd.h"
-#include "host.h"
+#include "abi.h"
 
 namespace gto {
 
@@ -188,10 +187,6 @@ thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 }  // namespace gto
 
-struct gto_spot {
-    std::unique_ptr<gto::Spot> spot;
-    gto_spot_view view;
-};
 
 extern "C" {
 

@@ -6,6 +6,8 @@
 // (tree.rs:546-548). All amounts are f64, as in the reference.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "host.h"
 
@@ -285,10 +287,16 @@ Tree build_tree(const TreeConfig& cfg, const std::vector<Card>& board, const siz
                     if (l == 1 && !donk_used) continue;
                     const std::vector<BetSize>& list = l == 0 ? sz.bet : sz.donk;
                     for (const BetSize& b : list)
-                        if (b.kind == SizeKind::PrevMult)
+                        if (b.kind == SizeKind::PrevMult) {
+                            char m[32];
+                            for (int p = 1; p <= 17; ++p) {  // shortest form that reads back
+                                std::snprintf(m, sizeof m, "%.*g", p, b.value);
+                                if (std::strtod(m, nullptr) == b.value) break;
+                            }
                             throw Error(std::string(street_name(s)) + (pl == 0 ? " OOP " : " IP ") +
-                                        (l == 0 ? "bet" : "donk") +
-                                        ": raise-only size (x) — use % of pot or 'a'");
+                                        (l == 0 ? "bet" : "donk") + ": '" + m +
+                                        "x' is a raise-only size — use % of pot or 'a'");
+                        }
                 }
             }
     }

@@ -19,6 +19,7 @@ class GpuSolver:
             device,
         )
         self.storage = storage
+        self.locks, self.lock_labels = {}, {}
         h = ctypes.c_void_p()
         L.check(L.lib().gto_hip_create(spot.view_ptr, ctypes.byref(opts), ctypes.byref(h)), "gpu create")
         self._h = h
@@ -73,6 +74,17 @@ class GpuSolver:
         )
         return r, s
 
+    def download_compressed(self, p):
+        """Raw compressed arenas of player p: (i16 regret quants, u16 strategy
+        quants, per-node regret scales, per-node strategy scales)."""
+        ds, n = self.spot.data_size[p], self.spot.num_nodes
+        rq, sq = np.zeros(ds, np.int16), np.zeros(ds, np.uint16)
+        rs, ss = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        L.check(L.lib().gto_hip_download_compressed(self._h, p, rq.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+                                                    sq.ctypes.data_as(L.c_u16p), rs.ctypes.data_as(L.c_f32p),
+                                                    ss.ctypes.data_as(L.c_f32p)), "download compressed")
+        return rq, sq, rs, ss
+
     def upload_arenas(self, regrets, strat):
         """f32 arenas (None for an arena, or for a pair, keeps it)."""
         regrets = regrets if regrets is not None else (None, None)
@@ -96,12 +108,24 @@ class GpuSolver:
             parts.append(sig)
             pos += sig.size
         sigma = np.concatenate(parts) if parts else np.zeros(1, dtype=np.float32)
+        self.locks = {int(n): np.asarray(locks[n], dtype=np.float32) for n in locks}
         L.check(
             L.lib().gto_hip_update_locks(
                 self._h, off.ctypes.data_as(L.c_i64p), sigma.ctypes.data_as(L.c_f32p), pos
             ),
             "update locks",
         )
+
+    def save(self, path, locks=None, labels=None):
+        """sync_to_cpu + Solver::save: download (decoding compressed arenas)
+        and write a GTOSOLVE2 file. Locks default to the ones last given to
+        update_locks."""
+        from .save import write_save
+
+        regrets, strat = self.download_arenas()
+        locks = self.locks if locks is None else locks
+        labels = self.lock_labels if labels is None else labels
+        write_save(path, self.spot, self.iteration, regrets, strat, locks=locks, labels=labels)
 
     def synchronize(self):
         L.check(L.lib().gto_hip_synchronize(self._h), "synchronize")

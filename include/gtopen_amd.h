@@ -138,6 +138,48 @@ int gto_canonical_flops(uint32_t n_subset, uint8_t* cards3, uint32_t* weights, u
                         uint32_t* n);
 
 /* ------------------------------------------------------------------------ */
+/* GTOSOLVE2 save files (crates/solver/src/save.rs)                          */
+/* ------------------------------------------------------------------------ */
+/* "GTOSOLVE2\n", one JSON header line {"config": SpotConfig, "iteration",
+ * "locks": [[node, [f32..]]..], "lock_labels": [[node, "label"]..]}, then
+ * regrets OOP, IP, strategy sums OOP, IP as (u64 LE length, f32 LE data).
+ * Always f32 on disk (compressed engines decode on download). Locks are
+ * given as n_locks entries: node, offset and length into lock_sigma. */
+typedef struct gto_save gto_save;
+
+/* Solver::save / write_save (save.rs:94-143): staged to "<path>.tmp",
+ * fsync'd, renamed over path. Arenas have data_size[0]/[1] entries. Locks
+ * are validated first (save.rs:42-70). */
+int gto_save_write(const char* path, const gto_spot* spot, uint32_t iteration, const float* regrets0,
+                   const float* regrets1, const float* strat0, const float* strat1, uint32_t n_locks,
+                   const uint32_t* lock_nodes, const uint64_t* lock_off, const uint64_t* lock_len,
+                   const float* lock_sigma, uint32_t n_labels, const uint32_t* label_nodes,
+                   const char* const* labels);
+/* First half of Solver::load_with_storage (save.rs:149-168): checks the
+ * magic ("not a solver save file") and parses the header ("bad header: ..").
+ * The caller rebuilds the spot leniently from gto_save_config_json (every
+ * SpotConfig field, serde layout), then validates locks and reads arenas. */
+int gto_save_open(const char* path, gto_save** out);
+void gto_save_close(gto_save* s);
+const char* gto_save_config_json(const gto_save* s);
+uint32_t gto_save_iteration(const gto_save* s);
+uint32_t gto_save_num_locks(const gto_save* s);
+int gto_save_lock(const gto_save* s, uint32_t k, uint32_t* node, const float** sigma, uint64_t* len);
+uint32_t gto_save_num_labels(const gto_save* s);
+int gto_save_label(const gto_save* s, uint32_t k, uint32_t* node, const char** label);
+/* validate_locks on the header's locks against the rebuilt spot, errors
+ * prefixed "bad lock section: " (save.rs:168-170). */
+int gto_save_validate_locks(const gto_save* s, const gto_spot* spot);
+/* The four arenas, sized against the rebuilt tree ("arena size mismatch:
+ * file N, expected M (tree config changed?)", save.rs:177-192). NULL skips
+ * an arena. */
+int gto_save_read_arenas(const gto_save* s, const gto_spot* spot, float* regrets0, float* regrets1,
+                         float* strat0, float* strat1);
+/* validate_locks (save.rs:42-70) for a lock list. */
+int gto_validate_locks(const gto_spot* spot, uint32_t n_locks, const uint32_t* lock_nodes,
+                       const uint64_t* lock_off, const uint64_t* lock_len, const float* lock_sigma);
+
+/* ------------------------------------------------------------------------ */
 /* Engine: the GPU DCFR hot path                                             */
 /* ------------------------------------------------------------------------ */
 
