@@ -171,6 +171,60 @@ class Engine {
         HIP_OK(hipStreamSynchronize(stream_));
     }
 
+    // Average strategy of one action node from the strategy arena, ignoring
+    // any lock (solved_strategy_into, cfr.rs:679-726): max(S,0)/sum with an
+    // f64 sum (f32 arenas) or q/sum over u16 quants (compressed), uniform
+    // where the sum is <= 1e-12 / 0.
+    void solved_strategy(uint32_t n, float* out) {
+        if (n >= num_nodes_ || node_kind_[n] != GTO_KIND_ACTION)
+            throw std::runtime_error("node " + std::to_string(n) + " is not an action node");
+        const int p = node_player_[n];
+        const size_t na = node_na_[n], nh = plan_.nh[p], cnt = na * nh;
+        const uint64_t off = node_off_[n];
+        if (!compressed_) {
+            std::vector<float> b(cnt);
+            HIP_OK(hipMemcpyAsync(b.data(), S_.S[p] + off, sizeof(float) * cnt, hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            for (size_t i = 0; i < nh; ++i) {
+                double sum = 0.0;
+                for (size_t a = 0; a < na; ++a) sum += (double)std::max(b[a * nh + i], 0.0f);
+                for (size_t a = 0; a < na; ++a)
+                    out[a * nh + i] =
+                        sum > 1e-12 ? (float)((double)std::max(b[a * nh + i], 0.0f) / sum) : 1.0f / (float)na;
+            }
+        } else {
+            std::vector<uint16_t> q(cnt);
+            HIP_OK(hipMemcpyAsync(q.data(), S_.Sq[p] + off, sizeof(uint16_t) * cnt, hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            for (size_t i = 0; i < nh; ++i) {
+                uint32_t sum = 0;
+                for (size_t a = 0; a < na; ++a) sum += q[a * nh + i];
+                const float inv = sum > 0 ? 1.0f / (float)sum : 0.0f;
+                for (size_t a = 0; a < na; ++a) out[a * nh + i] = sum > 0 ? (float)q[a * nh + i] * inv : 1.0f / (float)na;
+            }
+        }
+    }
+
+    // Value rows of player p at the given nodes after one sweep: mode 0 =
+    // best response (traverse_br, best_response.rs:19-124), 1 = average
+    // strategy (traverse_avg, 128-223). The row of node n is what those
+    // functions return for n given the opponent's reach at n.
+    void node_values(int p, int mode, uint32_t count, const uint32_t* nodes, float* out) {
+        for (uint32_t k = 0; k < count; ++k)
+            if (nodes[k] >= num_nodes_ || plan_.cfv_slot[nodes[k]] == 0xFFFFFFFFu)
+                throw std::runtime_error("node " + std::to_string(nodes[k]) + " is not in the reachable tree");
+        const int o = 1 - p;
+        launch_root(1 << o);
+        launch_down(1 << o, kAverageF32);
+        reach_valid_[o] = false;
+        launch_up(p, mode == 0 ? 1 : 2);
+        const size_t nh = plan_.nh[p];
+        for (uint32_t k = 0; k < count; ++k)
+            HIP_OK(hipMemcpyAsync(out + k * nh, S_.cfv + (size_t)plan_.cfv_slot[nodes[k]] * plan_.nh_max,
+                                  sizeof(float) * nh, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+
     // Range weights (the root reach); the spot's hands stay as built, and a
     // zero weight leaves its hand with zero reach.
     void set_weights(int q, const float* w) {
@@ -788,6 +842,22 @@ int gto_hip_root_cfv(gto_hip* h, int p, int mode, float* out) {
         h->e->root_cfv(p, mode, c);
         std::copy(c.begin(), c.end(), out);
     });
+}
+
+int gto_hip_solved_strategy(gto_hip* h, uint32_t node, float* out) {
+    if (!h || !out) {
+        set_last_error("gto_hip_solved_strategy: null argument");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] { h->e->solved_strategy(node, out); });
+}
+
+int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint32_t* nodes, float* out) {
+    if (!h || p < 0 || p > 1 || mode < 0 || mode > 1 || (count && (!nodes || !out))) {
+        set_last_error("gto_hip_node_values: bad argument");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] { h->e->node_values(p, mode, count, nodes, out); });
 }
 
 int gto_hip_set_weights(gto_hip* h, const float* w0, const float* w1) {

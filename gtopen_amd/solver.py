@@ -7,9 +7,10 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
+from .query import QueryMixin
 
 
-class GpuSolver:
+class GpuSolver(QueryMixin):
     def __init__(self, spot, storage="f32", algo="dcfr", use_graphs=True, device=0):
         self.spot = spot  # keeps the view's arrays alive
         opts = L.HipOptions(

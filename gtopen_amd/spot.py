@@ -130,6 +130,8 @@ class Spot:
         self.hand_c2 = [_arr(v.hand_c2[p], self.num_hands[p], np.uint8) for p in range(2)]
         self.weights = [_arr(v.weights[p], self.num_hands[p], np.float32) for p in range(2)]
         self.same_combo = [_arr(v.same_combo[p], self.num_hands[p], np.uint32) for p in range(2)]
+        self.put = _arr(v.node_put, 2 * n, np.float64).reshape(n, 2)
+        self.board = [int(v.board[k]) for k in range(int(v.board_len))]
         self.starting_pot = float(v.starting_pot)
         self.root_street = int(v.root_street)
 

@@ -230,6 +230,19 @@ int gto_hip_upload_arenas(gto_hip* h, const float* regrets0, const float* regret
 int gto_hip_download_compressed(gto_hip* h, int p, int16_t* regrets_q, uint16_t* strat_q,
                                 float* regret_scale, float* strat_scale);
 
+/* Average strategy of one action node from the device strategy arena,
+ * ignoring locks: solved_strategy_into (cfr.rs:679-726), na*nh_actor floats
+ * in arena layout. The query layer's walk_path / lock_node base
+ * (query.rs:304-350, 645-734). */
+int gto_hip_solved_strategy(gto_hip* h, uint32_t node, float* out);
+
+/* Per-node value rows of player p (nh_p floats per node) after one sweep:
+ * mode 0 = best response, mode 1 = average strategy; the row of node n is
+ * traverse_br / traverse_avg (best_response.rs:19-223) at n given the
+ * opponent's reach there. Feeds exploit_view (query.rs:800-961). Nodes
+ * outside the reachable tree are an error. */
+int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint32_t* nodes, float* out);
+
 /* Range weights per hand (the root reach), replacing the spot's; NULL keeps
  * a player's weights. The hand lists stay as built: a weight of 0 leaves the
  * hand in place with zero reach (the reference's builder would drop it,
