@@ -1,0 +1,32 @@
+"""Batches of GPU engines advanced by one graph launch per iteration
+(gto_batch_*): every engine's steady-state iteration is a child graph of one
+parent graph, so independent spots (the flops of a report) iterate
+concurrently on the device."""
+import ctypes
+
+from . import _lib as L
+
+
+class Batch:
+    def __init__(self, solvers):
+        self.solvers = list(solvers)  # keeps the engines alive
+        if not self.solvers:
+            raise ValueError("a batch needs at least one solver")
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        L.check(L.lib().gto_batch_create(arr, len(self.solvers), ctypes.byref(h)), "batch create")
+        self._h = h
+
+    def iterate(self, n=1):
+        L.check(L.lib().gto_batch_iterate(self._h, n), "batch iterate")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().gto_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -107,6 +107,39 @@ class Engine {
     }
 
     uint32_t iteration() const { return iteration_; }
+
+    // --- batch support: several engines' iterations in one parent graph ---
+    // Steady state: the regrets-changed reach of player 1 is the only stale
+    // one, which is what one captured iteration assumes.
+    bool steady() const { return reach_valid_[0] && !reach_valid_[1] && !prof_; }
+    hipStream_t stream() const { return stream_; }
+    int device() const { return opts_.device; }
+    // bumped whenever a captured iteration's kernel arguments go stale
+    uint64_t generation() const { return gen_; }
+    void reserve_iterations(uint32_t n) { ensure_disc(iteration_ + n + 1); }
+    // one steady-state iteration as an uninstantiated graph (caller owns it)
+    hipGraph_t capture_iteration() {
+        if (!steady()) throw std::runtime_error("engine is not in the steady state; iterate once first");
+        HIP_OK(hipStreamSynchronize(stream_));
+        HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            launch_iteration();
+        } catch (...) {
+            hipGraph_t g;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            reach_valid_[0] = true;
+            reach_valid_[1] = false;
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        HIP_OK(hipStreamEndCapture(stream_, &g));
+        reach_valid_[0] = true;
+        reach_valid_[1] = false;
+        return g;
+    }
+    // record n iterations run through a batch graph
+    void advanced(uint32_t n) { iteration_ += n; }
     void set_iteration(uint32_t t) {
         iteration_ = t;
         ensure_disc(t + 1);
@@ -449,6 +482,7 @@ class Engine {
     uint32_t show_waves_[2] = {kWaves, kWaves};
     size_t fold_lds_[2] = {0, 0};
     hipGraphExec_t graph_ = nullptr;
+    uint64_t gen_ = 0;
     gto_hip_profile* prof_ = nullptr;
     std::vector<hipEvent_t> evs_;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_;
@@ -598,6 +632,7 @@ class Engine {
     }
 
     void drop_graph() {
+        ++gen_;
         if (graph_) {
             (void)hipGraphExecDestroy(graph_);
             graph_ = nullptr;
@@ -792,6 +827,78 @@ struct gto_hip {
     std::unique_ptr<Engine> e;
 };
 
+// A batch of engines (independent spots, e.g. the flops of a report) whose
+// steady-state iterations are captured as child graphs of one parent graph:
+// one hipGraphLaunch advances every spot by an iteration, and the children,
+// having no edges between them, run concurrently on the device.
+struct gto_batch {
+    std::vector<Engine*> eng;
+    std::vector<hipEvent_t> ready;
+    std::vector<uint64_t> gens;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~gto_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (hipEvent_t e : ready) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void build() {
+        if (exec) {
+            HIP_OK(hipGraphExecDestroy(exec));
+            exec = nullptr;
+        }
+        hipGraph_t parent = nullptr;
+        HIP_OK(hipGraphCreate(&parent, 0));
+        try {
+            for (Engine* e : eng) {
+                hipGraph_t child = e->capture_iteration();
+                hipGraphNode_t node;
+                hipError_t rc = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, child);
+                (void)hipGraphDestroy(child);
+                HIP_OK(rc);
+            }
+            HIP_OK(hipGraphInstantiate(&exec, parent, nullptr, nullptr, 0));
+        } catch (...) {
+            (void)hipGraphDestroy(parent);
+            throw;
+        }
+        HIP_OK(hipGraphDestroy(parent));
+        gens.clear();
+        for (Engine* e : eng) gens.push_back(e->generation());
+    }
+
+    void iterate(uint32_t n) {
+        for (Engine* e : eng) e->reserve_iterations(n);
+        uint32_t k = 0;
+        // engines not in the steady state (fresh, or arenas just uploaded)
+        // take one iteration on their own first, so the counts stay aligned
+        bool all = true;
+        for (Engine* e : eng) all = all && e->steady();
+        if (!all && n > 0) {
+            for (Engine* e : eng) e->iterate(1);
+            k = 1;
+        }
+        if (k == n) {
+            for (Engine* e : eng) e->synchronize();
+            return;
+        }
+        bool stale = !exec;
+        for (size_t i = 0; i < eng.size() && !stale; ++i) stale = gens[i] != eng[i]->generation();
+        if (stale) build();
+        for (size_t i = 0; i < eng.size(); ++i) {
+            HIP_OK(hipEventRecord(ready[i], eng[i]->stream()));
+            HIP_OK(hipStreamWaitEvent(stream, ready[i], 0));
+        }
+        for (; k < n; ++k) {
+            HIP_OK(hipGraphLaunch(exec, stream));
+            for (Engine* e : eng) e->advanced(1);
+        }
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+};
+
 extern "C" {
 
 int gto_hip_create(const gto_spot_view* spot, const gto_hip_options* opts, gto_hip** out) {
@@ -843,6 +950,43 @@ int gto_hip_root_cfv(gto_hip* h, int p, int mode, float* out) {
         std::copy(c.begin(), c.end(), out);
     });
 }
+
+int gto_batch_create(gto_hip* const* engines, uint32_t n, gto_batch** out) {
+    if (!engines || !out || n == 0) {
+        set_last_error("gto_batch_create: need at least one engine");
+        return GTO_ERR_INVALID;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        auto b = std::make_unique<gto_batch>();
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!engines[i]) throw std::runtime_error("gto_batch_create: null engine");
+            for (Engine* e : b->eng)
+                if (e == engines[i]->e.get()) throw std::runtime_error("gto_batch_create: an engine is listed twice");
+            b->eng.push_back(engines[i]->e.get());
+            if (b->eng.back()->device() != b->eng[0]->device())
+                throw std::runtime_error("gto_batch_create: engines of one batch must share a device");
+        }
+        HIP_OK(hipSetDevice(b->eng[0]->device()));
+        HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        for (uint32_t i = 0; i < n; ++i) {
+            hipEvent_t ev;
+            HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            b->ready.push_back(ev);
+        }
+        *out = b.release();
+    });
+}
+
+int gto_batch_iterate(gto_batch* b, uint32_t n) {
+    if (!b) {
+        set_last_error("gto_batch_iterate: null batch");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] { b->iterate(n); });
+}
+
+void gto_batch_destroy(gto_batch* b) { delete b; }
 
 int gto_hip_solved_strategy(gto_hip* h, uint32_t node, float* out) {
     if (!h || !out) {

@@ -1,0 +1,145 @@
+"""Reports: many flops solved to a target, the role of the reference
+server's report worker (crates/server/src/main.rs:1583-1772, report_solve
+1774-1825, write_report 1827-1853), redesigned around the GPU batch.
+
+Flops are grouped into waves that fit the device memory (the engines' own
+VRAM estimates against free memory); a wave's engines iterate together,
+one parent-graph launch per iteration (gtopen_amd.batch). Every
+``check_every`` iterations each unfinished flop's exploitability is taken;
+a flop that reaches the target (or the iteration cap) is recorded and leaves
+the batch, so uneven trees finish at their own pace. Rows carry the
+reference's fields (board, weight, iterations, exploit_pct, seconds, per
+player EV, root action frequencies, IP's response to a root check); the
+equity columns of the reference (its equity enumeration) are not produced.
+The report file is rewritten through a temporary file every 8 flops and at
+the end, with "complete" false until every flop is in.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .batch import Batch
+from .solver import GpuSolver, vram_info
+from .spot import Spot, SpotConfig, canonical_flops
+
+
+def flop_configs(template, n_flops):
+    """canonical_flops_subset(n) (cards.rs:236-259) boards on a template
+    SpotConfig: [(SpotConfig, weight)]."""
+    out = []
+    for board, weight in canonical_flops(n_flops):
+        t = template.tree
+        out.append((SpotConfig(board, template.range_oop, template.range_ip, t), weight))
+    return out
+
+
+def _root_stats(gpu, spot):
+    """Per-player reach x valid weighted EV at the root, the root actor's
+    reach-weighted action frequencies, and IP's frequencies after a root
+    check (main.rs:1540-1581, 1714-1729)."""
+    players = []
+    for p in range(2):
+        cur = gpu.node_values(p, 1, [0])[0].astype(np.float64)
+        reach = spot.weights[p].astype(np.float64)
+        valid = gpu.valid_opp_sum(p, spot.weights[1 - p])
+        ok = valid > 1e-9
+        w = reach * valid
+        ev = np.where(ok, cur / np.where(ok, valid, 1.0) + spot.put[0][p], 0.0)
+        wt = float(w[ok].sum())
+        players.append({"ev": float((ev[ok] * w[ok]).sum() / wt) if wt > 1e-12 else 0.0})
+
+    def freqs(path):
+        v = gpu.node_view(path)
+        if v["player"] is None:
+            return None
+        hs = v["players"][v["player"]]["hands"]
+        tot = sum(h["reach"] for h in hs)
+        na = len(hs[0]["strategy"]) if hs else 0
+        f = [sum(h["strategy"][a] * h["reach"] for h in hs) / tot if tot > 1e-12 else 0.0 for a in range(na)]
+        return {"actor": v["player"], "freqs": f}
+
+    root = freqs([])
+    vs_check = None
+    if spot.kind[0] == L.KIND_ACTION and int(spot.num_children[0]) > 0:
+        # the reference's action list puts check first when the root can check
+        c = spot.child(0, 0)
+        if spot.kind[c] == L.KIND_ACTION and int(spot.player[c]) == 1:
+            vs_check = freqs([{"type": "action", "index": 0}])
+    return players, root, vs_check
+
+
+def _write(path, meta, rows, complete):
+    if not path:
+        return
+    out = dict(meta, created=int(time.time()), complete=complete, flops=rows)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(out, f)
+    os.replace(tmp, path)
+
+
+def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None, storage="f32",
+               mem_fraction=0.8, device=0, max_batch=64, name="report", progress=None):
+    """Solve [(SpotConfig, weight)] to ``target`` % pot; returns the report
+    dict with ``flops_per_hour``."""
+    t0 = time.time()
+    meta = {"name": name, "target_pct": target, "max_iterations": max_iterations, "storage": storage}
+    rows = []
+    _write(path, meta, rows, False)
+    free, _total = vram_info(device)
+    budget = free * mem_fraction
+    st = L.STORAGE_F32 if storage == "f32" else L.STORAGE_COMPRESSED
+    k = 0
+    waves = 0
+    while k < len(flops):
+        # fill a wave up to the memory budget (at least one flop)
+        wave, used = [], 0
+        while k < len(flops) and len(wave) < max_batch:
+            cfg, weight = flops[k]
+            spot = Spot(cfg)
+            need = int(L.lib().gto_hip_estimate_vram(spot.view_ptr, st))
+            if wave and used + need > budget:
+                break
+            wave.append((cfg, weight, spot))
+            used += need
+            k += 1
+        waves += 1
+        tw = time.time()
+        gpus = [GpuSolver(s, storage=storage, device=device) for _, _, s in wave]
+        active = list(range(len(wave)))
+        batch = None
+        while active:
+            if batch is None:
+                batch = Batch([gpus[i] for i in active])
+            it = gpus[active[0]].iteration
+            step = min(check_every - it % check_every, max_iterations - it)
+            batch.iterate(step)
+            it += step
+            done = []
+            for i in active:
+                cfg, weight, spot = wave[i]
+                pct = gpus[i].exploitability() / spot.starting_pot * 100.0
+                if pct <= target or it >= max_iterations:
+                    players, root, vs_check = _root_stats(gpus[i], spot)
+                    rows.append({"board": cfg.board, "weight": weight, "iterations": it, "exploit_pct": pct,
+                                 "seconds": time.time() - tw, "nodes": spot.num_nodes, "players": players,
+                                 "root": root, "vs_check": vs_check})
+                    done.append(i)
+                    if progress:
+                        progress(rows[-1])
+                    if len(rows) % 8 == 0:
+                        _write(path, meta, rows, False)
+            if done:
+                batch.close()
+                batch = None
+                for i in done:
+                    gpus[i].close()
+                active = [i for i in active if i not in done]
+        del gpus
+    elapsed = time.time() - t0
+    meta.update(seconds=elapsed, waves=waves, flops_per_hour=len(rows) / elapsed * 3600.0 if elapsed > 0 else 0.0)
+    _write(path, meta, rows, True)
+    return dict(meta, complete=True, flops=rows)

@@ -230,6 +230,21 @@ int gto_hip_upload_arenas(gto_hip* h, const float* regrets0, const float* regret
 int gto_hip_download_compressed(gto_hip* h, int p, int16_t* regrets_q, uint16_t* strat_q,
                                 float* regret_scale, float* strat_scale);
 
+/* Batches: several engines (independent spots on one device, e.g. the
+ * flops of a report, main.rs:1583-1772) advanced together. Each engine's
+ * steady-state iteration is captured as a child graph of one parent graph,
+ * so one hipGraphLaunch iterates every spot and the children run
+ * concurrently. Engines stay owned by the caller and must outlive the
+ * batch; between gto_batch_iterate calls each engine may be used on its own
+ * (exploitability, downloads, locks: the batch re-captures what changed).
+ * No reference counterpart: the reference solves report flops one by one. */
+typedef struct gto_batch gto_batch;
+int gto_batch_create(gto_hip* const* engines, uint32_t n, gto_batch** out);
+/* n iterations of every engine (an engine not in the steady state first
+ * takes one iteration on its own stream); returns when all are done. */
+int gto_batch_iterate(gto_batch* b, uint32_t n);
+void gto_batch_destroy(gto_batch* b);
+
 /* Average strategy of one action node from the device strategy arena,
  * ignoring locks: solved_strategy_into (cfr.rs:679-726), na*nh_actor floats
  * in arena layout. The query layer's walk_path / lock_node base

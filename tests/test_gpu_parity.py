@@ -184,3 +184,21 @@ def test_full_range_river_matches_oracle(built):
     rc, sc = cpu.arenas()
     _close_arenas(rg, rc, spot.starting_pot)
     _close_arenas(sg, sc, 1.0)
+
+
+def test_cfr_plus_matches_oracle(built):
+    """CFR+ (cfr.rs:57-61: positive regrets kept, negative ones floored by a
+    zero discount, strategy weights (t/(t+1))^2) on the GPU engine."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: turn spot, CFR+, 40 iterations, check every 10")
+    spot = Spot(workloads.turn_spot())
+    gpu, cpu = GpuSolver(spot, algo="cfr+"), oracle.Oracle(spot, algo="cfr+")
+    pot = spot.starting_pot
+    for k in range(4):
+        gpu.iterate(10)
+        cpu.iterate(10)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {10 * (k + 1)}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) <= POT_TOL * pot
