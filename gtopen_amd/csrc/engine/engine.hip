@@ -477,9 +477,13 @@ class Engine {
     float* disc_buf_ = nullptr;
     float* lock_sigma_buf_ = nullptr;
     float* grad_buf_ = nullptr;
+    // all-level terminal lists
+    FoldRec* fold_all_ = nullptr;
+    uint32_t n_fold_all_ = 0;
+    ShowRec* show_all_ = nullptr;
+    uint4* show_chunks_ = nullptr;
+    uint32_t n_show_chunks_ = 0;
     size_t show_lds_[2] = {0, 0};
-    uint32_t show_wave_[2] = {0, 0};
-    uint32_t show_waves_[2] = {kWaves, kWaves};
     size_t fold_lds_[2] = {0, 0};
     hipGraphExec_t graph_ = nullptr;
     uint64_t gen_ = 0;
@@ -543,17 +547,20 @@ class Engine {
             T_.sd_cpos_base[q] = upload_vec(plan_.sd_cpos_base[q]);
             T_.sd_cpos[q] = upload_vec(plan_.sd_cpos[q]);
             T_.sd_tab[q] = (const uint4*)upload_vec(plan_.sd_tab[q]);
+            T_.sd_kslot[q] = upload_vec(plan_.sd_kslot[q]);
         }
-        // LDS: showdown, per wavefront, for traverser p (opponent list length)
+        // LDS: showdown, per workgroup, for traverser p (largest board)
         for (int p = 0; p < 2; ++p) {
-            size_t mo = plan_.max_board_cnt[1 - p];
-            const size_t nh_o = plan_.nh[1 - p];
-            const size_t creg = std::max<size_t>(2 * mo + 52, (nh_o + 1) / 2);
-            show_wave_[p] = (uint32_t)((8 * (mo + 1) + 8 * creg + 4 * mo + 16 + 15) / 16 * 16);
-            show_waves_[p] = (uint32_t)std::min<size_t>(kWaves, (160 * 1024) / show_wave_[p]);
-            if (show_waves_[p] == 0)
-                throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
-            show_lds_[p] = (size_t)show_wave_[p] * show_waves_[p];
+            const uint32_t o = 1 - p;
+            size_t need = 0;
+            for (uint32_t b = 0; b < plan_.num_boards; ++b) {
+                const uint32_t* cf = &plan_.sd_coff[o][(size_t)b * 53];
+                for (int c = 0; c < 52; ++c)
+                    if (cf[c + 1] - cf[c] > (uint32_t)kMaxCardList)
+                        throw std::runtime_error("a card appears in more opponent hands than the showdown kernel holds");
+                need = std::max<size_t>(need, show_lds_layout(plan_.sd_cnt[o][b], plan_.nh[p], plan_.nh[o]).bytes);
+            }
+            show_lds_[p] = need;
             fold_lds_[p] = kFoldWaveBytes * kWaves;
         }
         size_t lds_max = std::max(std::max(show_lds_[0], show_lds_[1]), std::max(fold_lds_[0], fold_lds_[1]));
@@ -563,6 +570,13 @@ class Engine {
                                    (int)std::max(show_lds_[0], show_lds_[1])));
         HIP_OK(hipFuncSetAttribute((const void*)k_up_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)std::max(fold_lds_[0], fold_lds_[1])));
+        n_fold_all_ = (uint32_t)plan_.fold_all.size();
+        if (n_fold_all_) fold_all_ = upload_vec(plan_.fold_all);
+        n_show_chunks_ = (uint32_t)(plan_.show_chunks.size() / 4);
+        if (n_show_chunks_) {
+            show_all_ = upload_vec(plan_.show_all);
+            show_chunks_ = (uint4*)upload_vec(plan_.show_chunks);
+        }
         lv_.resize(plan_.levels.size());
         for (size_t l = 0; l < plan_.levels.size(); ++l) {
             const LevelLists& L = plan_.levels[l];
@@ -707,15 +721,15 @@ class Engine {
     // values, 4 = values of the strategy arena as given (f32 only); swap:
     // terminal payoffs of the opponent instead of the traverser
     void launch_up(int p, int mode, int swap = 0) {
+        // terminals of every level first: they read only the down pass
+        if (n_fold_all_)
+            launch(PK_UP_FOLD, k_up_fold, wave_grid(n_fold_all_), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
+                   n_fold_all_, p, swap);
+        if (n_show_chunks_)
+            launch(PK_UP_SHOW, k_up_show, wave_grid(n_show_chunks_, 1), show_lds_[p], T_, S_,
+                   (const ShowRec*)show_all_, (const uint4*)show_chunks_, n_show_chunks_, p, swap);
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
-            if (D.n_fold)
-                launch(PK_UP_FOLD, k_up_fold, wave_grid(D.n_fold), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
-                       D.n_fold, p, swap);
-            if (D.n_show)
-                launch_b(PK_UP_SHOW, k_up_show,
-                         wave_grid((D.n_show + kNodesPerWave - 1) / kNodesPerWave, show_waves_[p]), 64 * show_waves_[p],
-                         show_lds_[p], T_, S_, (const ShowRec*)D.show, D.n_show, p, show_wave_[p], swap);
             if (D.n_chance)
                 launch(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), 0, T_, S_, (const ChanceRec*)D.chance,
                        D.n_chance, p);

@@ -40,7 +40,40 @@ struct DevTree {
     const uint16_t* sd_cpos[2];       // sorted positions holding card c
     const uint16_t* sd_cpos_t[2];     // entry-major: [board][t * 52 + card]
     const uint4* sd_tab[2];           // [board][hand]: packed rank/segment table
+    const uint16_t* sd_kslot[2];      // [sorted hand]: slots in the per-card arrays (2 per hand)
 };
+
+// LDS layout of the showdown kernel for one board (byte offsets): the
+// board's static tables (per-hand entries, opponent sorted hand indices,
+// per-card offsets and positions) stay for a run of nodes; the per-node
+// rows (strength-order prefix P, per-card prefixes, reach in strength
+// order; the hand-order reach row is staged in the per-card region first)
+// are rebuilt per node.
+struct ShowLds {
+    uint32_t ent, P, cpre, wsum, rs, rcard, oidx, kslot, coff, nodes, bytes;
+};
+// per node of a run, staged once: value row, opponent reach slot, live
+// flag, win, lose, tie
+struct ShowNode {
+    uint32_t cfv, oslot, live;
+    float win, lose, tie;
+};
+__host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, uint32_t nho) {
+    ShowLds L;
+    const uint32_t creg = (2 * mo + 52) > ((nho + 1) / 2) ? (2 * mo + 52) : ((nho + 1) / 2);
+    L.ent = 0;                                  // nhp x uint4
+    L.P = L.ent + 16 * nhp;                     // (mo + 1) doubles
+    L.cpre = L.P + 8 * (mo + 1);                // creg doubles
+    L.wsum = L.cpre + 8 * creg;                 // 4 doubles (per-wave scan totals)
+    L.rs = L.wsum + 32;                         // mo floats: reach in strength order
+    L.rcard = L.rs + 4 * mo;                    // 2 mo floats: reach in per-card list order
+    L.oidx = L.rcard + 8 * mo;                  // mo u16
+    L.kslot = L.oidx + ((2 * mo + 3) & ~3u);    // 2 mo u16
+    L.coff = L.kslot + ((4 * mo + 3) & ~3u);    // 53 u32
+    L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
+    L.bytes = (L.nodes + 16 * sizeof(ShowNode) + 15) & ~15u;
+    return L;
+}
 
 // Mutable solver state.
 struct DevState {

@@ -555,13 +555,16 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
 }
 
 // ---------------------------------------------------------------------------
-// Up sweep, fold terminals (game.rs:470-496): value = amount * (opponent
-// reach compatible with the hand), with f64 totals. The opponent's row is
-// staged in LDS; per-card sums run serially in hand order (the CPU's
-// order), one lane per card, over the card's hand list fetched into
-// registers first. LDS per wavefront: nh_o floats + 52 doubles.
+// Up sweep, fold terminals (game.rs:470-496), all levels in one launch:
+// value = amount * (opponent reach compatible with the hand), with f64
+// totals. One wavefront per node: the opponent's row is scattered into a
+// combo-indexed LDS row (zero where the opponent has no such hand), one
+// lane per card sums the card's 51 partner combos in combo order (the
+// CPU's order, game.rs:478-485), and each hand reads its two card sums and
+// its own combo. LDS per wavefront: 52 doubles + 1328 floats.
 // ---------------------------------------------------------------------------
 constexpr int kMaxH = 21;         // hand rows per lane: 21 * 64 >= 1326
+constexpr int kRowPer = (1326 + kBlock - 1) / kBlock;  // hand rows per thread of a workgroup
 constexpr size_t kFoldWaveBytes = 52 * 8 + 1328 * 4;  // per-card sums + combo-indexed row
 
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
@@ -654,164 +657,210 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 }
 
 // ---------------------------------------------------------------------------
-// Up sweep, showdown terminals (game.rs:501-577), one wavefront per node.
-// The opponent's reach is staged in LDS in strength order; an f64 exclusive
-// prefix P over it gives the strictly-weaker total at each hand's static
-// rank lb (and the strictly-stronger total from ub); per-card prefix sums
-// (serial in strength order, one lane per card, the CPU's accumulation
-// order) remove the opponent hands sharing a card with ours. Hands are
-// written in hand order (coalesced) from one 16-byte static table entry
-// each. LDS per wavefront: 8 (m+1) + 8 (2m+52) + 4 m bytes, m = opponent
-// hands on the board.
+// Up sweep, showdown terminals (game.rs:501-577), all levels in one launch:
+// value = win * L + lose * H + tie * (V - L - H) with L/H/V the opponent
+// reach strictly weaker / strictly stronger / compatible, blockers removed.
+// One workgroup per run of nodes sharing a river board: the board's tables
+// are staged in LDS once, then per node (all 256 lanes on every phase):
+//   1. the opponent's reach row (hand order) into LDS; the next node's row
+//      is loaded meanwhile,
+//   2. reach in strength order + its exclusive prefix P (f64, per-thread
+//      chunks and a workgroup scan),
+//   3. per card, the exclusive prefix of the reach of the opponent hands
+//      holding it, in strength order (f64; four lanes per card, each a
+//      quarter of the card's list),
+//   4. per hand, O(1) lookups: L = P[lb] - (card prefixes at lb),
+//      H = (total - P[ub]) - (card totals - card prefixes at ub),
+//      V = total - card totals + identical combo.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
-constexpr int kShowBatch = 8;     // per-hand table entries in flight per lane
-static_assert(kMaxCardList % 8 == 0, "card lists are read 8 entries at a time");
-
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
-                                                    uint32_t count, int p, uint32_t per_wave, int swap) {
+                                                    const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
+                                                    int swap) {
     extern __shared__ double lds[];
-    const uint32_t first = wave_item() * kNodesPerWave;
-    if (first >= count) return;
-    const uint32_t end = min(first + kNodesPerWave, count);
+    const uint32_t c = xcd_slice(blockIdx.x, gridDim.x);
+    if (c >= n_chunks) return;
+    const uint4 ck = chunks[c];
+    const uint32_t b = ck.x, first = ck.y, cnt = ck.z;
     const int o = 1 - p;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nhp = T.nh[p], nho = T.nh[o];
-    char* base = (char*)lds + (size_t)wave * per_wave;
-    // per-board static data, kept in registers across this wavefront's run
-    // of nodes (showdown lists are sorted by board) and reloaded only when
-    // the board changes; the 16-byte per-hand entries are loaded per node
-    uint32_t cur_b = 0xFFFFFFFFu, mo = 0, c0 = 0, len = 0;
-    uint32_t oi[kMaxH];
-    uint32_t pos2[kMaxCardList / 2];
+    const uint32_t mo = T.sd_cnt[o][b];
+    const ShowLds L = show_lds_layout(mo, nhp, nho);
+    char* base = (char*)lds;
+    uint4* ent = (uint4*)(base + L.ent);
+    double* P = (double*)(base + L.P);
+    double* cpre = (double*)(base + L.cpre);
+    double* wsum = (double*)(base + L.wsum);
+    float* rs = (float*)(base + L.rs);
+    float* rcard = (float*)(base + L.rcard);
+    float* rhs = (float*)cpre;
+    uint16_t* oidx = (uint16_t*)(base + L.oidx);
+    uint32_t* kslot = (uint32_t*)(base + L.kslot);  // two u16 per sorted hand
+    uint32_t* coff = (uint32_t*)(base + L.coff);
+    ShowNode* nodes = (ShowNode*)(base + L.nodes);
+
+    // the run's records and alive flags, then the board statics
+    if (tid < cnt) {
+        const ShowRec rr = recs[first + tid];
+        const uint32_t os = pick(rr.own, o);
+        ShowNode nd;
+        nd.cfv = rr.cfv;
+        nd.oslot = os;
+        nd.live = (S.alive[o][os] && mo != 0) ? 1u : 0u;
+        nd.win = swap ? rr.lose : rr.win;
+        nd.lose = swap ? rr.win : rr.lose;
+        nd.tie = rr.tie;
+        nodes[tid] = nd;
+    }
+    {
+        const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
+        for (uint32_t i = tid; i < nhp; i += kBlock) ent[i] = tab[i];
+        const uint32_t so = T.sd_off[o][b];
+        const uint16_t* __restrict__ gi = T.sd_idx[o] + so;
+        for (uint32_t k = tid; k < mo; k += kBlock) oidx[k] = gi[k];
+        const uint32_t* __restrict__ gk = (const uint32_t*)(T.sd_kslot[o] + 2 * (size_t)so);
+        for (uint32_t k = tid; k < mo; k += kBlock) kslot[k] = gk[k];
+        if (tid < 53) coff[tid] = T.sd_coff[o][(size_t)b * 53 + tid];
+    }
+    __syncthreads();
+    // reach row of the first node
+    float rh[kRowPer];
+    {
+        const float* __restrict__ ro = S.reach[o] + (size_t)nodes[0].oslot * nho;
+#pragma unroll
+        for (int u = 0; u < kRowPer; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
+    }
+    // per-card segment of this thread (cards 0..51, four lanes each)
+    const uint32_t card = tid >> 2, seg = tid & 3;
+    constexpr uint32_t kSeg = kMaxCardList / 4;
+    // strength-order chunk of this thread
+    const uint32_t cs = (mo + kBlock - 1) / kBlock;
+    const uint32_t lo = min(tid * cs, mo), hi = min(lo + cs, mo);
+    uint32_t c0 = 0, len = 0, s0 = 0, sn = 0;
+    if (card < 52) {
+        c0 = coff[card];
+        len = coff[card + 1] - c0;
+        s0 = min(seg * kSeg, len);
+        sn = min(kSeg, len - s0);
+    }
+
 #pragma unroll 1
-    for (uint32_t w = first; w < end; ++w) {
-        const ShowRec r = recs[w];
-        float* out = S.cfv + (size_t)r.cfv * T.nh_max;
-        const uint32_t oslot = pick(r.own, o);
-        const uint32_t b = r.board;
-        if (b != cur_b) {
-            cur_b = b;
-            mo = T.sd_cnt[o][b];
-            const uint16_t* __restrict__ oidx = T.sd_idx[o] + T.sd_off[o][b];
-            const uint32_t* __restrict__ coff = T.sd_coff[o] + (size_t)b * 53;
-            c0 = 0;
-            len = 0;
-            if (lane < 52) {
-                c0 = coff[lane];
-                len = coff[lane + 1] - c0;
-            }
+    for (uint32_t n = 0; n < cnt; ++n) {
+        const ShowNode nd = nodes[n];
+        float* __restrict__ out = S.cfv + (size_t)nd.cfv * T.nh_max;
+        const bool live = nd.live != 0;
+        if (live) {
 #pragma unroll
-            for (int u = 0; u < kMaxH; ++u)
-                if (u * kWave < (int)mo) oi[u] = oidx[min(lane + u * kWave, mo - 1)];
-            const uint16_t* cp_src = T.sd_cpos_t[o] + (size_t)b * (kMaxCardList * 52) + min(lane, 51u);
-#pragma unroll
-            for (int t = 0; t < kMaxCardList; t += 2)
-                pos2[t / 2] = (uint32_t)cp_src[t * 52] | ((uint32_t)cp_src[(t + 1) * 52] << 16);
+            for (int u = 0; u < kRowPer; ++u)
+                if (tid + u * kBlock < nho) rhs[tid + u * kBlock] = rh[u];
         }
-        if (!S.alive[o][oslot] || mo == 0) {  // no reach, or no opponent hand on this board: all 0
-            for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
+        // 1. the next node's reach row, in flight during this node
+        if (n + 1 < cnt) {
+            const float* __restrict__ ro = S.reach[o] + (size_t)nodes[n + 1].oslot * nho;
+#pragma unroll
+            for (int u = 0; u < kRowPer; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
+        }
+        if (!live) {  // no reach, or no opponent hand on this board: all 0
+            for (uint32_t i = tid; i < nhp; i += kBlock) out[i] = 0.0f;
             continue;
         }
-        const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
-        float rh[kMaxH];
-#pragma unroll
-        for (int u = 0; u < kMaxH; ++u)
-            if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
-        // LDS: P (mo+1 doubles) | card prefixes (max(2mo+52, nho/2) doubles;
-        // the hand-order row is staged there first) | rs (mo floats)
-        const uint32_t creg = max(2 * mo + 52, (nho + 1) / 2);
-        double* P = (double*)base;
-        double* cpre = P + (mo + 1);
-        float* rs = (float*)(cpre + creg);
-        float* rhs = (float*)cpre;
-#pragma unroll
-        for (int u = 0; u < kMaxH; ++u)
-            if (lane + u * kWave < nho) rhs[lane + u * kWave] = rh[u];
-        // the first batch of this node's per-hand entries: in flight during
-        // the LDS phases
-        const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
-        uint4 ent[kShowBatch];
-#pragma unroll
-        for (int u = 0; u < kShowBatch; ++u) ent[u] = tab[min(lane + u * kWave, nhp - 1)];
-        wave_sync();
-#pragma unroll
-        for (int u = 0; u < kMaxH; ++u)
-            if (lane + u * kWave < mo) rs[lane + u * kWave] = rhs[oi[u]];
-        wave_sync();
-        // exclusive prefix over strength order: lane-contiguous chunks + a
-        // wavefront scan of the chunk totals
-        const uint32_t ch = (mo + kWave - 1) / kWave;
-        const uint32_t lo = min(lane * ch, mo), hi = min(lo + ch, mo);
+        __syncthreads();
+        // 2. strength order (also scattered into the per-card arrays) and the
+        //    chunk sums of the exclusive prefix
+        float v[kRowPer];
         double part = 0.0;
-        for (uint32_t k = lo; k < hi; ++k) part += (double)rs[k];
+#pragma unroll
+        for (int u = 0; u < kRowPer; ++u) {
+            const uint32_t k = min(lo + u, mo - 1);
+            v[u] = rhs[oidx[k]];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowPer; ++u) {
+            const uint32_t k = lo + u;
+            if (k < hi) {
+                const uint32_t ks = kslot[k];
+                rs[k] = v[u];
+                rcard[ks & 0xFFFFu] = v[u];
+                rcard[ks >> 16] = v[u];
+                part += (double)v[u];
+            }
+        }
         double incl = part;
         for (int off = 1; off < kWave; off <<= 1) {
-            double y = __shfl_up(incl, off);
+            const double y = __shfl_up(incl, off);
             if ((int)lane >= off) incl += y;
         }
-        double run = __shfl_up(incl, 1);
-        if (lane == 0) run = 0.0;
-        const double total = __shfl(incl, kWave - 1);
-        for (uint32_t k = lo; k < hi; ++k) {
-            P[k] = run;
-            run += (double)rs[k];
+        double excl = __shfl_up(incl, 1);
+        if (lane == 0) excl = 0.0;
+        if (lane == kWave - 1) wsum[wave] = incl;
+        __syncthreads();
+        double before = 0.0, total = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < kWaves; ++w2) {
+            const double t = wsum[w2];
+            before += (uint32_t)w2 < wave ? t : 0.0;
+            total += t;
         }
-        if (lane == 0) P[mo] = total;
-        // per-card exclusive prefix in strength order (the CPU's order), one
-        // lane per card
-        if (lane < 52) {
-            double* cp = cpre + c0 + lane;
+        double run = before + excl;
+#pragma unroll
+        for (int u = 0; u < kRowPer; ++u) {
+            const uint32_t k = lo + u;
+            if (k < hi) {
+                P[k] = run;
+                run += (double)v[u];
+            }
+        }
+        if (tid == 0) P[mo] = total;
+        // 3. per-card exclusive prefixes in strength order
+        if (card < 52) {
+            float x[kSeg];
+#pragma unroll
+            for (int u = 0; u < (int)kSeg; ++u) x[u] = rcard[c0 + min(s0 + u, len > 0 ? len - 1 : 0)];
             double a = 0.0;
 #pragma unroll
-            for (int t0 = 0; t0 < kMaxCardList; t0 += 8) {
-                float v[8];
+            for (int u = 0; u < (int)kSeg; ++u) a += (uint32_t)u < sn ? (double)x[u] : 0.0;
+            // exclusive scan over the card's four segments (lanes 4*card..+3)
+            const uint32_t l0 = lane & ~3u;
+            const double a0 = __shfl(a, l0), a1 = __shfl(a, l0 + 1), a2 = __shfl(a, l0 + 2), a3 = __shfl(a, l0 + 3);
+            double y = 0.0;
+            if (seg > 0) y += a0;
+            if (seg > 1) y += a1;
+            if (seg > 2) y += a2;
+            double* cp = cpre + c0 + card + s0;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const uint32_t w2 = pos2[(t0 + u) / 2];
-                    v[u] = rs[((t0 + u) & 1) ? (w2 >> 16) : (w2 & 0xFFFFu)];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (t0 + u < (int)len) cp[t0 + u] = a;
-                    a += (t0 + u) < (int)len ? (double)v[u] : 0.0;
-                }
+            for (int u = 0; u < (int)kSeg; ++u) {
+                if ((uint32_t)u < sn) cp[u] = y;
+                y += (uint32_t)u < sn ? (double)x[u] : 0.0;
             }
-            cp[len] = a;
+            if (seg == 3) cpre[c0 + card + len] = ((a0 + a1) + a2) + a3;
         }
-        wave_sync();
-        const float win = swap ? r.lose : r.win, lose = swap ? r.win : r.lose, tie = r.tie;
-#pragma unroll 1
-        for (uint32_t i0 = lane;; i0 += kShowBatch * kWave) {
+        __syncthreads();
+        // 4. per-hand values
+        const float win = nd.win, lose = nd.lose, tie = nd.tie;
 #pragma unroll
-            for (int u = 0; u < kShowBatch; ++u) {
-                const uint32_t i = i0 + u * kWave;
-                if (i >= nhp) break;
-                const uint4 e = ent[u];
-                if (e.x == 0xFFFFFFFFu) {
-                    out[i] = 0.0f;
-                    continue;
-                }
-                const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
-                const double* cp1 = cpre + (e.y & 0xFFFFu);
-                const double* cp2 = cpre + (e.y >> 16);
-                const double S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
-                const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
-                const float higher =
-                    (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
-                const uint32_t sp = e.w >> 16;
-                const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
-                const float valid = (float)(total - S1 - S2) + same_r;
-                const float tied = valid - lower - higher;
-                out[i] = win * lower + lose * higher + tie * tied;
+        for (int u = 0; u < kRowPer; ++u) {
+            const uint32_t i = tid + u * kBlock;
+            if (i >= nhp) break;
+            const uint4 e = ent[i];
+            if (e.x == 0xFFFFFFFFu) {
+                out[i] = 0.0f;
+                continue;
             }
-            const uint32_t n0 = i0 + kShowBatch * kWave;
-            if (n0 - lane >= nhp) break;
-#pragma unroll
-            for (int u = 0; u < kShowBatch; ++u) ent[u] = tab[min(n0 + u * kWave, nhp - 1)];
+            const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
+            const double* cp1 = cpre + (e.y & 0xFFFFu);
+            const double* cp2 = cpre + (e.y >> 16);
+            const double S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
+            const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
+            const float higher =
+                (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
+            const uint32_t sp = e.w >> 16;
+            const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
+            const float valid = (float)(total - S1 - S2) + same_r;
+            const float tied = valid - lower - higher;
+            out[i] = win * lower + lose * higher + tie * tied;
         }
-        wave_sync();  // this node's LDS reads complete before the next node's writes
+        __syncthreads();  // this node's LDS reads complete before the next node's writes
     }
 }
 

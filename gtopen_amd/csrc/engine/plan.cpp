@@ -187,9 +187,18 @@ Plan build_plan(const gto_spot_view& v) {
             }
             pl.sd_cpos_base[q][b] = (uint32_t)pl.sd_cpos[q].size();
             uint32_t acc = 0;
+            const size_t ks = pl.sd_kslot[q].size();
+            pl.sd_kslot[q].resize(ks + 2 * (size_t)cnt);
             for (int c = 0; c < 52; ++c) {
                 pl.sd_coff[q][(size_t)b * 53 + c] = acc;
-                for (uint16_t pos : by_card[c]) pl.sd_cpos[q].push_back(pos);
+                for (size_t t = 0; t < by_card[c].size(); ++t) {
+                    const uint16_t pos = by_card[c][t];
+                    pl.sd_cpos[q].push_back(pos);
+                    // first or second card of the hand at sorted position pos
+                    const uint16_t h = v.river_idx[q][off + pos];
+                    const int which = v.hand_c1[q][h] == c ? 0 : 1;
+                    pl.sd_kslot[q][ks + 2 * (size_t)pos + which] = (uint16_t)(acc + t);
+                }
                 acc += (uint32_t)by_card[c].size();
             }
             pl.sd_coff[q][(size_t)b * 53 + 52] = acc;
@@ -284,6 +293,7 @@ Plan build_plan(const gto_spot_view& v) {
                 for (uint32_t a = 0; a < r.na; ++a) pl.kids.push_back(pl.owner[q][v.children[cs + a]]);
                 ll.act_rec[q].push_back(r);
             }
+
         for (uint32_t n : ll.fold) {
             FoldRec r{};
             r.cfv = pl.cfv_slot[n];
@@ -328,6 +338,20 @@ Plan build_plan(const gto_spot_view& v) {
             r.card = ll.edge_card[e];
             ll.edge_rec.push_back(r);
         }
+    }
+
+    // all-level terminal lists; showdown runs per board
+    for (const LevelLists& ll : pl.levels) {
+        pl.fold_all.insert(pl.fold_all.end(), ll.fold_rec.begin(), ll.fold_rec.end());
+        pl.show_all.insert(pl.show_all.end(), ll.show_rec.begin(), ll.show_rec.end());
+    }
+    std::stable_sort(pl.show_all.begin(), pl.show_all.end(),
+                     [](const ShowRec& a, const ShowRec& b) { return a.board < b.board; });
+    for (uint32_t k = 0; k < pl.show_all.size();) {
+        uint32_t e = k;
+        while (e < pl.show_all.size() && e - k < kShowChunk && pl.show_all[e].board == pl.show_all[k].board) ++e;
+        pl.show_chunks.insert(pl.show_chunks.end(), {pl.show_all[k].board, k, e - k, 0u});
+        k = e;
     }
 
     // per-card hand lists (fold terminals)

@@ -27,6 +27,8 @@ struct LevelLists {
     std::vector<EdgeRec> edge_rec;
 };
 
+constexpr uint32_t kShowChunk = 16;    // showdown nodes of one board per workgroup run
+
 struct Plan {
     uint32_t num_nodes = 0;
     uint32_t nh[2] = {0, 0};
@@ -77,6 +79,10 @@ struct Plan {
     // per board, q's per-card position lists entry-major likewise:
     // [board][t * 52 + c], kCardListCap rows
     std::vector<uint16_t> sd_cpos_t[2];
+    // per board, for each of q's sorted hands: its slot in the per-card
+    // arrays of its two cards (coff[c] + index within card c's list), two
+    // u16 per hand, at sd_off[q][b] * 2
+    std::vector<uint16_t> sd_kslot[2];
 
     // children of action nodes referenced by ActRec::kid
     std::vector<uint32_t> kids;
@@ -88,6 +94,14 @@ struct Plan {
     // combo position (layout documented in build_plan)
     std::vector<uint32_t> sd_tab[2];
     uint32_t max_actions = 0;
+
+    // terminals of every level in one list each: their values depend only on
+    // the down pass, so one launch per sweep evaluates all of them
+    std::vector<FoldRec> fold_all;
+    std::vector<ShowRec> show_all;  // sorted by board slot
+    // runs of <= kShowChunk showdown records sharing a board:
+    // (board, first record, count, 0)
+    std::vector<uint32_t> show_chunks;
 
     // algorithmic arena entries over reached action nodes (sum na*nh_actor)
     uint64_t reached_entries = 0;

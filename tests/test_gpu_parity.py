@@ -202,3 +202,47 @@ def test_cfr_plus_matches_oracle(built):
         eg, ec = gpu.exploitability(), cpu.exploitability()
         _say(f"  iter {10 * (k + 1)}: gpu {eg:.6f} cpu {ec:.6f}")
         assert abs(eg - ec) <= POT_TOL * pot
+
+
+def test_turn_long_trajectory_compressed(built):
+    """150 iterations of the compressed engine on the full-range turn spot
+    (config 2) against the oracle every 25: quantization snaps both engines
+    to the same values, so the trajectories agree to the tolerance."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: turn spot, compressed arenas, 150 iterations, check every 25")
+    spot = Spot(workloads.turn_spot())
+    gpu, cpu = GpuSolver(spot, storage="compressed"), oracle.Oracle(spot, storage="compressed")
+    pot = spot.starting_pot
+    for k in range(6):
+        gpu.iterate(25)
+        cpu.iterate(25)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {25 * (k + 1)}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) <= POT_TOL * pot
+
+
+def test_turn_long_trajectory_f32_against_fp64(built):
+    """150 f32 iterations on the full-range turn spot. Over long runs two
+    f32 engines drift apart at the level of f32 rounding (the CPU f32 engine
+    itself sits ~2e-3 chips from the FP64 trajectory here), so the GPU is
+    held to the FP64 oracle: it may deviate from FP64 by no more than the
+    CPU f32 engine does, plus 1e-4 of the pot; the first 50 iterations must
+    also agree with the CPU f32 engine within 1e-4 of the pot."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: turn spot, f32 arenas, 150 iterations vs CPU f32 and FP64 oracles, check every 25")
+    spot = Spot(workloads.turn_spot())
+    gpu, cpu, ref = GpuSolver(spot), oracle.Oracle(spot), oracle.Oracle(spot, precision=64)
+    pot = spot.starting_pot
+    for k in range(6):
+        gpu.iterate(25)
+        cpu.iterate(25)
+        ref.iterate(25)
+        eg, ec, e64 = gpu.exploitability(), cpu.exploitability(), ref.exploitability()
+        _say(f"  iter {25 * (k + 1)}: gpu {eg:.6f} cpu f32 {ec:.6f} fp64 {e64:.6f}")
+        assert abs(eg - e64) <= abs(ec - e64) + POT_TOL * pot
+        if k < 2:
+            assert abs(eg - ec) <= POT_TOL * pot
