@@ -42,7 +42,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="flop_srp", choices=["river", "turn", "flop_srp"])
+    ap.add_argument("--config", default="flop_srp", choices=["river", "turn", "flop_srp", "batched_flops"])
+    ap.add_argument("--flops", type=int, default=184, help="batched_flops: number of seeded flops")
+    ap.add_argument("--flop-iters", type=int, default=300, help="batched_flops: DCFR iterations per flop")
     ap.add_argument("--target", type=float, default=0.3, help="exploitability target, % of pot")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
@@ -66,6 +68,9 @@ def main():
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(the tree would shard by turn card across GPUs; not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "batched_flops":
+        return bench_batched(args)
 
     from gtopen_amd import GpuSolver, Spot, workloads
 
@@ -173,6 +178,60 @@ def main():
         "whole_iteration_arena_GBps": round(whole, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args):
+    """Batched flops: seeded canonical flops with per-flop bet-size menus
+    (uneven trees), full-range Beta(0.7,0.7) ranges, solved in waves of
+    engines that iterate together (one parent-graph launch per iteration),
+    a fixed number of DCFR iterations each; value = flops/hour."""
+    from gtopen_amd import Spot, workloads
+    from gtopen_amd.reports import run_report
+
+    cfgs = workloads.batched_flops(n=args.flops)
+    done = []
+
+    def progress(row):
+        done.append(row)
+        log(f"bench: flop {len(done)}/{len(cfgs)} {row['board']} ({row['nodes']} nodes): "
+            f"{row['exploit_pct']:.3f}% pot after {row['iterations']} iterations, {row['seconds']:.1f}s in its wave")
+
+    t0 = time.perf_counter()
+    rep = run_report([(c, 1) for c in cfgs], target=0.0, max_iterations=args.flop_iters,
+                     check_every=args.flop_iters, progress=progress, name="bench")
+    dt = time.perf_counter() - t0
+    per_hour = len(rep["flops"]) / dt * 3600.0
+    log(f"bench: {len(rep['flops'])} flops x {args.flop_iters} iterations in {dt:.1f}s ({rep['waves']} waves)")
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        threads = os.cpu_count() or 1
+        oracle.set_threads(threads)
+        o = oracle.Oracle(Spot(cfgs[0]))
+        t1 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t1 < args.cpu_seconds and n < args.flop_iters:
+            o.iterate(1)
+            n += 1
+        cdt = time.perf_counter() - t1
+        cpu = {"value": round(n / cdt / args.flop_iters * 3600.0, 2), "unit": "flops/hour", "cores": threads,
+               "kind": "port",
+               "sample": f"first {n} DCFR iterations of flop 1 ({cfgs[0].board}) on the C++ oracle, "
+                         f"scaled to {args.flop_iters} iterations per flop"}
+        log(f"bench: cpu oracle {n} iterations of flop 1 in {cdt:.2f}s on {threads} threads")
+    out = {
+        "metric": f"flops/hour, {len(cfgs)} batched flops x {args.flop_iters} DCFR iterations (f32 arenas)",
+        "value": round(per_hour, 2), "unit": "flops/hour", "n_gpus": 1, "steps": args.flop_iters, "warmup": 0,
+        "ms_per_step": round(dt * 1e3 / max(1, args.flop_iters), 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched_flops: {len(cfgs)} seeded canonical flops, per-flop menus from "
+                               "{25,33,50,75,125,a}, 1176-combo Beta(0.7,0.7) ranges",
+                   "nodes_total": int(sum(r["nodes"] for r in rep["flops"])), "waves": rep["waves"],
+                   "parallelism": "single GPU, engines of a wave in one parent hipGraph"},
+        "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -89,8 +89,8 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
     meta = {"name": name, "target_pct": target, "max_iterations": max_iterations, "storage": storage}
     rows = []
     _write(path, meta, rows, False)
-    free, _total = vram_info(device)
-    budget = free * mem_fraction
+    free_mb, _total_mb = vram_info(device)
+    budget = free_mb * (1 << 20) * mem_fraction
     st = L.STORAGE_F32 if storage == "f32" else L.STORAGE_COMPRESSED
     k = 0
     waves = 0

@@ -144,6 +144,7 @@ class GpuSolver(QueryMixin):
 
 
 def vram_info(device=0):
+    """(free, total) device memory in MiB (vram_info_mb, gpu/mod.rs:953)."""
     fr, tot = ctypes.c_uint64(), ctypes.c_uint64()
     L.check(L.lib().gto_hip_vram_info(device, ctypes.byref(fr), ctypes.byref(tot)), "vram info")
     return fr.value, tot.value
