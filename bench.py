@@ -29,8 +29,9 @@ def measured_traffic(args, kind):
     if not os.path.exists(path):
         return None
     prof = json.load(open(path))
-    key = {"up_action": "k_up_action<0, %s>" % ("true" if args.storage == "compressed" else "false"),
-           "down_action": "k_down_action<%d>" % (2 if args.storage == "compressed" else 0)}.get(kind)
+    store = {"f32": 0, "compressed": 1, "half": 2}[args.storage]
+    key = {"up_action": "k_up_action<0, %d>" % store,
+           "down_action": "k_down_action<%d>" % {0: 0, 1: 2, 2: 5}[store]}.get(kind)
     k = prof["kernels"].get(key)
     if not k:
         return None
@@ -49,7 +50,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-target", action="store_true")
-    ap.add_argument("--storage", default="f32", choices=["f32", "compressed"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -168,7 +169,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.storage == "f32" else "i16/u16 arenas, f32 math",
+        "dtype": {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math"}[args.storage],
         "data": "synthetic",
         "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
                    "hands": list(spot.num_hands), "arena_entries_D": D, "parallelism": "single GPU"},

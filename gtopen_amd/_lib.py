@@ -21,7 +21,8 @@ c_f64p = ctypes.POINTER(ctypes.c_double)
 GTO_OK = 0
 KIND_ACTION, KIND_CHANCE, KIND_FOLD, KIND_SHOWDOWN = 0, 1, 2, 3
 SENTINEL = 0xFFFFFFFF
-STORAGE_F32, STORAGE_COMPRESSED = 0, 1
+STORAGE_F32, STORAGE_COMPRESSED, STORAGE_HALF = 0, 1, 2
+STORAGE = {"f32": STORAGE_F32, "compressed": STORAGE_COMPRESSED, "half": STORAGE_HALF}
 ALGO_DCFR, ALGO_CFRPLUS = 0, 1
 STRICT, LENIENT_LOAD = 0, 1
 PROF_KINDS = 8

@@ -51,6 +51,10 @@ struct DevLevel {
 
 // Workgroups for a list of `count` one-wavefront items, padded to a
 // multiple of the 8 XCDs (see xcd_slice).
+// IEEE half <-> float on the host (round to nearest even)
+inline float half_to_float(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
+inline uint16_t float_to_half(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
 uint32_t wave_grid(uint32_t count, uint32_t waves = kWaves) {
     uint32_t blocks = (count + waves - 1) / waves;
     return (blocks + 7u) & ~7u;
@@ -74,9 +78,10 @@ void discounts(int algo, uint32_t it, float* out3) {
 class Engine {
    public:
     Engine(const gto_spot_view& v, const gto_hip_options& o) : opts_(o) {
-        if (o.storage != GTO_STORAGE_F32 && o.storage != GTO_STORAGE_COMPRESSED)
+        if (o.storage != GTO_STORAGE_F32 && o.storage != GTO_STORAGE_COMPRESSED && o.storage != GTO_STORAGE_HALF)
             throw std::runtime_error("unknown storage mode");
-        compressed_ = o.storage == GTO_STORAGE_COMPRESSED;
+        compressed_ = o.storage != GTO_STORAGE_F32;
+        half_ = o.storage == GTO_STORAGE_HALF;
         HIP_OK(hipSetDevice(o.device));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         plan_ = build_plan(v);
@@ -225,6 +230,17 @@ class Engine {
                     out[a * nh + i] =
                         sum > 1e-12 ? (float)((double)std::max(b[a * nh + i], 0.0f) / sum) : 1.0f / (float)na;
             }
+        } else if (half_) {
+            std::vector<uint16_t> q(cnt);
+            HIP_OK(hipMemcpyAsync(q.data(), S_.Sq[p] + off, sizeof(uint16_t) * cnt, hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            for (size_t i = 0; i < nh; ++i) {
+                double sum = 0.0;
+                for (size_t a = 0; a < na; ++a) sum += (double)std::max(half_to_float(q[a * nh + i]), 0.0f);
+                for (size_t a = 0; a < na; ++a)
+                    out[a * nh + i] = sum > 0.0 ? (float)((double)std::max(half_to_float(q[a * nh + i]), 0.0f) / sum)
+                                                : 1.0f / (float)na;
+            }
         } else {
             std::vector<uint16_t> q(cnt);
             HIP_OK(hipMemcpyAsync(q.data(), S_.Sq[p] + off, sizeof(uint16_t) * cnt, hipMemcpyDeviceToHost, stream_));
@@ -332,6 +348,10 @@ class Engine {
             if (node_kind_[n] != GTO_KIND_ACTION || node_player_[n] != p) continue;
             size_t cnt = (size_t)node_na_[n] * plan_.nh[p];
             uint64_t off = node_off_[n];
+            if (half_) {  // f16 value times the node's power-of-two scale (exact)
+                for (size_t e = 0; e < cnt; ++e) dst[off + e] = half_to_float(q[off + e]) * sc[n];
+                continue;
+            }
             if (which == 0) {
                 float k = sc[n] / 32767.0f;
                 for (size_t e = 0; e < cnt; ++e) dst[off + e] = (float)(int16_t)q[off + e] * k;
@@ -356,6 +376,10 @@ class Engine {
             HIP_OK(hipMemcpyAsync(which == 0 ? S_.R[p] : S_.S[p], src, sizeof(float) * data_size_[p],
                                   hipMemcpyHostToDevice, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
+            return;
+        }
+        if (half_) {
+            upload_half(which, p, src);
             return;
         }
         // encode per action node against its block max (store.rs:199-215)
@@ -394,6 +418,41 @@ class Engine {
         HIP_OK(hipMemcpyAsync(which == 0 ? (void*)S_.Rq[p] : (void*)S_.Sq[p], q.data(),
                               sizeof(uint16_t) * data_size_[p], hipMemcpyHostToDevice, stream_));
         HIP_OK(hipMemcpyAsync(which == 0 ? S_.Rs[p] : S_.Ss[p], sc.data(), sizeof(float) * num_nodes_,
+                              hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+
+    // f16 arenas: per node, scale = half_scale(max |v|), values rounded to
+    // nearest even; the node's max is kept for the next update's bound.
+    void upload_half(int which, int p, const float* src) {
+        std::vector<uint16_t> q(data_size_[p], 0);
+        std::vector<float> sc(num_nodes_, 1.0f), mx(num_nodes_, 0.0f);
+        HIP_OK(hipMemcpyAsync(sc.data(), which == 0 ? S_.Rs[p] : S_.Ss[p], sizeof(float) * num_nodes_,
+                              hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipMemcpyAsync(mx.data(), which == 0 ? S_.Rm[p] : S_.Sm[p], sizeof(float) * num_nodes_,
+                              hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (uint32_t n = 0; n < num_nodes_; ++n) {
+            if (node_kind_[n] != GTO_KIND_ACTION || node_player_[n] != p) continue;
+            const size_t cnt = (size_t)node_na_[n] * plan_.nh[p];
+            const uint64_t off = node_off_[n];
+            float m = 0.0f;
+            for (size_t e = 0; e < cnt; ++e) {
+                const float v = which == 0 ? src[off + e] : std::fmax(src[off + e], 0.0f);
+                if (!std::isfinite(v)) throw std::runtime_error("non-finite arena value for node " + std::to_string(n));
+                m = std::fmax(m, std::fabs(v));
+            }
+            sc[n] = half_scale(m);
+            mx[n] = m;
+            const float inv = 1.0f / sc[n];
+            for (size_t e = 0; e < cnt; ++e)
+                q[off + e] = float_to_half((which == 0 ? src[off + e] : std::fmax(src[off + e], 0.0f)) * inv);
+        }
+        HIP_OK(hipMemcpyAsync(which == 0 ? (void*)S_.Rq[p] : (void*)S_.Sq[p], q.data(),
+                              sizeof(uint16_t) * data_size_[p], hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipMemcpyAsync(which == 0 ? S_.Rs[p] : S_.Ss[p], sc.data(), sizeof(float) * num_nodes_,
+                              hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipMemcpyAsync(which == 0 ? S_.Rm[p] : S_.Sm[p], mx.data(), sizeof(float) * num_nodes_,
                               hipMemcpyHostToDevice, stream_));
         HIP_OK(hipStreamSynchronize(stream_));
     }
@@ -456,7 +515,8 @@ class Engine {
 
    private:
     gto_hip_options opts_;
-    bool compressed_ = false;
+    bool compressed_ = false;  // 16-bit arenas (i16/u16 quants or f16)
+    bool half_ = false;        // f16 arenas with per-node power-of-two scales
     std::vector<uint8_t> node_kind_, node_player_, node_na_;
     std::vector<uint64_t> node_off_;
     hipStream_t stream_ = nullptr;
@@ -608,6 +668,12 @@ class Engine {
                 HIP_OK(hipMemset(S_.Sq[q], 0, sizeof(uint16_t) * ds));
                 HIP_OK(hipMemset(S_.Rs[q], 0, sizeof(float) * num_nodes_));
                 HIP_OK(hipMemset(S_.Ss[q], 0, sizeof(float) * num_nodes_));
+                if (half_) {
+                    S_.Rm[q] = (float*)dalloc(sizeof(float) * num_nodes_);
+                    S_.Sm[q] = (float*)dalloc(sizeof(float) * num_nodes_);
+                    HIP_OK(hipMemset(S_.Rm[q], 0, sizeof(float) * num_nodes_));
+                    HIP_OK(hipMemset(S_.Sm[q], 0, sizeof(float) * num_nodes_));
+                }
             } else {
                 S_.R[q] = (float*)dalloc(sizeof(float) * ds);
                 S_.S[q] = (float*)dalloc(sizeof(float) * ds);
@@ -616,9 +682,18 @@ class Engine {
             }
             S_.reach[q] = (float*)dalloc(sizeof(float) * (size_t)plan_.num_slots[q] * plan_.nh[q]);
             S_.alive[q] = (uint8_t*)dalloc(plan_.num_slots[q]);
+            S_.mass[q] = (float*)dalloc(sizeof(float) * plan_.num_slots[q]);
+            S_.rmax[q] = (float*)dalloc(sizeof(float) * plan_.num_slots[q]);
+            HIP_OK(hipMemset(S_.mass[q], 0, sizeof(float) * plan_.num_slots[q]));
+            HIP_OK(hipMemset(S_.rmax[q], 0, sizeof(float) * plan_.num_slots[q]));
             HIP_OK(hipMemset(S_.alive[q], 0, plan_.num_slots[q]));
         }
         S_.cfv = (float*)dalloc(sizeof(float) * (size_t)plan_.num_reached * plan_.nh_max);
+        S_.vmax = nullptr;
+        if (half_) {
+            S_.vmax = (float*)dalloc(sizeof(float) * std::max<uint32_t>(plan_.num_reached, 1));
+            HIP_OK(hipMemset(S_.vmax, 0, sizeof(float) * std::max<uint32_t>(plan_.num_reached, 1)));
+        }
         S_.iter = (uint32_t*)dalloc(sizeof(uint32_t));
         HIP_OK(hipMemset(S_.iter, 0, sizeof(uint32_t)));
         ensure_disc(1024);
@@ -704,10 +779,12 @@ class Engine {
                 if (kind == kRawF32) {
                     launch(PK_DOWN_ACTION, k_down_action<kRawF32>, g, 0, T_, S_, recs, n, q);
                 } else if (kind == kAverageF32) {
-                    if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, g, 0, T_, S_, recs, n, q);
+                    if (half_) launch(PK_DOWN_ACTION, k_down_action<kAverageH>, g, 0, T_, S_, recs, n, q);
+                    else if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, g, 0, T_, S_, recs, n, q);
                     else launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, g, 0, T_, S_, recs, n, q);
                 } else {
-                    if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kCurrentQ>, g, 0, T_, S_, recs, n, q);
+                    if (half_) launch(PK_DOWN_ACTION, k_down_action<kCurrentH>, g, 0, T_, S_, recs, n, q);
+                    else if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kCurrentQ>, g, 0, T_, S_, recs, n, q);
                     else launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, g, 0, T_, S_, recs, n, q);
                 }
             }
@@ -737,20 +814,27 @@ class Engine {
                 if (!D.n_act[q]) continue;
                 const ActRec* recs = D.act[q];
                 const uint32_t n = D.n_act[q], g = wave_grid(n);
-                if (compressed_) {
+                if (half_) {
                     switch (mode) {
-                        case 0: launch(PK_UP_ACTION, k_up_action<0, true>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 1: launch(PK_UP_ACTION, k_up_action<1, true>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 2: launch(PK_UP_ACTION, k_up_action<2, true>, g, 0, T_, S_, recs, n, p, q); break;
-                        default: launch(PK_UP_ACTION, k_up_action<3, true>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 0: launch(PK_UP_ACTION, k_up_action<0, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
+                        default: launch(PK_UP_ACTION, k_up_action<3, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
+                    }
+                } else if (compressed_) {
+                    switch (mode) {
+                        case 0: launch(PK_UP_ACTION, k_up_action<0, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
+                        default: launch(PK_UP_ACTION, k_up_action<3, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
                     }
                 } else {
                     switch (mode) {
-                        case 0: launch(PK_UP_ACTION, k_up_action<0, false>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 1: launch(PK_UP_ACTION, k_up_action<1, false>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 2: launch(PK_UP_ACTION, k_up_action<2, false>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 4: launch(PK_UP_ACTION, k_up_action<4, false>, g, 0, T_, S_, recs, n, p, q); break;
-                        default: launch(PK_UP_ACTION, k_up_action<3, false>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 0: launch(PK_UP_ACTION, k_up_action<0, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 4: launch(PK_UP_ACTION, k_up_action<4, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
+                        default: launch(PK_UP_ACTION, k_up_action<3, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
                     }
                 }
             }

@@ -14,6 +14,23 @@ constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxUnroll = 8;     // actions cached in registers per hand
 constexpr int kMaxActions = 64;   // engine limit on actions per node (one ballot bit each)
 
+// Arena storage inside the kernels: f32; the reference's i16/u16 quants
+// against a per-node block max; f16 values times a per-node power of two.
+enum Store { kStoreF32 = 0, kStoreQ = 1, kStoreH = 2 };
+
+// Power-of-two scale for f16 storage of a node whose values are bounded by
+// b: b / scale <= 2^15, well inside f16's range (65504), so the bound may
+// overshoot by 2x without overflow; relative precision is f16's 2^-11 at
+// every magnitude down to ~2^-24 of the scale.
+__host__ __device__ inline float half_scale(float b) {
+    if (!(b > 0.0f) || !(b < 3.0e38f)) return 1.0f;
+    int e;
+    (void)frexpf(b, &e);  // b = m * 2^e, m in [0.5, 1)
+    e -= 15;
+    if (e < -126) e = -126;
+    return ldexpf(1.0f, e);
+}
+
 // Static hand and table data (read-only for the whole solve).
 struct DevTree {
     const float* lock_sigma;
@@ -83,8 +100,13 @@ struct DevState {
     uint16_t* Sq[2];      // compressed strategy sums
     float* Rs[2];         // per-node regret scale
     float* Ss[2];         // per-node strategy scale
+    float* Rm[2];         // half storage: per-node max |regret| after the last update
+    float* Sm[2];         // half storage: per-node max strategy sum
     float* reach[2];      // [slot][nh_q]
     uint8_t* alive[2];    // [slot]: any reach > 0
+    float* mass[2];       // [slot]: sum of the reach row (a bound for value magnitudes)
+    float* rmax[2];       // [slot]: max of the reach row
+    float* vmax;          // [value row]: max |value| of the row (f16 engines; null otherwise)
     float* cfv;           // [cfv_slot][nh_max], traverser's values
     const float* disc;    // [iteration][3]: pos, neg, strat
     uint32_t* iter;       // device iteration counter (1-based after the first begin)

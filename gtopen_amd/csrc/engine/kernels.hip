@@ -66,7 +66,15 @@ __device__ __forceinline__ uint16_t quant_u16(float x) {
 // ---------------------------------------------------------------------------
 //   RAW:     the strategy arena read as a behavioural strategy, sigma = S
 //            as stored (the game-value pass, whose strategy is an input).
-enum SigmaKind { kCurrentF32 = 0, kAverageF32 = 1, kCurrentQ = 2, kAverageQ = 3, kRawF32 = 4 };
+//   *H:      f16 storage: the same arithmetic on the stored f16 values as
+//            f32 (the per-node power-of-two scale cancels exactly), uniform
+//            where the sum is 0.
+enum SigmaKind { kCurrentF32 = 0, kAverageF32 = 1, kCurrentQ = 2, kAverageQ = 3, kRawF32 = 4, kCurrentH = 5,
+                 kAverageH = 6 };
+
+__device__ __forceinline__ float h2f(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
+// round to nearest even (IEEE conversion)
+__device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
 template <int KIND>
 struct SigmaHand;
@@ -197,6 +205,49 @@ struct SigmaHand<kAverageQ> {
 };
 
 template <>
+struct SigmaHand<kCurrentH> {
+    const uint16_t* base;
+    uint32_t nh, i;
+    float inv, u;
+    bool uniform;
+    __device__ void init(const uint16_t* b, uint32_t nh_, uint32_t i_, int na) {
+        base = b;
+        nh = nh_;
+        i = i_;
+        float sum = 0.0f;
+        for (int a = 0; a < na; ++a) sum += fmaxf(h2f(base[(size_t)a * nh + i]), 0.0f);
+        uniform = !(sum > 0.0f);
+        inv = uniform ? 0.0f : 1.0f / sum;
+        u = 1.0f / (float)na;
+    }
+    template <bool C>
+    __device__ float sigma(int a) const { return uniform ? u : fmaxf(h2f(base[(size_t)a * nh + i]), 0.0f) * inv; }
+};
+
+template <>
+struct SigmaHand<kAverageH> {
+    const uint16_t* base;
+    uint32_t nh, i;
+    double sum;
+    float u;
+    bool uniform;
+    __device__ void init(const uint16_t* b, uint32_t nh_, uint32_t i_, int na) {
+        base = b;
+        nh = nh_;
+        i = i_;
+        double s2 = 0.0;
+        for (int a = 0; a < na; ++a) s2 += (double)fmaxf(h2f(base[(size_t)a * nh + i]), 0.0f);
+        sum = s2;
+        uniform = !(s2 > 0.0);
+        u = 1.0f / (float)na;
+    }
+    template <bool C>
+    __device__ float sigma(int a) const {
+        return uniform ? u : (float)((double)fmaxf(h2f(base[(size_t)a * nh + i]), 0.0f) / sum);
+    }
+};
+
+template <>
 struct SigmaHand<kRawF32> {
     const float* base;
     uint32_t nh, i;
@@ -314,6 +365,51 @@ struct HandSigma<NA, kAverageQ> {
 };
 
 template <int NA>
+struct HandSigma<NA, kCurrentH> {
+    float raw[NA], sig[NA];
+    __device__ __forceinline__ void fetch(const uint16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) raw[a] = h2f(b[(size_t)a * nh + i]);
+    }
+    __device__ __forceinline__ void finish() {
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sum += fmaxf(raw[a], 0.0f);
+        const bool uni = !(sum > 0.0f);
+        const float inv = 1.0f / sum;
+        const float u = 1.0f / (float)NA;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sig[a] = uni ? u : fmaxf(raw[a], 0.0f) * inv;
+    }
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+        fetch(b, nh, i);
+        finish();
+    }
+};
+
+template <int NA>
+struct HandSigma<NA, kAverageH> {
+    float raw[NA], sig[NA];
+    __device__ __forceinline__ void fetch(const uint16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) raw[a] = h2f(b[(size_t)a * nh + i]);
+    }
+    __device__ __forceinline__ void finish() {
+        double sum = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sum += (double)fmaxf(raw[a], 0.0f);
+        const bool uni = !(sum > 0.0);
+        const float u = 1.0f / (float)NA;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) sig[a] = uni ? u : (float)((double)fmaxf(raw[a], 0.0f) / sum);
+    }
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ b, uint32_t nh, uint32_t i) {
+        fetch(b, nh, i);
+        finish();
+    }
+};
+
+template <int NA>
 struct HandSigma<NA, kRawF32> {
     float sig[NA];
     __device__ __forceinline__ void fetch(const float* __restrict__ b, uint32_t nh, uint32_t i) {
@@ -357,6 +453,14 @@ struct ArenaOf<kCurrentQ> {
 };
 template <>
 struct ArenaOf<kAverageQ> {
+    static __device__ const uint16_t* get(const DevState& S, int q) { return S.Sq[q]; }
+};
+template <>
+struct ArenaOf<kCurrentH> {
+    static __device__ const uint16_t* get(const DevState& S, int q) { return (const uint16_t*)S.Rq[q]; }
+};
+template <>
+struct ArenaOf<kAverageH> {
     static __device__ const uint16_t* get(const DevState& S, int q) { return S.Sq[q]; }
 };
 template <>
@@ -408,16 +512,31 @@ __device__ __forceinline__ float wave_max_f32(float v) {
 // Root: reach = range weights.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_root(DevTree T, DevState S, int mask) {
+    __shared__ float part[kWaves], pmax[kWaves];
     for (int q = 0; q < 2; ++q) {
         if (!((mask >> q) & 1)) continue;
         bool any = false;
+        float sum = 0.0f, mx = 0.0f;
         for (uint32_t i = threadIdx.x; i < T.nh[q]; i += kBlock) {
             float w = T.weights[q][i];
             S.reach[q][i] = w;
             any |= w > 0.0f;
+            sum += w;
+            mx = fmaxf(mx, w);
+        }
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        mx = wave_max_f32(mx);
+        if ((threadIdx.x & 63) == 0) {
+            part[threadIdx.x >> 6] = sum;
+            pmax[threadIdx.x >> 6] = mx;
         }
         any = __syncthreads_or(any);
-        if (threadIdx.x == 0) S.alive[q][0] = any ? 1 : 0;
+        if (threadIdx.x == 0) {
+            S.alive[q][0] = any ? 1 : 0;
+            S.mass[q][0] = ((part[0] + part[1]) + part[2]) + part[3];
+            S.rmax[q][0] = fmaxf(fmaxf(pmax[0], pmax[1]), fmaxf(pmax[2], pmax[3]));
+        }
+        __syncthreads();
     }
 }
 
@@ -448,11 +567,20 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
             float* dst = reach + (size_t)kid_own[a] * nh;
             for (uint32_t i = lane; i < nh; i += kWave) dst[i] = 0.0f;
         }
-        for (int a = lane; a < na; a += kWave) S.alive[q][kid_own[a]] = 0;
+        for (int a = lane; a < na; a += kWave) {
+            S.alive[q][kid_own[a]] = 0;
+            S.mass[q][kid_own[a]] = 0.0f;
+            S.rmax[q][kid_own[a]] = 0.0f;
+        }
         return;
     }
     const float* __restrict__ pr = reach + (size_t)src * nh;
     uint64_t anym = 0;
+    // per-child reach sums (bounds for value magnitudes); the first
+    // kMaxUnroll children in registers, the rest summed from memory below
+    float msum[kMaxUnroll], mmax[kMaxUnroll];
+#pragma unroll
+    for (int a = 0; a < kMaxUnroll; ++a) msum[a] = mmax[a] = 0.0f;
     if (r.lock_off >= 0) {
         const float* sig = T.lock_sigma + r.lock_off;
         for (uint32_t i = lane; i < nh; i += kWave) {
@@ -490,6 +618,8 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
                             float v = pv[h] * hs[h].sig[a];
                             dst[a][i] = v;
                             anym |= (uint64_t)(v > 0.0f) << a;
+                            msum[a] += v;
+                            mmax[a] = fmaxf(mmax[a], v);
                         }
                     }
                 }
@@ -507,9 +637,32 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
             }
         });
     }
+    const bool fast = r.lock_off < 0 && na <= kMaxUnroll;  // msum filled in the loops above
     for (int a = 0; a < na; ++a) {
         const bool any = __ballot((anym >> a) & 1ull) != 0ull;
-        if (lane == 0) S.alive[q][kid_own[a]] = any ? 1 : 0;
+        float m = 0.0f, mx = 0.0f;
+        if (fast) {
+#pragma unroll
+            for (int a2 = 0; a2 < kMaxUnroll; ++a2) {
+                m = a2 == a ? msum[a2] : m;
+                mx = a2 == a ? mmax[a2] : mx;
+            }
+        } else {
+            // this wavefront's own writes above: read back in order
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            const float* row = reach + (size_t)kid_own[a] * nh;
+            for (uint32_t i = lane; i < nh; i += kWave) {
+                m += row[i];
+                mx = fmaxf(mx, row[i]);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off);
+        mx = wave_max_f32(mx);
+        if (lane == 0) {
+            S.alive[q][kid_own[a]] = any ? 1 : 0;
+            S.mass[q][kid_own[a]] = m;
+            S.rmax[q][kid_own[a]] = mx;
+        }
     }
 }
 
@@ -531,6 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
         const bool alive = S.alive[q][pick(r.src, q)];
         const float* in = S.reach[q] + (size_t)pick(r.src, q) * nh;
         bool any = false;
+        float msum = 0.0f, mmx = 0.0f;
         for (uint32_t i0 = lane; i0 < nh; i0 += kBatch * kWave) {
             float v[kBatch];
             uint64_t m[kBatch];
@@ -547,10 +701,18 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
                 const float x = (!alive || (m[u] & cm)) ? 0.0f : v[u];
                 out[i] = x;
                 any |= x > 0.0f;
+                msum += x;
+                mmx = fmaxf(mmx, x);
             }
         }
         any = __ballot(any) != 0ull;
-        if (lane == 0) S.alive[q][pick(r.dst, q)] = any ? 1 : 0;
+        for (int off = 32; off > 0; off >>= 1) msum += __shfl_xor(msum, off);
+        mmx = wave_max_f32(mmx);
+        if (lane == 0) {
+            S.alive[q][pick(r.dst, q)] = any ? 1 : 0;
+            S.mass[q][pick(r.dst, q)] = msum;
+            S.rmax[q][pick(r.dst, q)] = mmx;
+        }
     }
 }
 
@@ -593,6 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     const uint32_t oslot = pick(r.own, o);
     if (!S.alive[o][oslot]) {
         for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
+        if (S.vmax && lane == 0) S.vmax[r.cfv] = 0.0f;
         return;
     }
     const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
@@ -642,6 +805,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     // swap: the opponent's payoff (win and lose exchanged), for game-value
     // gradients with respect to the opponent's range and strategy
     const float amount = (r.folder == (uint32_t)p) != (swap != 0) ? r.t_lose : r.t_win;
+    float vm = 0.0f;
 #pragma unroll
     for (int u = 0; u < kMaxH; ++u) {
         const uint32_t i = lane + u * kWave;
@@ -652,7 +816,12 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
             const float same_r = rsc[combo_of(a1, a2)];
             const float valid = (float)(tot - s_card[a1] - s_card[a2]) + same_r;
             out[i] = amount * valid;
+            vm = fmaxf(vm, fabsf(amount * valid));
         }
+    }
+    if (S.vmax) {
+        vm = wave_max_f32(vm);
+        if (lane == 0) S.vmax[r.cfv] = vm;
     }
 }
 
@@ -763,6 +932,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         }
         if (!live) {  // no reach, or no opponent hand on this board: all 0
             for (uint32_t i = tid; i < nhp; i += kBlock) out[i] = 0.0f;
+            if (S.vmax && tid == 0) S.vmax[nd.cfv] = 0.0f;
             continue;
         }
         __syncthreads();
@@ -838,6 +1008,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         __syncthreads();
         // 4. per-hand values
         const float win = nd.win, lose = nd.lose, tie = nd.tie;
+        float vm = 0.0f;
 #pragma unroll
         for (int u = 0; u < kRowPer; ++u) {
             const uint32_t i = tid + u * kBlock;
@@ -858,9 +1029,17 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
             const float valid = (float)(total - S1 - S2) + same_r;
             const float tied = valid - lower - higher;
-            out[i] = win * lower + lose * higher + tie * tied;
+            const float val = win * lower + lose * higher + tie * tied;
+            out[i] = val;
+            vm = fmaxf(vm, fabsf(val));
+        }
+        if (S.vmax) {
+            vm = wave_max_f32(vm);
+            if (lane == 0) wsum[wave] = (double)vm;
         }
         __syncthreads();  // this node's LDS reads complete before the next node's writes
+        if (S.vmax && tid == 0)
+            S.vmax[nd.cfv] = (float)fmax(fmax(wsum[0], wsum[1]), fmax(wsum[2], wsum[3]));
     }
 }
 
@@ -881,8 +1060,10 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
     float* out = S.cfv + (size_t)r.cfv * T.nh_max;
     if (!S.alive[o][pick(r.own, o)]) {
         for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) out[i] = 0.0f;
+        if (S.vmax && threadIdx.x == 0) S.vmax[r.cfv] = 0.0f;
         return;
     }
+    float vm = 0.0f;
     const uint8_t* __restrict__ cards = T.chr_card + r.off;
     const uint32_t* __restrict__ rows = T.chr_cfv + r.off;
     const float* __restrict__ cfv = S.cfv;
@@ -901,6 +1082,14 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
         for (; t < r.cnt; ++t)
             if (!((hm >> cards[t]) & 1ull)) acc += cfv[(size_t)rows[t] * T.nh_max + i];
         out[i] = acc * r.inv;
+        vm = fmaxf(vm, fabsf(acc * r.inv));
+    }
+    if (S.vmax) {
+        __shared__ float wm[kWaves];
+        vm = wave_max_f32(vm);
+        if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = vm;
+        __syncthreads();
+        if (threadIdx.x == 0) S.vmax[r.cfv] = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
     }
 }
 
@@ -921,21 +1110,19 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
 //     strategy itself (kRawF32), no update.
 // Subtrees whose opponent reach is all zero produce zeros and no update.
 // ---------------------------------------------------------------------------
-template <int MODE, bool COMPRESSED>
-__global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
-                                                      uint32_t count, int p, int q) {
-    const uint32_t w = wave_item();
-    if (w >= count) return;
+template <int MODE, int STORE>
+// Returns this lane's max |value| over the row entries it wrote.
+__device__ __forceinline__ float up_action_node(const DevTree& T, const DevState& S, const ActRec& r, int p, int q,
+                                                uint32_t lane) {
+    float vm = 0.0f;
     const int o = 1 - p;
-    const uint32_t lane = threadIdx.x & 63;
-    const ActRec r = recs[w];
     const uint32_t nhp = T.nh[p];
     const uint32_t nh_max = T.nh_max;
     const int na = r.na;
     float* __restrict__ out = S.cfv + (size_t)r.cfv * nh_max;
     if (!S.alive[o][pick(r.own, o)]) {
         for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
-        return;
+        return 0.0f;
     }
     const uint32_t* kid_cfv = T.kids + r.kid;
     const float* __restrict__ cfv = S.cfv;
@@ -975,6 +1162,7 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                             for (int a = 1; a < NA; ++a) acc = v[h][a] > acc ? v[h][a] : acc;
                         }
                         out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
                     }
                 }
             } else {
@@ -987,18 +1175,20 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                         else if (a > 0 && v > acc) acc = v;
                     }
                     out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
                 }
             }
         });
-        return;
+        return vm;
     }
     const uint64_t off = r.doff;
     if constexpr (MODE >= 2) {
         // value of the average (2), current (3) or given (4) strategy, no update
-        static_assert(MODE != 4 || !COMPRESSED, "game values read an f32 strategy arena");
-        constexpr int KIND = MODE == 4   ? kRawF32
-                             : MODE == 2 ? (COMPRESSED ? kAverageQ : kAverageF32)
-                                         : (COMPRESSED ? kCurrentQ : kCurrentF32);
+        static_assert(MODE != 4 || STORE == kStoreF32, "game values read an f32 strategy arena");
+        constexpr int KIND = MODE == 4 ? kRawF32
+                             : MODE == 2
+                                 ? (STORE == kStoreQ ? kAverageQ : STORE == kStoreH ? kAverageH : kAverageF32)
+                                 : (STORE == kStoreQ ? kCurrentQ : STORE == kStoreH ? kCurrentH : kCurrentF32);
         const auto* arena = ArenaOf<KIND>::get(S, p) + off;
         dispatch_na(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
@@ -1010,6 +1200,7 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
 #pragma unroll
                     for (int a = 0; a < NA; ++a) acc += hs.sig[a] * row(a)[i];
                     out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
                 }
             } else {
                 for (uint32_t i = lane; i < nhp; i += kWave) {
@@ -1018,15 +1209,16 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                     float acc = 0.0f;
                     for (int a = 0; a < na; ++a) acc += sh.template sigma<false>(a) * row(a)[i];
                     out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
                 }
             }
         });
-        return;
+        return vm;
     } else {
         const uint32_t t = *S.iter;
         const float pos = S.disc[3 * t + 0], neg = S.disc[3 * t + 1], ds = S.disc[3 * t + 2];
         const float* __restrict__ rp = S.reach[p] + (size_t)pick(r.own, p) * nhp;
-        if constexpr (!COMPRESSED) {
+        if constexpr (STORE == kStoreF32) {
             float* __restrict__ R = S.R[p] + off;
             float* __restrict__ Sm = S.S[p] + off;
             dispatch_na(na, [&](auto NAc) {
@@ -1070,6 +1262,7 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                                 Sm[(size_t)a * nhp + i] = sm[h][a] * ds + reach[h] * hs[h].sig[a];
                             }
                             out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
                         }
                     }
                 } else {
@@ -1088,9 +1281,111 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                             Sm[idx] = Sm[idx] * ds + reach * sg;
                         }
                         out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
                     }
                 }
             });
+        } else if constexpr (STORE == kStoreH) {
+            // f16 arenas with a power-of-two scale per node (no counterpart in
+            // the reference). The new scales come from bounds of the new
+            // maxima, known before any value is read, so every value is
+            // read and written once:
+            //   |R'| <= max|R| max(pos, neg) + |v - out| <= mR max(pos,neg) + 2 max|v|
+            //   S' = S ds + reach sigma <= mS ds + max reach
+            // with max|v| the children's row maxima, recorded by the kernels
+            // that wrote those rows.
+            // Decoding multiplies by a power of two (exact); the f32
+            // arithmetic is the f32 engine's on the decoded values.
+            uint16_t* __restrict__ Rh = (uint16_t*)S.Rq[p] + off;
+            uint16_t* __restrict__ Sh = S.Sq[p] + off;
+            const float sR = S.Rs[p][r.node], sS = S.Ss[p][r.node];
+            // max |v| over the children's rows (recorded by their writers);
+            // reach * sigma <= max reach (recorded by the down sweep)
+            float mv = 0.0f;
+            for (int a = 0; a < na; ++a) mv = fmaxf(mv, S.vmax[kid_cfv[a]]);
+            const float mr = S.rmax[p][pick(r.own, p)];
+            const float kR = half_scale(S.Rm[p][r.node] * fmaxf(pos, neg) + 2.0f * mv);
+            const float kS = half_scale(S.Sm[p][r.node] * ds + mr);
+            const float iR = 1.0f / kR, iS = 1.0f / kS;
+            float xr = 0.0f, xs = 0.0f;
+            dispatch_na(na, [&](auto NAc) {
+                constexpr int NA = decltype(NAc)::value;
+                if constexpr (NA > 0) {
+                    const float* rw[NA];
+#pragma unroll
+                    for (int a = 0; a < NA; ++a) rw[a] = row(a);
+                    for (uint32_t i0 = lane; i0 < nhp; i0 += 2 * kWave) {
+                        HandSigma<NA, kCurrentH> hs[2];
+                        float v[2][NA], sm[2][NA], reach[2];
+                        bool ok[2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const uint32_t i = i0 + h * kWave;
+                            ok[h] = i < nhp;
+                            const uint32_t ii = ok[h] ? i : i0;
+                            hs[h].fetch(Rh, nhp, ii);
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) {
+                                v[h][a] = rw[a][ii];
+                                sm[h][a] = h2f(Sh[(size_t)a * nhp + ii]);
+                            }
+                            reach[h] = rp[ii];
+                        }
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            if (!ok[h]) continue;
+                            const uint32_t i = i0 + h * kWave;
+                            hs[h].finish();
+                            float acc = 0.0f;
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) acc += hs[h].sig[a] * v[h][a];
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) {
+                                const float rv = hs[h].raw[a] * sR;
+                                const float d = rv > 0.0f ? pos : neg;
+                                const float nr = rv * d + (v[h][a] - acc);
+                                const float ns = (sm[h][a] * sS) * ds + reach[h] * hs[h].sig[a];
+                                Rh[(size_t)a * nhp + i] = f2h(nr * iR);
+                                Sh[(size_t)a * nhp + i] = f2h(ns * iS);
+                                xr = fmaxf(xr, fabsf(nr));
+                                xs = fmaxf(xs, ns);
+                            }
+                            out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
+                        }
+                    }
+                } else {
+                    for (uint32_t i = lane; i < nhp; i += kWave) {
+                        SigmaHand<kCurrentH> sh;
+                        sh.init(Rh, nhp, i, na);
+                        float acc = 0.0f;
+                        for (int a = 0; a < na; ++a) acc += sh.template sigma<false>(a) * row(a)[i];
+                        const float reach = rp[i];
+                        for (int a = 0; a < na; ++a) {
+                            const size_t idx = (size_t)a * nhp + i;
+                            const float rv = h2f(Rh[idx]) * sR;
+                            const float sg = sh.template sigma<false>(a);
+                            const float d = rv > 0.0f ? pos : neg;
+                            const float nr = rv * d + (row(a)[i] - acc);
+                            const float ns = (h2f(Sh[idx]) * sS) * ds + reach * sg;
+                            Rh[idx] = f2h(nr * iR);
+                            Sh[idx] = f2h(ns * iS);
+                            xr = fmaxf(xr, fabsf(nr));
+                            xs = fmaxf(xs, ns);
+                        }
+                        out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
+                    }
+                }
+            });
+            xr = wave_max_f32(xr);
+            xs = wave_max_f32(xs);
+            if (lane == 0) {
+                S.Rs[p][r.node] = kR;
+                S.Ss[p][r.node] = kS;
+                S.Rm[p][r.node] = xr;
+                S.Sm[p][r.node] = xs;
+            }
         } else {
             // Compressed arenas (cfr.rs:350-385). Pass 1 finds the node's new
             // block maxima; pass 2 recomputes the same values
@@ -1145,10 +1440,12 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                     }
                 };
                 for (uint32_t i = lane; i < nhp; i += kWave) {
-                    out[i] = hand(i, [&](size_t, float nr, float ns) {
+                    const float acc = hand(i, [&](size_t, float nr, float ns) {
                         rmax = fmaxf(rmax, fabsf(nr));
                         smax = fmaxf(smax, ns);
                     });
+                    out[i] = acc;
+                    vm = fmaxf(vm, fabsf(acc));
                 }
                 RM = wave_max_f32(rmax);
                 SM = wave_max_f32(smax);
@@ -1168,6 +1465,21 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
                 S.Ss[p][r.node] = sok ? SM : 0.0f;
             }
         }
+    }
+    return vm;
+}
+
+template <int MODE, int STORE>
+__global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
+                                                      uint32_t count, int p, int q) {
+    const uint32_t w = wave_item();
+    if (w >= count) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const ActRec r = recs[w];
+    float vm = up_action_node<MODE, STORE>(T, S, r, p, q, lane);
+    if (S.vmax) {  // the row's max |value|, for the parent's f16 scale
+        vm = wave_max_f32(vm);
+        if (lane == 0) S.vmax[r.cfv] = vm;
     }
 }
 

@@ -91,7 +91,7 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
     _write(path, meta, rows, False)
     free_mb, _total_mb = vram_info(device)
     budget = free_mb * (1 << 20) * mem_fraction
-    st = L.STORAGE_F32 if storage == "f32" else L.STORAGE_COMPRESSED
+    st = L.STORAGE[storage]
     k = 0
     waves = 0
     while k < len(flops):

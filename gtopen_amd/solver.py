@@ -14,7 +14,7 @@ class GpuSolver(QueryMixin):
     def __init__(self, spot, storage="f32", algo="dcfr", use_graphs=True, device=0):
         self.spot = spot  # keeps the view's arrays alive
         opts = L.HipOptions(
-            L.STORAGE_F32 if storage == "f32" else L.STORAGE_COMPRESSED,
+            L.STORAGE[storage],
             L.ALGO_DCFR if algo == "dcfr" else L.ALGO_CFRPLUS,
             1 if use_graphs else 0,
             device,

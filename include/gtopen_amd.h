@@ -47,6 +47,11 @@ extern "C" {
 /* Arena storage (crates/solver/src/store.rs:69-76). */
 #define GTO_STORAGE_F32 0
 #define GTO_STORAGE_COMPRESSED 1 /* i16 regrets, u16 strategy sums, f32 scale per node */
+/* f16 regrets and strategy sums times a per-node power-of-two scale: the same
+ * 2 bytes per entry as COMPRESSED, 11-bit relative precision at every
+ * magnitude. No counterpart in the reference (an engine-side format; saves
+ * and downloads are f32 as always). */
+#define GTO_STORAGE_HALF 2
 
 /* Algorithms (crates/solver/src/cfr.rs:13-35); the engine runs DCFR and CFR+. */
 #define GTO_ALGO_DCFR 0

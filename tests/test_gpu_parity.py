@@ -233,16 +233,54 @@ def test_turn_long_trajectory_f32_against_fp64(built):
     import oracle
     from gtopen_amd import GpuSolver, Spot, workloads
 
-    _say("case: turn spot, f32 arenas, 150 iterations vs CPU f32 and FP64 oracles, check every 25")
+    _say("case: turn spot, f32 and f16 arenas, 150 iterations vs CPU f32 and FP64 oracles, check every 25")
     spot = Spot(workloads.turn_spot())
     gpu, cpu, ref = GpuSolver(spot), oracle.Oracle(spot), oracle.Oracle(spot, precision=64)
+    half = GpuSolver(spot, storage="half")
     pot = spot.starting_pot
     for k in range(6):
-        gpu.iterate(25)
-        cpu.iterate(25)
-        ref.iterate(25)
-        eg, ec, e64 = gpu.exploitability(), cpu.exploitability(), ref.exploitability()
-        _say(f"  iter {25 * (k + 1)}: gpu {eg:.6f} cpu f32 {ec:.6f} fp64 {e64:.6f}")
+        for e in (gpu, cpu, ref, half):
+            e.iterate(25)
+        eg, ec, e64, eh = gpu.exploitability(), cpu.exploitability(), ref.exploitability(), half.exploitability()
+        _say(f"  iter {25 * (k + 1)}: gpu {eg:.6f} cpu f32 {ec:.6f} fp64 {e64:.6f} gpu f16 {eh:.6f}")
         assert abs(eg - e64) <= abs(ec - e64) + POT_TOL * pot
+        # the f16 arenas (an engine format of this build) are held to the
+        # same band around the FP64 trajectory
+        assert abs(eh - e64) <= abs(ec - e64) + POT_TOL * pot
         if k < 2:
             assert abs(eg - ec) <= POT_TOL * pot
+
+
+def test_half_arena_transfer(built, tmp_path):
+    """f16 arenas: uploads round to f16 against a per-node power-of-two
+    scale (relative error <= 2^-11 for every value above 2^-14 of its node's
+    largest), downloads decode exactly, and a save/load round trip through
+    the f32 file keeps the exploitability."""
+    from gtopen_amd import GpuSolver, Spot, workloads
+    from gtopen_amd.save import load_solver
+
+    _say("case: river spot, f16 arenas: upload/download and save/load")
+    spot = Spot(workloads.river_spot())
+    g = GpuSolver(spot, storage="half")
+    g.iterate(40)
+    e0 = g.exploitability()
+    r, s = g.download_arenas()
+    g.upload_arenas(r, s)  # values already on the f16 grid: unchanged
+    r2, s2 = g.download_arenas()
+    for x, y in zip(r + s, r2 + s2):
+        assert np.array_equal(x, y)
+    assert g.exploitability() == e0
+    rng = np.random.default_rng(9)
+    x = [rng.standard_normal(spot.data_size[q]).astype(np.float32) for q in range(2)]
+    g.upload_arenas(x, None)
+    y, _ = g.download_arenas()
+    for q in range(2):
+        big = np.abs(x[q]) > 1e-3
+        rel = np.abs(y[q][big] - x[q][big]) / np.abs(x[q][big])
+        assert rel.max() <= 2.0 ** -11
+    g2 = GpuSolver(spot, storage="half")
+    g2.iterate(40)
+    path = str(tmp_path / "h.gto")
+    g2.save(path)
+    _, g3, _ = load_solver(path, storage="half")
+    assert abs(g3.exploitability() - g2.exploitability()) < 1e-9
