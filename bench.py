@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-target", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the f16 and i16/u16 arena comparison")
     ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (calibrates HBM byte counters)")
@@ -125,18 +126,34 @@ def main():
 
     sec_to_target, iters_to_target, e_final = None, None, None
     if not args.no_target:
-        g2 = GpuSolver(spot, storage=args.storage)
-        t0 = time.perf_counter()
-        while g2.iteration < 2000:
-            g2.iterate(10)
-            e_final = g2.exploitability() / pot * 100.0
-            if e_final <= args.target:
-                break
-        sec_to_target = time.perf_counter() - t0
-        iters_to_target = g2.iteration
+        sec_to_target, iters_to_target, e_final = time_to_target(spot, args.storage, args.target, 2000)
         log(f"bench: {iters_to_target} iterations to {e_final:.4f}% pot in {sec_to_target:.3f}s (checks every 10)")
-        g2.close()
     gpu.close()
+
+    # the mixed-precision forms on the same spot: f16 arenas (this build's
+    # format) and the reference's i16/u16 arenas
+    mixed = None
+    if not args.no_target and not args.no_mixed and args.storage == "f32":
+        mixed = {}
+        for st, cap in (("half", 2000), ("compressed", 800)):
+            g3 = GpuSolver(spot, storage=st)
+            g3.iterate(args.warmup)
+            g3.synchronize()
+            t0 = time.perf_counter()
+            g3.iterate(args.steps)
+            g3.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            g3.close()
+            sec, its_t, e_t = time_to_target(spot, st, args.target, cap)
+            reached = e_t <= args.target
+            mixed[st] = {"arenas": {"half": "f16 x per-node power of two", "compressed": "i16/u16 (reference)"}[st],
+                         "ms_per_step": round(ms, 4), "iters_per_s": round(1e3 / ms, 3),
+                         "sec_to_target": round(sec, 3) if reached else None,
+                         "iters_to_target": its_t if reached else None,
+                         "exploitability_pct_at_stop": round(e_t, 4)}
+            log(f"bench: {st} arenas: {ms:.3f} ms/iter; " + (
+                f"{its_t} iterations to {e_t:.4f}% pot in {sec:.3f}s" if reached
+                else f"target not reached: {e_t:.4f}% pot after {its_t} iterations"))
 
     cpu = None
     if not args.no_cpu:
@@ -159,7 +176,8 @@ def main():
         log(f"bench: cpu oracle {n} iterations in {cdt:.2f}s on {threads} threads")
 
     out = {
-        "metric": "DCFR iters/s on the flop SRP spot (f32 arenas); sec to 0.3%-pot exploitability alongside",
+        "metric": f"DCFR iters/s on the {args.config} spot ({args.storage} arenas); sec to "
+                  f"{args.target}%-pot exploitability and the mixed-precision forms alongside",
         "value": round(its, 3),
         "unit": "DCFR iters/s",
         "n_gpus": 1,
@@ -179,8 +197,28 @@ def main():
         "whole_iteration_arena_GBps": round(whole, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out), flush=True)
+
+
+def time_to_target(spot, storage, target, cap):
+    """Wall time of a fresh engine to `target` % pot, best-response check
+    every 10 iterations, at most `cap` iterations: (seconds, iterations, %pot)."""
+    from gtopen_amd import GpuSolver
+
+    g = GpuSolver(spot, storage=storage)
+    t0 = time.perf_counter()
+    e = None
+    while g.iteration < cap:
+        g.iterate(10)
+        e = g.exploitability() / spot.starting_pot * 100.0
+        if e <= target:
+            break
+    dt = time.perf_counter() - t0
+    its = g.iteration
+    g.close()
+    return dt, its, e
 
 
 def bench_batched(args):
