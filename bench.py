@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the f16 and i16/u16 arena comparison")
+    ap.add_argument("--showdown", default="exact", choices=["exact", "bf16"],
+                    help="showdown evaluation: exact f64 prefix sums or the BF16 matrix-core product")
     ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (calibrates HBM byte counters)")
@@ -83,7 +85,7 @@ def main():
         f"{spot.num_action_nodes} action nodes, hands {spot.num_hands}")
     pot = spot.starting_pot
 
-    gpu = GpuSolver(spot, storage=args.storage)
+    gpu = GpuSolver(spot, storage=args.storage, showdown=args.showdown)
     D = gpu.reached_entries
     gpu.iterate(args.warmup)
     gpu.synchronize()
@@ -126,7 +128,8 @@ def main():
 
     sec_to_target, iters_to_target, e_final = None, None, None
     if not args.no_target:
-        sec_to_target, iters_to_target, e_final = time_to_target(spot, args.storage, args.target, 2000)
+        sec_to_target, iters_to_target, e_final = time_to_target(spot, args.storage, args.target, 2000,
+                                                                 args.showdown)
         log(f"bench: {iters_to_target} iterations to {e_final:.4f}% pot in {sec_to_target:.3f}s (checks every 10)")
     gpu.close()
 
@@ -135,8 +138,9 @@ def main():
     mixed = None
     if not args.no_target and not args.no_mixed and args.storage == "f32":
         mixed = {}
-        for st, cap in (("half", 2000), ("compressed", 800)):
-            g3 = GpuSolver(spot, storage=st)
+        for key, st, sd, cap in (("half", "half", "exact", 2000), ("half_bf16_showdown", "half", "bf16", 2000),
+                                 ("compressed", "compressed", "exact", 800)):
+            g3 = GpuSolver(spot, storage=st, showdown=sd)
             g3.iterate(args.warmup)
             g3.synchronize()
             t0 = time.perf_counter()
@@ -144,14 +148,15 @@ def main():
             g3.synchronize()
             ms = (time.perf_counter() - t0) * 1e3 / args.steps
             g3.close()
-            sec, its_t, e_t = time_to_target(spot, st, args.target, cap)
+            sec, its_t, e_t = time_to_target(spot, st, args.target, cap, sd)
             reached = e_t <= args.target
-            mixed[st] = {"arenas": {"half": "f16 x per-node power of two", "compressed": "i16/u16 (reference)"}[st],
-                         "ms_per_step": round(ms, 4), "iters_per_s": round(1e3 / ms, 3),
+            mixed[key] = {"arenas": {"half": "f16 x per-node power of two", "compressed": "i16/u16 (reference)"}[st],
+                          "showdown": {"exact": "f64 prefix sums", "bf16": "BF16 matrix-core product, f32 accumulate"}[sd],
+                          "ms_per_step": round(ms, 4), "iters_per_s": round(1e3 / ms, 3),
                          "sec_to_target": round(sec, 3) if reached else None,
                          "iters_to_target": its_t if reached else None,
                          "exploitability_pct_at_stop": round(e_t, 4)}
-            log(f"bench: {st} arenas: {ms:.3f} ms/iter; " + (
+            log(f"bench: {key}: {ms:.3f} ms/iter; " + (
                 f"{its_t} iterations to {e_t:.4f}% pot in {sec:.3f}s" if reached
                 else f"target not reached: {e_t:.4f}% pot after {its_t} iterations"))
 
@@ -187,7 +192,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math"}[args.storage],
+        "dtype": {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math"}[args.storage]
+                 + ("" if args.showdown == "exact" else ", bf16 showdown products"),
         "data": "synthetic",
         "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
                    "hands": list(spot.num_hands), "arena_entries_D": D, "parallelism": "single GPU"},
@@ -202,12 +208,12 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def time_to_target(spot, storage, target, cap):
+def time_to_target(spot, storage, target, cap, showdown="exact"):
     """Wall time of a fresh engine to `target` % pot, best-response check
     every 10 iterations, at most `cap` iterations: (seconds, iterations, %pot)."""
     from gtopen_amd import GpuSolver
 
-    g = GpuSolver(spot, storage=storage)
+    g = GpuSolver(spot, storage=storage, showdown=showdown)
     t0 = time.perf_counter()
     e = None
     while g.iteration < cap:

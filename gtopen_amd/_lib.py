@@ -103,6 +103,7 @@ SIGNATURES = [
     ("gto_hip_download_compressed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int16), c_u16p, c_f32p, c_f32p]),
     ("gto_hip_update_locks", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_f32p, ctypes.c_uint64]),
     ("gto_hip_set_weights", ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p]),
+    ("gto_hip_set_showdown", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("gto_hip_solved_strategy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_f32p]),
     ("gto_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
     ("gto_batch_iterate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),

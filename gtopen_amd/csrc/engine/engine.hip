@@ -11,6 +11,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/gtopen_amd.h"
@@ -488,6 +489,65 @@ class Engine {
 
     void synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }
 
+    // Bitsets of card-compatible opponent hands for the MFMA showdown: per
+    // board and sorted hand i of p, bit k set when i and the opponent's k-th
+    // strongest-ordered hand share no card (padding bits 0).
+    void build_mma_tables() {
+        if (mma_tables_) return;
+        for (int p = 0; p < 2; ++p) {
+            const int o = 1 - p;
+            std::vector<uint32_t> off(plan_.num_boards + 1, 0);
+            for (uint32_t b = 0; b < plan_.num_boards; ++b)
+                off[b + 1] = off[b] + plan_.sd_cnt[p][b] * mma_lds_layout(plan_.sd_cnt[o][b], 0).words;
+            std::vector<uint32_t> bits(std::max<uint32_t>(off[plan_.num_boards], 1), 0u);
+            auto work = [&](uint32_t b0, uint32_t b1) {
+                std::vector<uint64_t> om;
+                for (uint32_t b = b0; b < b1; ++b) {
+                    const uint32_t mo = plan_.sd_cnt[o][b], mp = plan_.sd_cnt[p][b];
+                    const uint32_t words = mma_lds_layout(mo, 0).words;
+                    om.resize(mo);
+                    for (uint32_t k = 0; k < mo; ++k) {
+                        const uint16_t j = plan_.sd_idx[o][plan_.sd_off[o][b] + k];
+                        om[k] = (1ull << c1_[o][j]) | (1ull << c2_[o][j]);
+                    }
+                    for (uint32_t i = 0; i < mp; ++i) {
+                        const uint16_t h = plan_.sd_idx[p][plan_.sd_off[p][b] + i];
+                        const uint64_t mh = (1ull << c1_[p][h]) | (1ull << c2_[p][h]);
+                        uint32_t* row = &bits[off[b] + (size_t)i * words];
+                        for (uint32_t k = 0; k < mo; ++k)
+                            if ((mh & om[k]) == 0) row[k >> 5] |= 1u << (k & 31);
+                    }
+                }
+            };
+            const uint32_t nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            const uint32_t per = (plan_.num_boards + nt - 1) / nt;
+            for (uint32_t t = 0; t < nt; ++t) {
+                const uint32_t b0 = std::min(plan_.num_boards, t * per), b1 = std::min(plan_.num_boards, b0 + per);
+                if (b0 < b1) pool.emplace_back(work, b0, b1);
+            }
+            for (auto& th : pool) th.join();
+            T_.sd_djbits[p] = upload_vec(bits);
+            off.pop_back();
+            T_.sd_djoff[p] = upload_vec(off);
+        }
+        mma_tables_ = true;
+    }
+
+    void set_showdown(int mode) {
+        if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16) throw std::runtime_error("unknown showdown mode");
+        if (mode == GTO_SHOWDOWN_BF16) {
+            build_mma_tables();
+            if (show_mma_lds_ > 160 * 1024)
+                throw std::runtime_error("showdown matrix operands exceed the LDS budget (too many hands per board)");
+            HIP_OK(hipFuncSetAttribute((const void*)k_up_show_mma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)std::max<size_t>(show_mma_lds_, 1)));
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+        drop_graph();
+        show_mode_ = mode;
+    }
+
     void profile(uint32_t n, gto_hip_profile* out) {
         std::memset(out, 0, sizeof(*out));
         prof_ = out;
@@ -543,6 +603,11 @@ class Engine {
     ShowRec* show_all_ = nullptr;
     uint4* show_chunks_ = nullptr;
     uint32_t n_show_chunks_ = 0;
+    uint4* show_chunks_mma_ = nullptr;
+    uint32_t n_show_chunks_mma_ = 0;
+    size_t show_mma_lds_ = 0;
+    int show_mode_ = GTO_SHOWDOWN_EXACT;
+    bool mma_tables_ = false;
     size_t show_lds_[2] = {0, 0};
     size_t fold_lds_[2] = {0, 0};
     hipGraphExec_t graph_ = nullptr;
@@ -633,10 +698,16 @@ class Engine {
         n_fold_all_ = (uint32_t)plan_.fold_all.size();
         if (n_fold_all_) fold_all_ = upload_vec(plan_.fold_all);
         n_show_chunks_ = (uint32_t)(plan_.show_chunks.size() / 4);
+        n_show_chunks_mma_ = (uint32_t)(plan_.show_chunks_mma.size() / 4);
         if (n_show_chunks_) {
             show_all_ = upload_vec(plan_.show_all);
             show_chunks_ = (uint4*)upload_vec(plan_.show_chunks);
+            show_chunks_mma_ = (uint4*)upload_vec(plan_.show_chunks_mma);
         }
+        for (uint32_t b = 0; b < plan_.num_boards; ++b)
+            for (int q = 0; q < 2; ++q)
+                show_mma_lds_ = std::max<size_t>(show_mma_lds_,
+                                                 mma_lds_layout(plan_.sd_cnt[1 - q][b], plan_.sd_cnt[q][b]).bytes);
         lv_.resize(plan_.levels.size());
         for (size_t l = 0; l < plan_.levels.size(); ++l) {
             const LevelLists& L = plan_.levels[l];
@@ -802,7 +873,10 @@ class Engine {
         if (n_fold_all_)
             launch(PK_UP_FOLD, k_up_fold, wave_grid(n_fold_all_), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
                    n_fold_all_, p, swap);
-        if (n_show_chunks_)
+        if (n_show_chunks_ && show_mode_ == GTO_SHOWDOWN_BF16)
+            launch(PK_UP_SHOW, k_up_show_mma, wave_grid(n_show_chunks_mma_, 1), show_mma_lds_, T_, S_,
+                   (const ShowRec*)show_all_, (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap);
+        else if (n_show_chunks_)
             launch(PK_UP_SHOW, k_up_show, wave_grid(n_show_chunks_, 1), show_lds_[p], T_, S_,
                    (const ShowRec*)show_all_, (const uint4*)show_chunks_, n_show_chunks_, p, swap);
         for (size_t l = lv_.size(); l-- > 0;) {
@@ -1100,6 +1174,14 @@ int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint3
         return GTO_ERR_INVALID;
     }
     return guarded([&] { h->e->node_values(p, mode, count, nodes, out); });
+}
+
+int gto_hip_set_showdown(gto_hip* h, int mode) {
+    if (!h) {
+        set_last_error("gto_hip_set_showdown: null handle");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] { h->e->set_showdown(mode); });
 }
 
 int gto_hip_set_weights(gto_hip* h, const float* w0, const float* w1) {

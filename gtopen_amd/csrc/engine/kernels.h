@@ -58,6 +58,10 @@ struct DevTree {
     const uint16_t* sd_cpos_t[2];     // entry-major: [board][t * 52 + card]
     const uint4* sd_tab[2];           // [board][hand]: packed rank/segment table
     const uint16_t* sd_kslot[2];      // [sorted hand]: slots in the per-card arrays (2 per hand)
+    // MFMA showdown: per board, for each sorted hand of q, a bitset over the
+    // opponent's sorted hands (1 = no shared card), padded to 32-bit words
+    const uint32_t* sd_djbits[2];
+    const uint32_t* sd_djoff[2];      // per board: first word of the board's bitsets
 };
 
 // LDS layout of the showdown kernel for one board (byte offsets): the
@@ -89,6 +93,28 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, ui
     L.coff = L.kslot + ((4 * mo + 3) & ~3u);    // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
     L.bytes = (L.nodes + 16 * sizeof(ShowNode) + 15) & ~15u;
+    return L;
+}
+
+// LDS layout of the MFMA showdown for one board: opponent hand masks in
+// strength order, then the reach of up to 32 nodes in strength order as
+// bf16 ([node][k], k padded to 16), the nodes' table, per-node max |value|.
+struct MmaLds {
+    uint32_t B, nodes, vmx, oidx, pidx, bytes, kpad, rs, words;
+};
+__host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp) {
+    MmaLds L;
+    L.kpad = (mo + 15) & ~15u;
+    L.words = (L.kpad + 31) / 32;         // bitset words per hand
+    // row stride of B in bf16: 16-byte fragment reads of the 32 node rows
+    // then start 4 banks apart (stride = 4 mod 64 dwords)
+    L.rs = ((L.kpad + 127) & ~127u) + 8;
+    L.B = 0;                              // 32 x rs bf16
+    L.nodes = L.B + 2 * 32 * L.rs;        // 32 ShowNode
+    L.vmx = L.nodes + 32 * 24;            // 32 u32
+    L.oidx = L.vmx + 32 * 4;              // kpad u16: opponent hand of sorted position k
+    L.pidx = L.oidx + 2 * L.kpad;         // mp u16: p's hand of sorted position i
+    L.bytes = (L.pidx + 2 * mp + 15) & ~15u;
     return L;
 }
 

@@ -1044,6 +1044,188 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
 }
 
 // ---------------------------------------------------------------------------
+// Up sweep, showdown terminals as a matrix product on the matrix cores
+// (the BF16 precision form; the exact form is k_up_show). For the nodes of
+// one river board, sorted by strength on both sides:
+//   L = W R, H = G R, V = D R      (hands of p x nodes)
+// with R[k][n] the opponent reach of sorted hand k at node n (bf16),
+// D[i][k] = 1 if hands i and k share no card, W = D where k is strictly
+// weaker than i (k < lb_i), G = D where k is strictly stronger (k >= ub_i),
+// and value = win L + lose H + tie (V - L - H): the identity of game.rs's
+// showdown sums, stacked over the board's nodes. One workgroup per run of
+// up to 32 nodes (one 32-wide N tile); each wavefront takes 32-hand M tiles
+// and walks K in steps of 16 with v_mfma_f32_32x32x16_bf16, skipping the L
+// and H products where the whole tile's W or G block is zero. Operand A is
+// built in registers from the hands' card masks and rank bounds; operand B
+// is staged in LDS. Accumulation is f32.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+
+// Eight 0/1 flags (one byte) as eight bf16 values (0 or 1.0), two per word.
+__device__ __forceinline__ bf16x8_t bits_to_bf16(uint32_t x) {
+    uint32_t w[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t lo = (x >> (2 * t)) & 1u, hi = (x >> (2 * t + 1)) & 1u;
+        w[t] = ((hi << 16) | lo) * 0x3F80u;  // bf16 1.0 in each set half
+    }
+    return __builtin_bit_cast(bf16x8_t, w);
+}
+
+__global__ __launch_bounds__(kBlock) void k_up_show_mma(DevTree T, DevState S, const ShowRec* __restrict__ recs,
+                                                        const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
+                                                        int swap) {
+    extern __shared__ double lds[];
+    const uint32_t c = xcd_slice(blockIdx.x, gridDim.x);
+    if (c >= n_chunks) return;
+    const uint4 ck = chunks[c];
+    const uint32_t b = ck.x, first = ck.y, cnt = ck.z;
+    const int o = 1 - p;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nhp = T.nh[p], nho = T.nh[o];
+    const uint32_t mo = T.sd_cnt[o][b], mp = T.sd_cnt[p][b];
+    const uint32_t so = T.sd_off[o][b], sp = T.sd_off[p][b];
+    const MmaLds L = mma_lds_layout(mo, mp);
+    const uint32_t kp = L.kpad, rs = L.rs;
+    char* base = (char*)lds;
+    uint16_t* B = (uint16_t*)(base + L.B);
+    ShowNode* nodes = (ShowNode*)(base + L.nodes);
+    uint32_t* vmx = (uint32_t*)(base + L.vmx);
+    uint16_t* oidx = (uint16_t*)(base + L.oidx);
+    uint16_t* pidx = (uint16_t*)(base + L.pidx);
+
+    if (tid < 32) {
+        ShowNode nd{0u, 0u, 0u, 0.0f, 0.0f, 0.0f};
+        if (tid < cnt) {
+            const ShowRec rr = recs[first + tid];
+            const uint32_t os = pick(rr.own, o);
+            nd.cfv = rr.cfv;
+            nd.oslot = os;
+            nd.live = (S.alive[o][os] && mo != 0) ? 1u : 0u;
+            nd.win = swap ? rr.lose : rr.win;
+            nd.lose = swap ? rr.win : rr.lose;
+            nd.tie = rr.tie;
+        }
+        nodes[tid] = nd;
+        vmx[tid] = 0u;
+    }
+    for (uint32_t k = tid; k < kp; k += kBlock) oidx[k] = k < mo ? T.sd_idx[o][so + k] : (uint16_t)0;
+    for (uint32_t i = tid; i < mp; i += kBlock) pidx[i] = T.sd_idx[p][sp + i];
+    // board-conflicting hands of this thread (table entries loaded now,
+    // zeroed once the node table is visible)
+    uint32_t bad = 0;
+    {
+        const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
+#pragma unroll
+        for (int u = 0; u < kRowPer; ++u) {
+            const uint32_t h = tid + u * kBlock;
+            if (h < nhp && tab[h].x == 0xFFFFFFFFu) bad |= 1u << u;
+        }
+    }
+    __syncthreads();
+    if (bad)
+        for (uint32_t n = 0; n < cnt; ++n) {
+            float* row = S.cfv + (size_t)nodes[n].cfv * T.nh_max;
+#pragma unroll
+            for (int u = 0; u < kRowPer; ++u)
+                if ((bad >> u) & 1u) row[tid + u * kBlock] = 0.0f;
+        }
+    // R in strength order, bf16, [node][k] (row stride rs): wavefront per
+    // node, every gather of the node in flight at once
+    for (uint32_t n = wave; n < 32; n += kWaves) {
+        const ShowNode nd = nodes[n];
+        const bool use = n < cnt && nd.live;
+        const float* __restrict__ ro = S.reach[o] + (size_t)nd.oslot * nho;
+        float v[kMaxH];
+#pragma unroll
+        for (int u = 0; u < kMaxH; ++u) {
+            const uint32_t k = lane + u * kWave;
+            v[u] = 0.0f;
+            if (use && k < mo) v[u] = ro[oidx[k]];
+        }
+#pragma unroll
+        for (int u = 0; u < kMaxH; ++u) {
+            const uint32_t k = lane + u * kWave;
+            if (k < kp) B[n * rs + k] = __builtin_bit_cast(uint16_t, (__bf16)v[u]);
+        }
+    }
+    __syncthreads();
+
+    const uint32_t col = lane & 31, half = lane >> 5;
+    const ShowNode nd = nodes[col];
+    float* __restrict__ out = S.cfv + (size_t)nd.cfv * T.nh_max;
+    const uint32_t* __restrict__ bits = T.sd_djbits[p] + T.sd_djoff[p][b];
+    float vm = 0.0f;
+    for (uint32_t mt = wave; mt * 32 < mp; mt += kWaves) {
+        const uint32_t iv = min(mt * 32 + col, mp - 1);
+        const uint32_t hs = pidx[iv];
+        const uint32_t lbub = T.sd_tab[p][(size_t)b * nhp + hs].x;  // valid on this board
+        const uint32_t lb = lbub & 0xFFFFu, ub = lbub >> 16;
+        const uint32_t* __restrict__ row = bits + (size_t)iv * L.words;
+        uint32_t lbmax = lb, ubmin = ub;
+        for (int off = 32; off > 0; off >>= 1) {
+            lbmax = max(lbmax, (uint32_t)__shfl_xor((int)lbmax, off));
+            ubmin = min(ubmin, (uint32_t)__shfl_xor((int)ubmin, off));
+        }
+        lbmax = __builtin_amdgcn_readfirstlane(lbmax);
+        ubmin = __builtin_amdgcn_readfirstlane(ubmin);
+        f32x16_t aL = {}, aH = {}, aV = {};
+        // K in blocks of 256 (8 bitset words); the next block's words load
+        // while this block's 16 steps run
+        const uint32_t nblk = (kp + 255) / 256;
+        uint32_t cw[8], nw[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cw[j] = (uint32_t)j < L.words ? row[j] : 0u;
+        for (uint32_t blk = 0; blk < nblk; ++blk) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t wi = (blk + 1) * 8 + j;
+                nw[j] = wi < L.words ? row[wi] : 0u;
+            }
+#pragma unroll
+            for (int st = 0; st < 16; ++st) {
+                const uint32_t k0 = blk * 256 + st * 16;
+                if (k0 >= kp) break;
+                const uint32_t kb = k0 + 8 * half;
+                const uint32_t d8 = (cw[st >> 1] >> (((st & 1) << 4) + 8 * half)) & 0xFFu;
+                // k < lb and k >= ub within this lane's eight k
+                const uint32_t nlo = (uint32_t)min(max((int)lb - (int)kb, 0), 8);
+                const uint32_t nhi = (uint32_t)min(max((int)ub - (int)kb, 0), 8);
+                const uint32_t w8 = d8 & ((1u << nlo) - 1u);
+                const uint32_t g8 = d8 & ~((1u << nhi) - 1u);
+                const bf16x8_t bv = *(const bf16x8_t*)(B + col * rs + kb);
+                aV = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bits_to_bf16(d8), bv, aV, 0, 0, 0);
+                if (k0 < lbmax) aL = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bits_to_bf16(w8), bv, aL, 0, 0, 0);
+                if (k0 + 16 > ubmin)
+                    aH = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bits_to_bf16(g8), bv, aH, 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cw[j] = nw[j];
+        }
+        // D layout: element e of lane l is row 8*(e/4) + 4*(l/32) + e%4, column l%32
+        if (col < cnt) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t i = mt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
+                if (i < mp) {
+                    const float lo = aL[e], hi = aH[e], va = aV[e];
+                    const float val = nd.win * lo + nd.lose * hi + nd.tie * (va - lo - hi);
+                    out[pidx[i]] = val;
+                    vm = fmaxf(vm, fabsf(val));
+                }
+            }
+        }
+    }
+    if (S.vmax) {
+        if (col < cnt) atomicMax(&vmx[col], __float_as_uint(vm));
+        __syncthreads();
+        if (tid < cnt) S.vmax[nodes[tid].cfv] = __uint_as_float(vmx[tid]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Up sweep, chance nodes (cfr.rs:637-663): sum the children's values over
 // the dealt cards in card order, skipping hands that hold the card, then
 // scale by 1/(46 - street). One workgroup per node (a node reads up to 49

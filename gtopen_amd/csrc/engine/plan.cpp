@@ -371,11 +371,14 @@ Plan build_plan(const gto_spot_view& v) {
     }
     std::stable_sort(pl.show_all.begin(), pl.show_all.end(),
                      [](const ShowRec& a, const ShowRec& b) { return a.board < b.board; });
-    for (uint32_t k = 0; k < pl.show_all.size();) {
-        uint32_t e = k;
-        while (e < pl.show_all.size() && e - k < kShowChunk && pl.show_all[e].board == pl.show_all[k].board) ++e;
-        pl.show_chunks.insert(pl.show_chunks.end(), {pl.show_all[k].board, k, e - k, 0u});
-        k = e;
+    for (uint32_t cap : {kShowChunk, kMmaNodes}) {
+        std::vector<uint32_t>& out = cap == kShowChunk ? pl.show_chunks : pl.show_chunks_mma;
+        for (uint32_t k = 0; k < pl.show_all.size();) {
+            uint32_t e = k;
+            while (e < pl.show_all.size() && e - k < cap && pl.show_all[e].board == pl.show_all[k].board) ++e;
+            out.insert(out.end(), {pl.show_all[k].board, k, e - k, 0u});
+            k = e;
+        }
     }
 
     // per-card hand lists (fold terminals)

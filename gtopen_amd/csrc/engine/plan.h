@@ -28,6 +28,7 @@ struct LevelLists {
 };
 
 constexpr uint32_t kShowChunk = 16;    // showdown nodes of one board per workgroup run
+constexpr uint32_t kMmaNodes = 32;     // showdown nodes per MFMA workgroup (one 32-wide N tile)
 
 struct Plan {
     uint32_t num_nodes = 0;
@@ -102,6 +103,8 @@ struct Plan {
     // runs of <= kShowChunk showdown records sharing a board:
     // (board, first record, count, 0)
     std::vector<uint32_t> show_chunks;
+    // the same runs capped at kMmaNodes nodes (the MFMA showdown's N tile)
+    std::vector<uint32_t> show_chunks_mma;
 
     // algorithmic arena entries over reached action nodes (sum na*nh_actor)
     uint64_t reached_entries = 0;

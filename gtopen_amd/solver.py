@@ -11,7 +11,7 @@ from .query import QueryMixin
 
 
 class GpuSolver(QueryMixin):
-    def __init__(self, spot, storage="f32", algo="dcfr", use_graphs=True, device=0):
+    def __init__(self, spot, storage="f32", algo="dcfr", use_graphs=True, device=0, showdown="exact"):
         self.spot = spot  # keeps the view's arrays alive
         opts = L.HipOptions(
             L.STORAGE[storage],
@@ -24,6 +24,14 @@ class GpuSolver(QueryMixin):
         h = ctypes.c_void_p()
         L.check(L.lib().gto_hip_create(spot.view_ptr, ctypes.byref(opts), ctypes.byref(h)), "gpu create")
         self._h = h
+        if showdown != "exact":
+            self.set_showdown(showdown)
+
+    def set_showdown(self, mode):
+        """'exact' (f64 prefix sums, the CPU engine's arithmetic) or 'bf16'
+        (same-board showdowns stacked into one matrix-core product)."""
+        L.check(L.lib().gto_hip_set_showdown(self._h, {"exact": 0, "bf16": 1}[mode]), "set showdown")
+        self.showdown = mode
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

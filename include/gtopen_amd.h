@@ -263,6 +263,16 @@ int gto_hip_solved_strategy(gto_hip* h, uint32_t node, float* out);
  * outside the reachable tree are an error. */
 int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint32_t* nodes, float* out);
 
+/* Showdown evaluation. EXACT (default): strength-order prefix sums in f64,
+ * the CPU engine's arithmetic (game.rs:501-577) up to the association of
+ * f64 sums. BF16: the showdown nodes of each river board stacked into one
+ * matrix product on the matrix cores, value = win*L + lose*H +
+ * tie*(V-L-H) with [L;H;V] = [W;G;D] x R, R = opponent reach in bf16, f32
+ * accumulation (the precision form; no counterpart in the reference). */
+#define GTO_SHOWDOWN_EXACT 0
+#define GTO_SHOWDOWN_BF16 1
+int gto_hip_set_showdown(gto_hip* h, int mode);
+
 /* Range weights per hand (the root reach), replacing the spot's; NULL keeps
  * a player's weights. The hand lists stay as built: a weight of 0 leaves the
  * hand in place with zero reach (the reference's builder would drop it,
