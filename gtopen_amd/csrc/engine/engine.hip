@@ -498,13 +498,14 @@ class Engine {
             const int o = 1 - p;
             std::vector<uint32_t> off(plan_.num_boards + 1, 0);
             for (uint32_t b = 0; b < plan_.num_boards; ++b)
-                off[b + 1] = off[b] + plan_.sd_cnt[p][b] * mma_lds_layout(plan_.sd_cnt[o][b], 0).words;
-            std::vector<uint32_t> bits(std::max<uint32_t>(off[plan_.num_boards], 1), 0u);
+                off[b + 1] = off[b] + plan_.sd_cnt[p][b] * mma_lds_layout(plan_.sd_cnt[o][b], 0, 0).words;
+            // 8 words of padding: the kernel reads whole 8-word groups
+            std::vector<uint32_t> bits(off[plan_.num_boards] + 8, 0u);
             auto work = [&](uint32_t b0, uint32_t b1) {
                 std::vector<uint64_t> om;
                 for (uint32_t b = b0; b < b1; ++b) {
                     const uint32_t mo = plan_.sd_cnt[o][b], mp = plan_.sd_cnt[p][b];
-                    const uint32_t words = mma_lds_layout(mo, 0).words;
+                    const uint32_t words = mma_lds_layout(mo, 0, 0).words;
                     om.resize(mo);
                     for (uint32_t k = 0; k < mo; ++k) {
                         const uint16_t j = plan_.sd_idx[o][plan_.sd_off[o][b] + k];
@@ -516,6 +517,13 @@ class Engine {
                         uint32_t* row = &bits[off[b] + (size_t)i * words];
                         for (uint32_t k = 0; k < mo; ++k)
                             if ((mh & om[k]) == 0) row[k >> 5] |= 1u << (k & 31);
+                        // pair-split words: bit 2i -> bit i, bit 2i+1 -> bit 16+i
+                        for (uint32_t w = 0; w < words; ++w) {
+                            uint32_t z = 0;
+                            for (uint32_t i = 0; i < 16; ++i)
+                                z |= ((row[w] >> (2 * i)) & 1u) << i | ((row[w] >> (2 * i + 1)) & 1u) << (16 + i);
+                            row[w] = z;
+                        }
                     }
                 }
             };
@@ -528,6 +536,7 @@ class Engine {
             }
             for (auto& th : pool) th.join();
             T_.sd_djbits[p] = upload_vec(bits);
+            T_.sd_lbub[p] = upload_vec(plan_.sd_lbub[p]);
             off.pop_back();
             T_.sd_djoff[p] = upload_vec(off);
         }
@@ -538,7 +547,7 @@ class Engine {
         if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16) throw std::runtime_error("unknown showdown mode");
         if (mode == GTO_SHOWDOWN_BF16) {
             build_mma_tables();
-            if (show_mma_lds_ > 160 * 1024)
+            if (show_mma_lds_ > kLdsBudget)
                 throw std::runtime_error("showdown matrix operands exceed the LDS budget (too many hands per board)");
             HIP_OK(hipFuncSetAttribute((const void*)k_up_show_mma, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)std::max<size_t>(show_mma_lds_, 1)));
@@ -606,6 +615,7 @@ class Engine {
     uint4* show_chunks_mma_ = nullptr;
     uint32_t n_show_chunks_mma_ = 0;
     size_t show_mma_lds_ = 0;
+    bool show_mma_staged_ = false;
     int show_mode_ = GTO_SHOWDOWN_EXACT;
     bool mma_tables_ = false;
     size_t show_lds_[2] = {0, 0};
@@ -689,7 +699,7 @@ class Engine {
             fold_lds_[p] = kFoldWaveBytes * kWaves;
         }
         size_t lds_max = std::max(std::max(show_lds_[0], show_lds_[1]), std::max(fold_lds_[0], fold_lds_[1]));
-        if (lds_max > 160 * 1024)
+        if (lds_max > kLdsBudget)
             throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
         HIP_OK(hipFuncSetAttribute((const void*)k_up_show, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)std::max(show_lds_[0], show_lds_[1])));
@@ -704,10 +714,16 @@ class Engine {
             show_chunks_ = (uint4*)upload_vec(plan_.show_chunks);
             show_chunks_mma_ = (uint4*)upload_vec(plan_.show_chunks_mma);
         }
+        // MFMA showdown: assemble output rows in LDS when they fit
+        size_t direct = 0, staged = 0;
         for (uint32_t b = 0; b < plan_.num_boards; ++b)
-            for (int q = 0; q < 2; ++q)
-                show_mma_lds_ = std::max<size_t>(show_mma_lds_,
-                                                 mma_lds_layout(plan_.sd_cnt[1 - q][b], plan_.sd_cnt[q][b]).bytes);
+            for (int q = 0; q < 2; ++q) {
+                const uint32_t mo = plan_.sd_cnt[1 - q][b], mp = plan_.sd_cnt[q][b];
+                direct = std::max<size_t>(direct, mma_lds_layout(mo, mp, 0).bytes);
+                staged = std::max<size_t>(staged, mma_lds_layout(mo, mp, v.num_hands[q]).bytes);
+            }
+        show_mma_staged_ = staged <= kLdsBudget;
+        show_mma_lds_ = show_mma_staged_ ? staged : direct;
         lv_.resize(plan_.levels.size());
         for (size_t l = 0; l < plan_.levels.size(); ++l) {
             const LevelLists& L = plan_.levels[l];
@@ -874,8 +890,9 @@ class Engine {
             launch(PK_UP_FOLD, k_up_fold, wave_grid(n_fold_all_), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
                    n_fold_all_, p, swap);
         if (n_show_chunks_ && show_mode_ == GTO_SHOWDOWN_BF16)
-            launch(PK_UP_SHOW, k_up_show_mma, wave_grid(n_show_chunks_mma_, 1), show_mma_lds_, T_, S_,
-                   (const ShowRec*)show_all_, (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap);
+            launch_b(PK_UP_SHOW, k_up_show_mma, wave_grid(n_show_chunks_mma_, 1), kMmaBlock, show_mma_lds_, T_, S_,
+                   (const ShowRec*)show_all_, (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap,
+                   show_mma_staged_ ? 1 : 0);
         else if (n_show_chunks_)
             launch(PK_UP_SHOW, k_up_show, wave_grid(n_show_chunks_, 1), show_lds_[p], T_, S_,
                    (const ShowRec*)show_all_, (const uint4*)show_chunks_, n_show_chunks_, p, swap);

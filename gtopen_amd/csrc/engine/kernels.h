@@ -60,8 +60,10 @@ struct DevTree {
     const uint16_t* sd_kslot[2];      // [sorted hand]: slots in the per-card arrays (2 per hand)
     // MFMA showdown: per board, for each sorted hand of q, a bitset over the
     // opponent's sorted hands (1 = no shared card), padded to 32-bit words
+    // and stored pair-split (bit 2i of a word at bit i, bit 2i+1 at 16+i)
     const uint32_t* sd_djbits[2];
     const uint32_t* sd_djoff[2];      // per board: first word of the board's bitsets
+    const uint32_t* sd_lbub[2];       // [sorted hand]: lb | ub << 16 (opponent hands weaker / not stronger)
 };
 
 // LDS layout of the showdown kernel for one board (byte offsets): the
@@ -99,10 +101,14 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, ui
 // LDS layout of the MFMA showdown for one board: opponent hand masks in
 // strength order, then the reach of up to 32 nodes in strength order as
 // bf16 ([node][k], k padded to 16), the nodes' table, per-node max |value|.
+constexpr size_t kLdsBudget = 160 * 1024;  // LDS per workgroup on gfx950
+
 struct MmaLds {
-    uint32_t B, nodes, vmx, oidx, pidx, bytes, kpad, rs, words;
+    uint32_t B, nodes, vmx, oidx, pidx, lbub, O, os, bytes, kpad, rs, words;
 };
-__host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp) {
+// nout != 0: the node rows (nout values each) are assembled in LDS and
+// written out whole; nout == 0: values are stored to the rows directly.
+__host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp, uint32_t nout) {
     MmaLds L;
     L.kpad = (mo + 15) & ~15u;
     L.words = (L.kpad + 31) / 32;         // bitset words per hand
@@ -112,12 +118,16 @@ __host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp) {
     L.B = 0;                              // 32 x rs bf16
     L.nodes = L.B + 2 * 32 * L.rs;        // 32 ShowNode
     L.vmx = L.nodes + 32 * 24;            // 32 u32
-    L.oidx = L.vmx + 32 * 4;              // kpad u16: opponent hand of sorted position k
+    L.lbub = L.vmx + 32 * 4;              // mp u32: rank bounds of p's sorted position i
+    L.oidx = L.lbub + 4 * mp;             // kpad u16: opponent hand of sorted position k
     L.pidx = L.oidx + 2 * L.kpad;         // mp u16: p's hand of sorted position i
-    L.bytes = (L.pidx + 2 * mp + 15) & ~15u;
+    // 32 x os f32 output rows; os odd so the 32 node rows of one scattered
+    // write start in different banks
+    L.os = nout ? (((nout + 63) & ~63u) + 1) : 0;
+    L.O = (L.pidx + 2 * mp + 15) & ~15u;
+    L.bytes = L.O + ((32 * L.os * 4 + 15) & ~15u);
     return L;
 }
-
 // Mutable solver state.
 struct DevState {
     float* R[2];          // f32 regrets (storage F32)

@@ -1047,36 +1047,52 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
 // Up sweep, showdown terminals as a matrix product on the matrix cores
 // (the BF16 precision form; the exact form is k_up_show). For the nodes of
 // one river board, sorted by strength on both sides:
-//   L = W R, H = G R, V = D R      (hands of p x nodes)
+//   L = W R, H = G R, Q = T R      (hands of p x nodes)
 // with R[k][n] the opponent reach of sorted hand k at node n (bf16),
 // D[i][k] = 1 if hands i and k share no card, W = D where k is strictly
 // weaker than i (k < lb_i), G = D where k is strictly stronger (k >= ub_i),
-// and value = win L + lose H + tie (V - L - H): the identity of game.rs's
+// T = D - W - G (ties), and value = win L + lose H + tie Q: game.rs's
 // showdown sums, stacked over the board's nodes. One workgroup per run of
 // up to 32 nodes (one 32-wide N tile); each wavefront takes 32-hand M tiles
-// and walks K in steps of 16 with v_mfma_f32_32x32x16_bf16, skipping the L
-// and H products where the whole tile's W or G block is zero. Operand A is
-// built in registers from the hands' card masks and rank bounds; operand B
-// is staged in LDS. Accumulation is f32.
+// and walks K in steps of 16 with v_mfma_f32_32x32x16_bf16. Rows of a tile
+// are consecutive in strength, so most steps lie wholly below, above or
+// inside every row's tie band and issue one product; only steps that
+// straddle a bound issue three. Operand A comes from the card-disjointness
+// bitsets (D); operand B is staged in LDS. Accumulation is f32.
 // ---------------------------------------------------------------------------
+constexpr int kMmaBlock = 512;                                     // 8 wavefronts share one B operand
+constexpr int kMmaWaves = kMmaBlock / kWave;
+constexpr int kMmaRowPer = (1326 + kMmaBlock - 1) / kMmaBlock;     // hand rows per thread
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 
-// Eight 0/1 flags (one byte) as eight bf16 values (0 or 1.0), two per word.
-__device__ __forceinline__ bf16x8_t bits_to_bf16(uint32_t x) {
+// A-operand fragments of the 0/1 showdown matrices. Bitset words are stored
+// pair-split (bit 2i of a 32-k word at bit i, bit 2i+1 at bit 16+i), so the
+// two k of one bf16 pair sit 16 bits apart: a rotation puts them at bits 14
+// and 30 and a mask leaves bf16 2.0 (0x4000) or 0 in each half. Products
+// carry the factor 2, removed exactly in the epilogue. Lane l of a
+// 32x32x16 step holds k = 8*(l/32) .. +7 of its row, i.e. pairs
+// i0 .. i0+3 with i0 = 8*(step&1) + 4*(l/32) of the step's word.
+__device__ __forceinline__ bf16x8_t pairs_to_bf16(uint32_t z, uint32_t i0) {
     uint32_t w[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const uint32_t lo = (x >> (2 * t)) & 1u, hi = (x >> (2 * t + 1)) & 1u;
-        w[t] = ((hi << 16) | lo) * 0x3F80u;  // bf16 1.0 in each set half
-    }
+    for (int t = 0; t < 4; ++t) w[t] = __builtin_rotateright32(z, (i0 + t - 14) & 31u) & 0x40004000u;
     return __builtin_bit_cast(bf16x8_t, w);
 }
 
-__global__ __launch_bounds__(kBlock) void k_up_show_mma(DevTree T, DevState S, const ShowRec* __restrict__ recs,
-                                                        const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
-                                                        int swap) {
+// Pair-split mask of this lane's k (k = kb .. kb+7 at pairs i0 .. i0+3)
+// with k < lim.
+__device__ __forceinline__ uint32_t pairs_below(int lim, int kb, uint32_t i0) {
+    const int d = lim - kb;
+    const uint32_t ne = (uint32_t)min(max((d + 1) >> 1, 0), 4);  // even k: kb + 2t < lim
+    const uint32_t no = (uint32_t)min(max(d >> 1, 0), 4);        // odd k: kb + 2t + 1 < lim
+    return (((1u << ne) - 1u) << i0) | (((1u << no) - 1u) << (16 + i0));
+}
+
+__global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S, const ShowRec* __restrict__ recs,
+                                                           const uint4* __restrict__ chunks, uint32_t n_chunks,
+                                                           int p, int swap, int staged) {
     extern __shared__ double lds[];
     const uint32_t c = xcd_slice(blockIdx.x, gridDim.x);
     if (c >= n_chunks) return;
@@ -1087,15 +1103,30 @@ __global__ __launch_bounds__(kBlock) void k_up_show_mma(DevTree T, DevState S, c
     const uint32_t nhp = T.nh[p], nho = T.nh[o];
     const uint32_t mo = T.sd_cnt[o][b], mp = T.sd_cnt[p][b];
     const uint32_t so = T.sd_off[o][b], sp = T.sd_off[p][b];
-    const MmaLds L = mma_lds_layout(mo, mp);
+    const MmaLds L = mma_lds_layout(mo, mp, staged ? nhp : 0);
     const uint32_t kp = L.kpad, rs = L.rs;
     char* base = (char*)lds;
     uint16_t* B = (uint16_t*)(base + L.B);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
     uint32_t* vmx = (uint32_t*)(base + L.vmx);
+    uint32_t* lbub = (uint32_t*)(base + L.lbub);
     uint16_t* oidx = (uint16_t*)(base + L.oidx);
     uint16_t* pidx = (uint16_t*)(base + L.pidx);
+    float* O = (float*)(base + L.O);
+    const uint32_t col = lane & 31, half = lane >> 5;
+    const uint32_t* __restrict__ bits = T.sd_djbits[p] + T.sd_djoff[p][b];
+    const uint32_t nblk = (kp + 255) / 256;  // K blocks of 256 (8 bitset words)
 
+    // this wavefront's first M tile: its first bitset block loads while the
+    // operands are staged
+    uint32_t cw[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) cw[j] = 0u;
+    if (wave * 32 < mp) {
+        const uint32_t* __restrict__ row = bits + (size_t)min(wave * 32 + col, mp - 1) * L.words;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cw[j] = row[j];
+    }
     if (tid < 32) {
         ShowNode nd{0u, 0u, 0u, 0.0f, 0.0f, 0.0f};
         if (tid < cnt) {
@@ -1111,30 +1142,38 @@ __global__ __launch_bounds__(kBlock) void k_up_show_mma(DevTree T, DevState S, c
         nodes[tid] = nd;
         vmx[tid] = 0u;
     }
-    for (uint32_t k = tid; k < kp; k += kBlock) oidx[k] = k < mo ? T.sd_idx[o][so + k] : (uint16_t)0;
-    for (uint32_t i = tid; i < mp; i += kBlock) pidx[i] = T.sd_idx[p][sp + i];
-    // board-conflicting hands of this thread (table entries loaded now,
-    // zeroed once the node table is visible)
-    uint32_t bad = 0;
-    {
+    for (uint32_t k = tid; k < kp; k += kMmaBlock) oidx[k] = k < mo ? T.sd_idx[o][so + k] : (uint16_t)0;
+    for (uint32_t i = tid; i < mp; i += kMmaBlock) {
+        pidx[i] = T.sd_idx[p][sp + i];
+        lbub[i] = T.sd_lbub[p][sp + i];
+    }
+    if (staged) {
+        // output rows start at zero: hands that conflict with the board keep it
+        float4* o4 = (float4*)O;
+        for (uint32_t i = tid; i < (32 * L.os + 3) / 4; i += kMmaBlock) o4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        __syncthreads();
+    } else {
+        // board-conflicting hands of this thread (table entries loaded now,
+        // zeroed once the node table is visible)
+        uint32_t bad = 0;
         const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
 #pragma unroll
-        for (int u = 0; u < kRowPer; ++u) {
-            const uint32_t h = tid + u * kBlock;
+        for (int u = 0; u < kMmaRowPer; ++u) {
+            const uint32_t h = tid + u * kMmaBlock;
             if (h < nhp && tab[h].x == 0xFFFFFFFFu) bad |= 1u << u;
         }
-    }
-    __syncthreads();
-    if (bad)
-        for (uint32_t n = 0; n < cnt; ++n) {
-            float* row = S.cfv + (size_t)nodes[n].cfv * T.nh_max;
+        __syncthreads();
+        if (bad)
+            for (uint32_t n = 0; n < cnt; ++n) {
+                float* row = S.cfv + (size_t)nodes[n].cfv * T.nh_max;
 #pragma unroll
-            for (int u = 0; u < kRowPer; ++u)
-                if ((bad >> u) & 1u) row[tid + u * kBlock] = 0.0f;
-        }
+                for (int u = 0; u < kMmaRowPer; ++u)
+                    if ((bad >> u) & 1u) row[tid + u * kMmaBlock] = 0.0f;
+            }
+    }
     // R in strength order, bf16, [node][k] (row stride rs): wavefront per
     // node, every gather of the node in flight at once
-    for (uint32_t n = wave; n < 32; n += kWaves) {
+    for (uint32_t n = wave; n < 32; n += kMmaWaves) {
         const ShowNode nd = nodes[n];
         const bool use = n < cnt && nd.live;
         const float* __restrict__ ro = S.reach[o] + (size_t)nd.oslot * nho;
@@ -1153,56 +1192,74 @@ __global__ __launch_bounds__(kBlock) void k_up_show_mma(DevTree T, DevState S, c
     }
     __syncthreads();
 
-    const uint32_t col = lane & 31, half = lane >> 5;
     const ShowNode nd = nodes[col];
     float* __restrict__ out = S.cfv + (size_t)nd.cfv * T.nh_max;
-    const uint32_t* __restrict__ bits = T.sd_djbits[p] + T.sd_djoff[p][b];
     float vm = 0.0f;
-    for (uint32_t mt = wave; mt * 32 < mp; mt += kWaves) {
+    for (uint32_t mt = wave; mt * 32 < mp; mt += kMmaWaves) {
         const uint32_t iv = min(mt * 32 + col, mp - 1);
-        const uint32_t hs = pidx[iv];
-        const uint32_t lbub = T.sd_tab[p][(size_t)b * nhp + hs].x;  // valid on this board
-        const uint32_t lb = lbub & 0xFFFFu, ub = lbub >> 16;
+        const uint32_t lb = lbub[iv] & 0xFFFFu, ub = lbub[iv] >> 16;
         const uint32_t* __restrict__ row = bits + (size_t)iv * L.words;
-        uint32_t lbmax = lb, ubmin = ub;
+        // the next tile's rows: its first block loads during this tile's last
+        const uint32_t* __restrict__ nrow = bits + (size_t)min((mt + kMmaWaves) * 32 + col, mp - 1) * L.words;
+        uint32_t lbmin = lb, lbmax = lb, ubmin = ub, ubmax = ub;
         for (int off = 32; off > 0; off >>= 1) {
+            lbmin = min(lbmin, (uint32_t)__shfl_xor((int)lbmin, off));
             lbmax = max(lbmax, (uint32_t)__shfl_xor((int)lbmax, off));
             ubmin = min(ubmin, (uint32_t)__shfl_xor((int)ubmin, off));
+            ubmax = max(ubmax, (uint32_t)__shfl_xor((int)ubmax, off));
         }
+        lbmin = __builtin_amdgcn_readfirstlane(lbmin);
         lbmax = __builtin_amdgcn_readfirstlane(lbmax);
         ubmin = __builtin_amdgcn_readfirstlane(ubmin);
-        f32x16_t aL = {}, aH = {}, aV = {};
-        // K in blocks of 256 (8 bitset words); the next block's words load
-        // while this block's 16 steps run
-        const uint32_t nblk = (kp + 255) / 256;
-        uint32_t cw[8], nw[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cw[j] = (uint32_t)j < L.words ? row[j] : 0u;
+        ubmax = __builtin_amdgcn_readfirstlane(ubmax);
+        // win (k < lb), lose (k >= ub) and tie sums. A 16-k step wholly on
+        // one side of every row's bounds feeds one product; steps that
+        // straddle a bound split the row bits three ways.
+        f32x16_t aW = {}, aG = {}, aT = {};
+        uint32_t nw[8];
         for (uint32_t blk = 0; blk < nblk; ++blk) {
+            // eight words from the rest of this row or the next tile's row
+            // (the bitset array carries 8 words of padding, so a read past
+            // a row's end stays in bounds; such words feed no product)
+            const uint32_t* __restrict__ np = blk + 1 < nblk ? row + (blk + 1) * 8 : nrow;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t wi = (blk + 1) * 8 + j;
-                nw[j] = wi < L.words ? row[wi] : 0u;
-            }
+            for (int j = 0; j < 8; ++j) nw[j] = np[j];
+            // B fragments two steps ahead of their products (reads past kp
+            // stay inside the workgroup's LDS and feed no product)
+            const uint16_t* __restrict__ bp = B + col * rs + blk * 256 + 8 * half;
+            bf16x8_t bq[16];
+#pragma unroll
+            for (int st = 0; st < 2; ++st) bq[st] = *(const bf16x8_t*)(bp + st * 16);
 #pragma unroll
             for (int st = 0; st < 16; ++st) {
                 const uint32_t k0 = blk * 256 + st * 16;
-                if (k0 >= kp) break;
-                const uint32_t kb = k0 + 8 * half;
-                const uint32_t d8 = (cw[st >> 1] >> (((st & 1) << 4) + 8 * half)) & 0xFFu;
-                // k < lb and k >= ub within this lane's eight k
-                const uint32_t nlo = (uint32_t)min(max((int)lb - (int)kb, 0), 8);
-                const uint32_t nhi = (uint32_t)min(max((int)ub - (int)kb, 0), 8);
-                const uint32_t w8 = d8 & ((1u << nlo) - 1u);
-                const uint32_t g8 = d8 & ~((1u << nhi) - 1u);
-                const bf16x8_t bv = *(const bf16x8_t*)(B + col * rs + kb);
-                aV = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bits_to_bf16(d8), bv, aV, 0, 0, 0);
-                if (k0 < lbmax) aL = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bits_to_bf16(w8), bv, aL, 0, 0, 0);
-                if (k0 + 16 > ubmin)
-                    aH = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bits_to_bf16(g8), bv, aH, 0, 0, 0);
+                if (st + 2 < 16) bq[st + 2] = *(const bf16x8_t*)(bp + (st + 2) * 16);
+                if (k0 < kp) {
+                    const uint32_t z = cw[st >> 1], i0 = 8 * (st & 1) + 4 * half;
+                    const bf16x8_t bv = bq[st];
+                    const bool onlyw = k0 + 16 <= lbmin, onlyg = k0 >= ubmax;
+                    const bool onlyt = k0 >= lbmax && k0 + 16 <= ubmin;
+                    const bool split = !(onlyw || onlyg || onlyt);
+                    uint32_t zw = z, zg = z, zt = z;
+                    if (split) {
+                        const int kb = (int)(k0 + 8 * half);
+                        const uint32_t mw = pairs_below((int)lb, kb, i0);
+                        const uint32_t mg = ~pairs_below((int)ub, kb, i0);
+                        zw = z & mw;
+                        zg = z & mg;
+                        zt = z & ~(mw | mg);
+                    }
+                    if (onlyw || split) aW = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zw, i0), bv, aW, 0, 0, 0);
+                    if (onlyg || split) aG = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zg, i0), bv, aG, 0, 0, 0);
+                    if (onlyt || split) aT = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zt, i0), bv, aT, 0, 0, 0);
+                }
             }
+            // the next block's words are waited for here, not at their first use
 #pragma unroll
-            for (int j = 0; j < 8; ++j) cw[j] = nw[j];
+            for (int j = 0; j < 8; ++j) {
+                asm volatile("" : "+v"(nw[j]));
+                cw[j] = nw[j];
+            }
         }
         // D layout: element e of lane l is row 8*(e/4) + 4*(l/32) + e%4, column l%32
         if (col < cnt) {
@@ -1210,15 +1267,36 @@ __global__ __launch_bounds__(kBlock) void k_up_show_mma(DevTree T, DevState S, c
             for (int e = 0; e < 16; ++e) {
                 const uint32_t i = mt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
                 if (i < mp) {
-                    const float lo = aL[e], hi = aH[e], va = aV[e];
-                    const float val = nd.win * lo + nd.lose * hi + nd.tie * (va - lo - hi);
-                    out[pidx[i]] = val;
-                    vm = fmaxf(vm, fabsf(val));
+                    const float val = 0.5f * (nd.win * aW[e] + nd.lose * aG[e] + nd.tie * aT[e]);
+                    if (staged) {
+                        O[col * L.os + pidx[i]] = val;
+                    } else {
+                        out[pidx[i]] = val;
+                        vm = fmaxf(vm, fabsf(val));
+                    }
                 }
             }
         }
     }
-    if (S.vmax) {
+    if (staged) {
+        // whole rows, in hand order, one wavefront per node
+        __syncthreads();
+        for (uint32_t n = wave; n < cnt; n += kMmaWaves) {
+            const uint32_t slot = nodes[n].cfv;
+            float* __restrict__ orow = S.cfv + (size_t)slot * T.nh_max;
+            const float* src = O + n * L.os;
+            float m = 0.0f;
+            for (uint32_t h = lane; h < nhp; h += kWave) {
+                const float v = src[h];
+                orow[h] = v;
+                m = fmaxf(m, fabsf(v));
+            }
+            if (S.vmax) {
+                m = wave_max_f32(m);
+                if (lane == 0) S.vmax[slot] = m;
+            }
+        }
+    } else if (S.vmax) {
         if (col < cnt) atomicMax(&vmx[col], __float_as_uint(vm));
         __syncthreads();
         if (tid < cnt) S.vmax[nodes[tid].cfv] = __uint_as_float(vmx[tid]);
