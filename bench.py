@@ -51,8 +51,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the f16 and i16/u16 arena comparison")
-    ap.add_argument("--showdown", default="exact", choices=["exact", "bf16"],
-                    help="showdown evaluation: exact f64 prefix sums or the BF16 matrix-core product")
+    ap.add_argument("--showdown", default="exact", choices=["exact", "bf16", "fp8"],
+                    help="showdown evaluation: exact f64 prefix sums or the BF16 / FP8 matrix-core products")
     ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (calibrates HBM byte counters)")
@@ -139,7 +139,7 @@ def main():
     if not args.no_target and not args.no_mixed and args.storage == "f32":
         mixed = {}
         for key, st, sd, cap in (("half", "half", "exact", 2000), ("half_bf16_showdown", "half", "bf16", 2000),
-                                 ("compressed", "compressed", "exact", 800)):
+                                 ("half_fp8_showdown", "half", "fp8", 1000), ("compressed", "compressed", "exact", 800)):
             g3 = GpuSolver(spot, storage=st, showdown=sd)
             g3.iterate(args.warmup)
             g3.synchronize()
@@ -151,7 +151,8 @@ def main():
             sec, its_t, e_t = time_to_target(spot, st, args.target, cap, sd)
             reached = e_t <= args.target
             mixed[key] = {"arenas": {"half": "f16 x per-node power of two", "compressed": "i16/u16 (reference)"}[st],
-                          "showdown": {"exact": "f64 prefix sums", "bf16": "BF16 matrix-core product, f32 accumulate"}[sd],
+                          "showdown": {"exact": "f64 prefix sums", "bf16": "BF16 matrix-core products, f32 accumulate",
+                                       "fp8": "FP8 (e4m3, per-node 2^k scale) matrix-core products, f32 accumulate"}[sd],
                           "ms_per_step": round(ms, 4), "iters_per_s": round(1e3 / ms, 3),
                          "sec_to_target": round(sec, 3) if reached else None,
                          "iters_to_target": its_t if reached else None,
@@ -193,7 +194,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math"}[args.storage]
-                 + ("" if args.showdown == "exact" else ", bf16 showdown products"),
+                 + ("" if args.showdown == "exact" else f", {args.showdown} showdown products"),
         "data": "synthetic",
         "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
                    "hands": list(spot.num_hands), "arena_entries_D": D, "parallelism": "single GPU"},

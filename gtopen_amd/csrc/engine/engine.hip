@@ -265,9 +265,15 @@ class Engine {
                 throw std::runtime_error("node " + std::to_string(nodes[k]) + " is not in the reachable tree");
         const int o = 1 - p;
         launch_root(1 << o);
-        launch_down(1 << o, kAverageF32);
-        reach_valid_[o] = false;
-        launch_up(p, mode == 0 ? 1 : 2);
+        if (mode == 2) {
+            launch_down(1 << o, kCurrentF32);
+            reach_valid_[o] = true;
+            launch_up(p, 3);
+        } else {
+            launch_down(1 << o, kAverageF32);
+            reach_valid_[o] = false;
+            launch_up(p, mode == 0 ? 1 : 2);
+        }
         const size_t nh = plan_.nh[p];
         for (uint32_t k = 0; k < count; ++k)
             HIP_OK(hipMemcpyAsync(out + k * nh, S_.cfv + (size_t)plan_.cfv_slot[nodes[k]] * plan_.nh_max,
@@ -544,13 +550,27 @@ class Engine {
     }
 
     void set_showdown(int mode) {
-        if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16) throw std::runtime_error("unknown showdown mode");
-        if (mode == GTO_SHOWDOWN_BF16) {
+        if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16 && mode != GTO_SHOWDOWN_FP8)
+            throw std::runtime_error("unknown showdown mode");
+        if (mode != GTO_SHOWDOWN_EXACT) {
+            const bool fp8 = mode == GTO_SHOWDOWN_FP8;
             build_mma_tables();
-            if (show_mma_lds_ > kLdsBudget)
+            // assemble output rows in LDS when they fit
+            size_t direct = 0, staged = 0;
+            for (uint32_t b = 0; b < plan_.num_boards; ++b)
+                for (int q = 0; q < 2; ++q) {
+                    const uint32_t mo = plan_.sd_cnt[1 - q][b], mp = plan_.sd_cnt[q][b];
+                    direct = std::max<size_t>(direct, mma_lds_layout(mo, mp, 0, fp8).bytes);
+                    staged = std::max<size_t>(staged, mma_lds_layout(mo, mp, plan_.nh[q], fp8).bytes);
+                }
+            const bool use_staged = staged <= kLdsBudget;
+            const size_t need = use_staged ? staged : direct;
+            if (need > kLdsBudget)
                 throw std::runtime_error("showdown matrix operands exceed the LDS budget (too many hands per board)");
-            HIP_OK(hipFuncSetAttribute((const void*)k_up_show_mma, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)std::max<size_t>(show_mma_lds_, 1)));
+            const void* fn = fp8 ? (const void*)k_up_show_mma<true> : (const void*)k_up_show_mma<false>;
+            HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(need, 1)));
+            show_mma_staged_ = use_staged;
+            show_mma_lds_ = need;
         }
         HIP_OK(hipStreamSynchronize(stream_));
         drop_graph();
@@ -714,16 +734,6 @@ class Engine {
             show_chunks_ = (uint4*)upload_vec(plan_.show_chunks);
             show_chunks_mma_ = (uint4*)upload_vec(plan_.show_chunks_mma);
         }
-        // MFMA showdown: assemble output rows in LDS when they fit
-        size_t direct = 0, staged = 0;
-        for (uint32_t b = 0; b < plan_.num_boards; ++b)
-            for (int q = 0; q < 2; ++q) {
-                const uint32_t mo = plan_.sd_cnt[1 - q][b], mp = plan_.sd_cnt[q][b];
-                direct = std::max<size_t>(direct, mma_lds_layout(mo, mp, 0).bytes);
-                staged = std::max<size_t>(staged, mma_lds_layout(mo, mp, v.num_hands[q]).bytes);
-            }
-        show_mma_staged_ = staged <= kLdsBudget;
-        show_mma_lds_ = show_mma_staged_ ? staged : direct;
         lv_.resize(plan_.levels.size());
         for (size_t l = 0; l < plan_.levels.size(); ++l) {
             const LevelLists& L = plan_.levels[l];
@@ -883,16 +893,19 @@ class Engine {
 
     // mode 0 = DCFR update, 1 = best response, 2 = average values, 3 = current
     // values, 4 = values of the strategy arena as given (f32 only); swap:
-    // terminal payoffs of the opponent instead of the traverser
+    // terminal payoffs of the opponent instead of the traverser. A showdown
+    // precision form serves the solve's own sweeps (0, 3); best response,
+    // average-strategy values and the game value always take the exact
+    // showdown, so exploitability judges every form with the same arithmetic.
     void launch_up(int p, int mode, int swap = 0) {
         // terminals of every level first: they read only the down pass
         if (n_fold_all_)
             launch(PK_UP_FOLD, k_up_fold, wave_grid(n_fold_all_), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
                    n_fold_all_, p, swap);
-        if (n_show_chunks_ && show_mode_ == GTO_SHOWDOWN_BF16)
-            launch_b(PK_UP_SHOW, k_up_show_mma, wave_grid(n_show_chunks_mma_, 1), kMmaBlock, show_mma_lds_, T_, S_,
-                   (const ShowRec*)show_all_, (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap,
-                   show_mma_staged_ ? 1 : 0);
+        if (n_show_chunks_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3))
+            launch_b(PK_UP_SHOW, show_mode_ == GTO_SHOWDOWN_FP8 ? k_up_show_mma<true> : k_up_show_mma<false>,
+                     wave_grid(n_show_chunks_mma_, 1), kMmaBlock, show_mma_lds_, T_, S_, (const ShowRec*)show_all_,
+                     (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap, show_mma_staged_ ? 1 : 0);
         else if (n_show_chunks_)
             launch(PK_UP_SHOW, k_up_show, wave_grid(n_show_chunks_, 1), show_lds_[p], T_, S_,
                    (const ShowRec*)show_all_, (const uint4*)show_chunks_, n_show_chunks_, p, swap);
@@ -1186,7 +1199,7 @@ int gto_hip_solved_strategy(gto_hip* h, uint32_t node, float* out) {
 }
 
 int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint32_t* nodes, float* out) {
-    if (!h || p < 0 || p > 1 || mode < 0 || mode > 1 || (count && (!nodes || !out))) {
+    if (!h || p < 0 || p > 1 || mode < 0 || mode > 2 || (count && (!nodes || !out))) {
         set_last_error("gto_hip_node_values: bad argument");
         return GTO_ERR_INVALID;
     }

@@ -104,19 +104,24 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, ui
 constexpr size_t kLdsBudget = 160 * 1024;  // LDS per workgroup on gfx950
 
 struct MmaLds {
-    uint32_t B, nodes, vmx, oidx, pidx, lbub, O, os, bytes, kpad, rs, words;
+    uint32_t B, bsc, nodes, vmx, oidx, pidx, lbub, O, os, bytes, kpad, rs, words;
 };
-// nout != 0: the node rows (nout values each) are assembled in LDS and
-// written out whole; nout == 0: values are stored to the rows directly.
-__host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp, uint32_t nout) {
+// fp8 == false: B in bf16, [node][k], k padded to 16, row stride rs
+// elements. fp8 == true: B in e4m3 bytes, k padded to 32 and permuted within
+// each 32-k word (the order the A operand's byte extraction yields), row
+// stride rs bytes, with a per-node power-of-two scale. nout != 0: the node
+// rows (nout values each) are assembled in LDS and written out whole;
+// nout == 0: values are stored to the rows directly.
+__host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp, uint32_t nout, bool fp8 = false) {
     MmaLds L;
-    L.kpad = (mo + 15) & ~15u;
+    L.kpad = fp8 ? (mo + 31) & ~31u : (mo + 15) & ~15u;
     L.words = (L.kpad + 31) / 32;         // bitset words per hand
-    // row stride of B in bf16: 16-byte fragment reads of the 32 node rows
-    // then start 4 banks apart (stride = 4 mod 64 dwords)
-    L.rs = ((L.kpad + 127) & ~127u) + 8;
-    L.B = 0;                              // 32 x rs bf16
-    L.nodes = L.B + 2 * 32 * L.rs;        // 32 ShowNode
+    // 16-byte (bf16) or 8-byte (fp8) fragment reads of the 32 node rows
+    // then start 4 banks apart
+    L.rs = fp8 ? ((L.kpad + 255) & ~255u) + 16 : ((L.kpad + 127) & ~127u) + 8;
+    L.B = 0;                              // 32 x rs elements
+    L.bsc = L.B + (fp8 ? 1 : 2) * 32 * L.rs;  // 32 f32: per-node inverse scale of B
+    L.nodes = L.bsc + 32 * 4;             // 32 ShowNode
     L.vmx = L.nodes + 32 * 24;            // 32 u32
     L.lbub = L.vmx + 32 * 4;              // mp u32: rank bounds of p's sorted position i
     L.oidx = L.lbub + 4 * mp;             // kpad u16: opponent hand of sorted position k

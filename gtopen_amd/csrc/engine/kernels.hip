@@ -1090,6 +1090,35 @@ __device__ __forceinline__ uint32_t pairs_below(int lim, int kb, uint32_t i0) {
     return (((1u << ne) - 1u) << i0) | (((1u << no) - 1u) << (16 + i0));
 }
 
+// e4m3 A-operand fragment (OCP e4m3: 0x40 = 2.0) from a pair-split word.
+// A rotation puts bits i, i+8, i+16, i+24 of the word at bits 6, 14, 22,
+// 30, i.e. k = 2i, 2i+16, 2i+1, 2i+17 of the 32-k word, one per byte; the
+// second dword does the same from bit i+1. Lane group g = 2*(step&1) +
+// lane/32 takes i = 2g, so a step's two lane groups hold k = 4g..4g+3 and
+// 4g+16..4g+19: the four groups partition the word's 32 k, and B is stored
+// in this byte order (fp8_bpos).
+__device__ __forceinline__ long pairs_to_fp8(uint32_t z, uint32_t i) {
+    const uint32_t d0 = __builtin_rotateright32(z, (i - 6) & 31u) & 0x40404040u;
+    const uint32_t d1 = __builtin_rotateright32(z, (i - 5) & 31u) & 0x40404040u;
+    return (long)((uint64_t)d0 | ((uint64_t)d1 << 32));
+}
+
+// LDS byte position of k in an e4m3 B row (the order of pairs_to_fp8)
+__device__ __forceinline__ uint32_t fp8_bpos(uint32_t k) {
+    const uint32_t r = k & 31u, rr = r & 15u;
+    return (k & ~31u) + 8 * (rr >> 2) + 2 * (rr & 3u) + (r >> 4);
+}
+
+// Pair-split mask of the lane's k (kw + 2i + {0..3} and kw + 2i + 16 +
+// {0..3}, kw the word's first k) with k < lim.
+__device__ __forceinline__ uint32_t fp8_below(int lim, int kw, uint32_t i) {
+    const int d1 = lim - kw - 2 * (int)i, d2 = d1 - 16;
+    const uint32_t c1 = (uint32_t)min(max(d1, 0), 4), c2 = (uint32_t)min(max(d2, 0), 4);
+    return (((1u << ((c1 + 1) >> 1)) - 1u) << i) | (((1u << (c1 >> 1)) - 1u) << (16 + i)) |
+           (((1u << ((c2 + 1) >> 1)) - 1u) << (i + 8)) | (((1u << (c2 >> 1)) - 1u) << (24 + i));
+}
+
+template <bool FP8>
 __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                            const uint4* __restrict__ chunks, uint32_t n_chunks,
                                                            int p, int swap, int staged) {
@@ -1103,10 +1132,12 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
     const uint32_t nhp = T.nh[p], nho = T.nh[o];
     const uint32_t mo = T.sd_cnt[o][b], mp = T.sd_cnt[p][b];
     const uint32_t so = T.sd_off[o][b], sp = T.sd_off[p][b];
-    const MmaLds L = mma_lds_layout(mo, mp, staged ? nhp : 0);
+    const MmaLds L = mma_lds_layout(mo, mp, staged ? nhp : 0, FP8);
     const uint32_t kp = L.kpad, rs = L.rs;
     char* base = (char*)lds;
     uint16_t* B = (uint16_t*)(base + L.B);
+    uint8_t* B8 = (uint8_t*)(base + L.B);
+    float* bsc = (float*)(base + L.bsc);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
     uint32_t* vmx = (uint32_t*)(base + L.vmx);
     uint32_t* lbub = (uint32_t*)(base + L.lbub);
@@ -1171,8 +1202,9 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
                     if ((bad >> u) & 1u) row[tid + u * kMmaBlock] = 0.0f;
             }
     }
-    // R in strength order, bf16, [node][k] (row stride rs): wavefront per
-    // node, every gather of the node in flight at once
+    // R in strength order, [node][k] (row stride rs): wavefront per node,
+    // every gather of the node in flight at once. bf16; or e4m3 after a
+    // power-of-two scale that puts the row's maximum in [128, 256)
     for (uint32_t n = wave; n < 32; n += kMmaWaves) {
         const ShowNode nd = nodes[n];
         const bool use = n < cnt && nd.live;
@@ -1184,16 +1216,39 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
             v[u] = 0.0f;
             if (use && k < mo) v[u] = ro[oidx[k]];
         }
+        if constexpr (FP8) {
+            float m = 0.0f;
 #pragma unroll
-        for (int u = 0; u < kMaxH; ++u) {
-            const uint32_t k = lane + u * kWave;
-            if (k < kp) B[n * rs + k] = __builtin_bit_cast(uint16_t, (__bf16)v[u]);
+            for (int u = 0; u < kMaxH; ++u) m = fmaxf(m, v[u]);
+            m = wave_max_f32(m);
+            int e = 0;
+            if (m > 0.0f) {
+                frexpf(m, &e);  // m < 2^e
+                e = min(max(e, -100), 100);
+            }
+            const float sc = ldexpf(1.0f, 8 - e);
+            if (lane == 0) bsc[n] = ldexpf(1.0f, e - 8);
+#pragma unroll
+            for (int u = 0; u < kMaxH; ++u) {
+                const uint32_t k = lane + u * kWave;
+                if (k < kp)
+                    B8[n * rs + fp8_bpos(k)] = (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(v[u] * sc, 0.0f, 0, false) & 0xFF);
+            }
+        } else {
+            if (lane == 0) bsc[n] = 1.0f;
+#pragma unroll
+            for (int u = 0; u < kMaxH; ++u) {
+                const uint32_t k = lane + u * kWave;
+                if (k < kp) B[n * rs + k] = __builtin_bit_cast(uint16_t, (__bf16)v[u]);
+            }
         }
     }
     __syncthreads();
 
     const ShowNode nd = nodes[col];
     float* __restrict__ out = S.cfv + (size_t)nd.cfv * T.nh_max;
+    // the operands' factor 2 (A = 2.0) and B's scale
+    const float osc = 0.5f * bsc[col];
     float vm = 0.0f;
     for (uint32_t mt = wave; mt * 32 < mp; mt += kMmaWaves) {
         const uint32_t iv = min(mt * 32 + col, mp - 1);
@@ -1226,32 +1281,69 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
             for (int j = 0; j < 8; ++j) nw[j] = np[j];
             // B fragments two steps ahead of their products (reads past kp
             // stay inside the workgroup's LDS and feed no product)
-            const uint16_t* __restrict__ bp = B + col * rs + blk * 256 + 8 * half;
-            bf16x8_t bq[16];
+            if constexpr (FP8) {
+                const uint8_t* __restrict__ bp = B8 + col * rs + blk * 256 + 8 * half;
+                long bq[16];
 #pragma unroll
-            for (int st = 0; st < 2; ++st) bq[st] = *(const bf16x8_t*)(bp + st * 16);
+                for (int st = 0; st < 2; ++st) bq[st] = *(const long*)(bp + st * 16);
 #pragma unroll
-            for (int st = 0; st < 16; ++st) {
-                const uint32_t k0 = blk * 256 + st * 16;
-                if (st + 2 < 16) bq[st + 2] = *(const bf16x8_t*)(bp + (st + 2) * 16);
-                if (k0 < kp) {
-                    const uint32_t z = cw[st >> 1], i0 = 8 * (st & 1) + 4 * half;
-                    const bf16x8_t bv = bq[st];
-                    const bool onlyw = k0 + 16 <= lbmin, onlyg = k0 >= ubmax;
-                    const bool onlyt = k0 >= lbmax && k0 + 16 <= ubmin;
-                    const bool split = !(onlyw || onlyg || onlyt);
-                    uint32_t zw = z, zg = z, zt = z;
-                    if (split) {
-                        const int kb = (int)(k0 + 8 * half);
-                        const uint32_t mw = pairs_below((int)lb, kb, i0);
-                        const uint32_t mg = ~pairs_below((int)ub, kb, i0);
-                        zw = z & mw;
-                        zg = z & mg;
-                        zt = z & ~(mw | mg);
+                for (int st = 0; st < 16; ++st) {
+                    // step st: word st/2, lane group g, k = kw + 2g*2 + {0..3, 16..19}
+                    const uint32_t kw = blk * 256 + (st >> 1) * 32;
+                    if (st + 2 < 16) bq[st + 2] = *(const long*)(bp + (st + 2) * 16);
+                    if (kw < kp) {
+                        const uint32_t z = cw[st >> 1], i = 2 * (2 * (st & 1) + half);
+                        const long bv = bq[st];
+                        const bool onlyw = kw + 32 <= lbmin, onlyg = kw >= ubmax;
+                        const bool onlyt = kw >= lbmax && kw + 32 <= ubmin;
+                        const bool split = !(onlyw || onlyg || onlyt);
+                        uint32_t zw = z, zg = z, zt = z;
+                        if (split) {
+                            const uint32_t mw = fp8_below((int)lb, (int)kw, i);
+                            const uint32_t mg = ~fp8_below((int)ub, (int)kw, i);
+                            zw = z & mw;
+                            zg = z & mg;
+                            zt = z & ~(mw | mg);
+                        }
+                        if (onlyw || split)
+                            aW = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(pairs_to_fp8(zw, i), bv, aW, 0, 0, 0);
+                        if (onlyg || split)
+                            aG = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(pairs_to_fp8(zg, i), bv, aG, 0, 0, 0);
+                        if (onlyt || split)
+                            aT = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(pairs_to_fp8(zt, i), bv, aT, 0, 0, 0);
                     }
-                    if (onlyw || split) aW = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zw, i0), bv, aW, 0, 0, 0);
-                    if (onlyg || split) aG = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zg, i0), bv, aG, 0, 0, 0);
-                    if (onlyt || split) aT = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zt, i0), bv, aT, 0, 0, 0);
+                }
+            } else {
+                const uint16_t* __restrict__ bp = B + col * rs + blk * 256 + 8 * half;
+                bf16x8_t bq[16];
+#pragma unroll
+                for (int st = 0; st < 2; ++st) bq[st] = *(const bf16x8_t*)(bp + st * 16);
+#pragma unroll
+                for (int st = 0; st < 16; ++st) {
+                    const uint32_t k0 = blk * 256 + st * 16;
+                    if (st + 2 < 16) bq[st + 2] = *(const bf16x8_t*)(bp + (st + 2) * 16);
+                    if (k0 < kp) {
+                        const uint32_t z = cw[st >> 1], i0 = 8 * (st & 1) + 4 * half;
+                        const bf16x8_t bv = bq[st];
+                        const bool onlyw = k0 + 16 <= lbmin, onlyg = k0 >= ubmax;
+                        const bool onlyt = k0 >= lbmax && k0 + 16 <= ubmin;
+                        const bool split = !(onlyw || onlyg || onlyt);
+                        uint32_t zw = z, zg = z, zt = z;
+                        if (split) {
+                            const int kb = (int)(k0 + 8 * half);
+                            const uint32_t mw = pairs_below((int)lb, kb, i0);
+                            const uint32_t mg = ~pairs_below((int)ub, kb, i0);
+                            zw = z & mw;
+                            zg = z & mg;
+                            zt = z & ~(mw | mg);
+                        }
+                        if (onlyw || split)
+                            aW = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zw, i0), bv, aW, 0, 0, 0);
+                        if (onlyg || split)
+                            aG = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zg, i0), bv, aG, 0, 0, 0);
+                        if (onlyt || split)
+                            aT = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pairs_to_bf16(zt, i0), bv, aT, 0, 0, 0);
+                    }
                 }
             }
             // the next block's words are waited for here, not at their first use
@@ -1267,7 +1359,7 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
             for (int e = 0; e < 16; ++e) {
                 const uint32_t i = mt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
                 if (i < mp) {
-                    const float val = 0.5f * (nd.win * aW[e] + nd.lose * aG[e] + nd.tie * aT[e]);
+                    const float val = osc * (nd.win * aW[e] + nd.lose * aG[e] + nd.tie * aT[e]);
                     if (staged) {
                         O[col * L.os + pidx[i]] = val;
                     } else {

@@ -130,7 +130,8 @@ class QueryMixin:
         return self.solved_strategy(node)
 
     def node_values(self, p, mode, nodes):
-        """(len(nodes), nh_p) value rows: mode 0 best response, 1 average."""
+        """(len(nodes), nh_p) value rows: mode 0 best response, 1 average
+        strategy, 2 current strategies (with the engine's showdown form)."""
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         out = np.zeros((nodes.size, self.spot.num_hands[p]), dtype=np.float32)
         L.check(L.lib().gto_hip_node_values(self._h, p, mode, nodes.size, nodes.ctypes.data_as(L.c_u32p),

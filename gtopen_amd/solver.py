@@ -28,9 +28,11 @@ class GpuSolver(QueryMixin):
             self.set_showdown(showdown)
 
     def set_showdown(self, mode):
-        """'exact' (f64 prefix sums, the CPU engine's arithmetic) or 'bf16'
-        (same-board showdowns stacked into one matrix-core product)."""
-        L.check(L.lib().gto_hip_set_showdown(self._h, {"exact": 0, "bf16": 1}[mode]), "set showdown")
+        """'exact' (f64 prefix sums, the CPU engine's arithmetic), 'bf16' or
+        'fp8' (same-board showdowns stacked into matrix-core products with
+        bf16 or e4m3 reach, f32 accumulation). The precision forms serve the
+        DCFR sweeps; exploitability and average-strategy values stay exact."""
+        L.check(L.lib().gto_hip_set_showdown(self._h, {"exact": 0, "bf16": 1, "fp8": 2}[mode]), "set showdown")
         self.showdown = mode
 
     def close(self):

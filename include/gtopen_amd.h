@@ -259,18 +259,24 @@ int gto_hip_solved_strategy(gto_hip* h, uint32_t node, float* out);
 /* Per-node value rows of player p (nh_p floats per node) after one sweep:
  * mode 0 = best response, mode 1 = average strategy; the row of node n is
  * traverse_br / traverse_avg (best_response.rs:19-223) at n given the
- * opponent's reach there. Feeds exploit_view (query.rs:800-961). Nodes
- * outside the reachable tree are an error. */
+ * opponent's reach there. Feeds exploit_view (query.rs:800-961). Mode 2 =
+ * both players' current (regret-matched) strategies, the values a DCFR
+ * update sees, with the showdown form set by gto_hip_set_showdown (modes 0
+ * and 1 always use the exact showdown). Nodes outside the reachable tree
+ * are an error. */
 int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint32_t* nodes, float* out);
 
 /* Showdown evaluation. EXACT (default): strength-order prefix sums in f64,
  * the CPU engine's arithmetic (game.rs:501-577) up to the association of
- * f64 sums. BF16: the showdown nodes of each river board stacked into one
- * matrix product on the matrix cores, value = win*L + lose*H +
- * tie*(V-L-H) with [L;H;V] = [W;G;D] x R, R = opponent reach in bf16, f32
- * accumulation (the precision form; no counterpart in the reference). */
+ * f64 sums. BF16 / FP8: the showdown nodes of each river board stacked into
+ * matrix products on the matrix cores, value = win*L + lose*H + tie*Q with
+ * [L;H;Q] = [W;G;T] x R (0/1 matrices of strictly weaker, strictly stronger
+ * and tied card-disjoint opponent hands), R = opponent reach in bf16, or in
+ * e4m3 with a power-of-two scale per node, f32 accumulation (precision
+ * forms; no counterpart in the reference). */
 #define GTO_SHOWDOWN_EXACT 0
 #define GTO_SHOWDOWN_BF16 1
+#define GTO_SHOWDOWN_FP8 2
 int gto_hip_set_showdown(gto_hip* h, int mode);
 
 /* Range weights per hand (the root reach), replacing the spot's; NULL keeps
