@@ -78,6 +78,13 @@ class HipProfile(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * PROF_KINDS), ("launches", ctypes.c_uint32 * PROF_KINDS)]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("show_nodes", ctypes.c_uint32), ("per_level", ctypes.c_uint32)]
+
+
+SCHEDULE_CANDIDATES = 6
+
+
 # (name, restype, argtypes) for every symbol declared in include/gtopen_amd.h
 SIGNATURES = [
     ("gto_last_error", ctypes.c_char_p, []),
@@ -104,6 +111,10 @@ SIGNATURES = [
     ("gto_hip_update_locks", ctypes.c_int, [ctypes.c_void_p, c_i64p, c_f32p, ctypes.c_uint64]),
     ("gto_hip_set_weights", ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p]),
     ("gto_hip_set_showdown", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("gto_hip_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Schedule)]),
+    ("gto_hip_get_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Schedule)]),
+    ("gto_hip_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Schedule), c_f32p]),
+    ("gto_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Schedule), c_f32p]),
     ("gto_hip_solved_strategy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_f32p]),
     ("gto_batch_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
     ("gto_batch_iterate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),

@@ -20,6 +20,16 @@ class Batch:
     def iterate(self, n=1):
         L.check(L.lib().gto_batch_iterate(self._h, n), "batch iterate")
 
+    def autotune(self, iters=2):
+        """Every candidate launch schedule applied to all engines and timed
+        over `iters` batch iterations (plus one warm-up each; ordinary
+        iterations of every solve); the fastest stays applied. Returns
+        (schedule, [ms per batch iteration per candidate])."""
+        best = L.Schedule()
+        ms = (ctypes.c_float * L.SCHEDULE_CANDIDATES)()
+        L.check(L.lib().gto_batch_autotune(self._h, iters, ctypes.byref(best), ms), "batch autotune")
+        return {"show_nodes": best.show_nodes, "per_level": bool(best.per_level)}, [float(x) for x in ms]
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             L.lib().gto_batch_destroy(self._h)

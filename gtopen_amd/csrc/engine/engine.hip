@@ -44,6 +44,8 @@ struct DevLevel {
     uint32_t n_fold = 0;
     ShowRec* show = nullptr;
     uint32_t n_show = 0;
+    uint4* show_chunks[3] = {nullptr, nullptr, nullptr};  // runs per entry of kShowCaps
+    uint32_t n_show_chunks[3] = {0, 0, 0};
     ChanceRec* chance = nullptr;
     uint32_t n_chance = 0;
     EdgeRec* edge = nullptr;
@@ -577,6 +579,50 @@ class Engine {
         show_mode_ = mode;
     }
 
+    // Launch schedule of the up sweep; every schedule computes the same
+    // values (terminals depend only on the down pass, and a showdown run's
+    // length does not change any node's arithmetic).
+    void set_schedule(const gto_schedule& sc) {
+        int c = -1;
+        for (int k = 0; k < 3; ++k)
+            if (kShowCaps[k] == sc.show_nodes) c = k;
+        if (c < 0 || sc.per_level > 1) throw std::runtime_error("unknown schedule");
+        if (c == cap_ && sc.per_level == sched_.per_level) return;
+        HIP_OK(hipStreamSynchronize(stream_));
+        drop_graph();
+        cap_ = c;
+        sched_ = sc;
+    }
+    gto_schedule schedule() const { return sched_; }
+
+    // Candidate i of the autotuner: per-level terminals for i >= 3, showdown
+    // runs of kShowCaps[i % 3] nodes.
+    static gto_schedule candidate(int i) { return gto_schedule{kShowCaps[i % 3], (uint32_t)(i / 3)}; }
+
+    // GPU milliseconds per iteration of `iters` iterations after a warm-up
+    // iteration (which captures the graph of a new schedule)
+    float time_iterations(uint32_t iters) {
+        iterate(1);
+        hipEvent_t e[2];
+        HIP_OK(hipEventCreate(&e[0]));
+        if (hipEventCreate(&e[1]) != hipSuccess) {
+            (void)hipEventDestroy(e[0]);
+            throw std::runtime_error("hipEventCreate failed");
+        }
+        float ms = 0.0f;
+        hipError_t rc = hipEventRecord(e[0], stream_);
+        if (rc == hipSuccess) {
+            iterate(iters);
+            rc = hipEventRecord(e[1], stream_);
+        }
+        if (rc == hipSuccess) rc = hipEventSynchronize(e[1]);
+        if (rc == hipSuccess) rc = hipEventElapsedTime(&ms, e[0], e[1]);
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+        HIP_OK(rc);
+        return ms / (float)std::max(1u, iters);
+    }
+
     void profile(uint32_t n, gto_hip_profile* out) {
         std::memset(out, 0, sizeof(*out));
         prof_ = out;
@@ -630,8 +676,10 @@ class Engine {
     FoldRec* fold_all_ = nullptr;
     uint32_t n_fold_all_ = 0;
     ShowRec* show_all_ = nullptr;
-    uint4* show_chunks_ = nullptr;
-    uint32_t n_show_chunks_ = 0;
+    uint4* show_chunks_[3] = {nullptr, nullptr, nullptr};  // all-level runs per entry of kShowCaps
+    uint32_t n_show_chunks_[3] = {0, 0, 0};
+    gto_schedule sched_{kShowCaps[0], 0};
+    int cap_ = 0;  // index of sched_.show_nodes in kShowCaps
     uint4* show_chunks_mma_ = nullptr;
     uint32_t n_show_chunks_mma_ = 0;
     size_t show_mma_lds_ = 0;
@@ -727,11 +775,13 @@ class Engine {
                                    (int)std::max(fold_lds_[0], fold_lds_[1])));
         n_fold_all_ = (uint32_t)plan_.fold_all.size();
         if (n_fold_all_) fold_all_ = upload_vec(plan_.fold_all);
-        n_show_chunks_ = (uint32_t)(plan_.show_chunks.size() / 4);
         n_show_chunks_mma_ = (uint32_t)(plan_.show_chunks_mma.size() / 4);
-        if (n_show_chunks_) {
+        if (!plan_.show_all.empty()) {
             show_all_ = upload_vec(plan_.show_all);
-            show_chunks_ = (uint4*)upload_vec(plan_.show_chunks);
+            for (int c = 0; c < 3; ++c) {
+                n_show_chunks_[c] = (uint32_t)(plan_.show_chunks[c].size() / 4);
+                show_chunks_[c] = (uint4*)upload_vec(plan_.show_chunks[c]);
+            }
             show_chunks_mma_ = (uint4*)upload_vec(plan_.show_chunks_mma);
         }
         lv_.resize(plan_.levels.size());
@@ -745,7 +795,13 @@ class Engine {
             D.n_fold = (uint32_t)L.fold_rec.size();
             if (D.n_fold) D.fold = upload_vec(L.fold_rec);
             D.n_show = (uint32_t)L.show_rec.size();
-            if (D.n_show) D.show = upload_vec(L.show_rec);
+            if (D.n_show) {
+                D.show = upload_vec(L.show_rec);
+                for (int c = 0; c < 3; ++c) {
+                    D.n_show_chunks[c] = (uint32_t)(L.show_chunks[c].size() / 4);
+                    D.show_chunks[c] = (uint4*)upload_vec(L.show_chunks[c]);
+                }
+            }
             D.n_chance = (uint32_t)L.chance_rec.size();
             if (D.n_chance) D.chance = upload_vec(L.chance_rec);
             D.n_edge = (uint32_t)L.edge_rec.size();
@@ -898,19 +954,31 @@ class Engine {
     // average-strategy values and the game value always take the exact
     // showdown, so exploitability judges every form with the same arithmetic.
     void launch_up(int p, int mode, int swap = 0) {
-        // terminals of every level first: they read only the down pass
-        if (n_fold_all_)
-            launch(PK_UP_FOLD, k_up_fold, wave_grid(n_fold_all_), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
-                   n_fold_all_, p, swap);
-        if (n_show_chunks_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3))
+        const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
+        if (mma)
             launch_b(PK_UP_SHOW, show_mode_ == GTO_SHOWDOWN_FP8 ? k_up_show_mma<true> : k_up_show_mma<false>,
                      wave_grid(n_show_chunks_mma_, 1), kMmaBlock, show_mma_lds_, T_, S_, (const ShowRec*)show_all_,
                      (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap, show_mma_staged_ ? 1 : 0);
-        else if (n_show_chunks_)
-            launch(PK_UP_SHOW, k_up_show, wave_grid(n_show_chunks_, 1), show_lds_[p], T_, S_,
-                   (const ShowRec*)show_all_, (const uint4*)show_chunks_, n_show_chunks_, p, swap);
+        if (!sched_.per_level) {
+            // terminals of every level first: they read only the down pass
+            if (n_fold_all_)
+                launch(PK_UP_FOLD, k_up_fold, wave_grid(n_fold_all_), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
+                       n_fold_all_, p, swap);
+            if (!mma && n_show_chunks_[cap_])
+                launch(PK_UP_SHOW, k_up_show, wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
+                       (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_], p, swap);
+        }
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
+            if (sched_.per_level) {
+                // this level's terminals, just before the kernels that read them
+                if (D.n_fold)
+                    launch(PK_UP_FOLD, k_up_fold, wave_grid(D.n_fold), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
+                           D.n_fold, p, swap);
+                if (!mma && D.n_show_chunks[cap_])
+                    launch(PK_UP_SHOW, k_up_show, wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
+                           (const ShowRec*)D.show, (const uint4*)D.show_chunks[cap_], D.n_show_chunks[cap_], p, swap);
+            }
             if (D.n_chance)
                 launch(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), 0, T_, S_, (const ChanceRec*)D.chance,
                        D.n_chance, p);
@@ -1099,7 +1167,37 @@ struct gto_batch {
         }
         HIP_OK(hipStreamSynchronize(stream));
     }
+
+    // wall milliseconds per batch iteration of `iters` iterations after a
+    // warm-up iteration (which rebuilds the parent graph after a change)
+    float time_iterations(uint32_t iters) {
+        iterate(1);
+        const auto t0 = std::chrono::steady_clock::now();
+        iterate(iters);
+        const std::chrono::duration<double, std::milli> dt = std::chrono::steady_clock::now() - t0;
+        return (float)(dt.count() / std::max(1u, iters));
+    }
 };
+
+// The autotuner: every candidate schedule for `iters` timed iterations,
+// then the fastest is applied. Schedules do not change results, so the
+// iterations spent tuning are ordinary iterations of the solve.
+template <class Apply, class Time>
+static gto_schedule autotune(uint32_t iters, Apply apply, Time time, float* ms) {
+    int best = 0;
+    float tbest = 0.0f;
+    for (int i = 0; i < GTO_SCHEDULE_CANDIDATES; ++i) {
+        apply(Engine::candidate(i));
+        const float t = time(iters);
+        if (ms) ms[i] = t;
+        if (i == 0 || t < tbest) {
+            tbest = t;
+            best = i;
+        }
+    }
+    apply(Engine::candidate(best));
+    return Engine::candidate(best);
+}
 
 extern "C" {
 
@@ -1212,6 +1310,53 @@ int gto_hip_set_showdown(gto_hip* h, int mode) {
         return GTO_ERR_INVALID;
     }
     return guarded([&] { h->e->set_showdown(mode); });
+}
+
+int gto_hip_set_schedule(gto_hip* h, const gto_schedule* sc) {
+    if (!h || !sc) {
+        set_last_error("gto_hip_set_schedule: null argument");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] { h->e->set_schedule(*sc); });
+}
+
+int gto_hip_get_schedule(const gto_hip* h, gto_schedule* sc) {
+    if (!h || !sc) {
+        set_last_error("gto_hip_get_schedule: null argument");
+        return GTO_ERR_INVALID;
+    }
+    *sc = h->e->schedule();
+    return GTO_OK;
+}
+
+int gto_hip_autotune(gto_hip* h, uint32_t iters, gto_schedule* best, float* ms) {
+    if (!h || iters == 0) {
+        set_last_error("gto_hip_autotune: null handle or zero iterations");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        Engine* e = h->e.get();
+        const gto_schedule b = autotune(
+            iters, [&](const gto_schedule& sc) { e->set_schedule(sc); },
+            [&](uint32_t n) { return e->time_iterations(n); }, ms);
+        if (best) *best = b;
+    });
+}
+
+int gto_batch_autotune(gto_batch* b, uint32_t iters, gto_schedule* best, float* ms) {
+    if (!b || iters == 0) {
+        set_last_error("gto_batch_autotune: null batch or zero iterations");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        const gto_schedule r = autotune(
+            iters,
+            [&](const gto_schedule& sc) {
+                for (Engine* e : b->eng) e->set_schedule(sc);
+            },
+            [&](uint32_t n) { return b->time_iterations(n); }, ms);
+        if (best) *best = r;
+    });
 }
 
 int gto_hip_set_weights(gto_hip* h, const float* w0, const float* w1) {

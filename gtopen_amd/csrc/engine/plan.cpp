@@ -371,15 +371,20 @@ Plan build_plan(const gto_spot_view& v) {
     }
     std::stable_sort(pl.show_all.begin(), pl.show_all.end(),
                      [](const ShowRec& a, const ShowRec& b) { return a.board < b.board; });
-    for (uint32_t cap : {kShowChunk, kMmaNodes}) {
-        std::vector<uint32_t>& out = cap == kShowChunk ? pl.show_chunks : pl.show_chunks_mma;
-        for (uint32_t k = 0; k < pl.show_all.size();) {
+    // runs of at most `cap` board-sorted records sharing a board
+    auto runs = [](const std::vector<ShowRec>& recs, uint32_t cap, std::vector<uint32_t>& out) {
+        for (uint32_t k = 0; k < recs.size();) {
             uint32_t e = k;
-            while (e < pl.show_all.size() && e - k < cap && pl.show_all[e].board == pl.show_all[k].board) ++e;
-            out.insert(out.end(), {pl.show_all[k].board, k, e - k, 0u});
+            while (e < recs.size() && e - k < cap && recs[e].board == recs[k].board) ++e;
+            out.insert(out.end(), {recs[k].board, k, e - k, 0u});
             k = e;
         }
+    };
+    for (int c = 0; c < 3; ++c) {
+        runs(pl.show_all, kShowCaps[c], pl.show_chunks[c]);
+        for (LevelLists& ll : pl.levels) runs(ll.show_rec, kShowCaps[c], ll.show_chunks[c]);
     }
+    runs(pl.show_all, kMmaNodes, pl.show_chunks_mma);
 
     // per-card hand lists (fold terminals)
     for (int q = 0; q < 2; ++q) {

@@ -22,12 +22,16 @@ struct LevelLists {
     // work records, in the same order as the lists above
     std::vector<ActRec> act_rec[2];
     std::vector<FoldRec> fold_rec;
-    std::vector<ShowRec> show_rec;
+    std::vector<ShowRec> show_rec;  // sorted by board slot
     std::vector<ChanceRec> chance_rec;
     std::vector<EdgeRec> edge_rec;
+    // runs of show_rec sharing a board, one list per entry of kShowCaps:
+    // (board, first record, count, 0)
+    std::vector<uint32_t> show_chunks[3];
 };
 
-constexpr uint32_t kShowChunk = 16;    // showdown nodes of one board per workgroup run
+constexpr uint32_t kShowChunk = 16;    // most showdown nodes of one board per workgroup run
+constexpr uint32_t kShowCaps[3] = {16, 8, 4};  // run lengths a schedule may choose
 constexpr uint32_t kMmaNodes = 32;     // showdown nodes per MFMA workgroup (one 32-wide N tile)
 
 struct Plan {
@@ -100,9 +104,9 @@ struct Plan {
     // the down pass, so one launch per sweep evaluates all of them
     std::vector<FoldRec> fold_all;
     std::vector<ShowRec> show_all;  // sorted by board slot
-    // runs of <= kShowChunk showdown records sharing a board:
-    // (board, first record, count, 0)
-    std::vector<uint32_t> show_chunks;
+    // runs of show_all records sharing a board, one list per entry of
+    // kShowCaps: (board, first record, count, 0)
+    std::vector<uint32_t> show_chunks[3];
     // the same runs capped at kMmaNodes nodes (the MFMA showdown's N tile)
     std::vector<uint32_t> show_chunks_mma;
 

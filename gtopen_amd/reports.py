@@ -82,9 +82,12 @@ def _write(path, meta, rows, complete):
 
 
 def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None, storage="f32",
-               mem_fraction=0.8, device=0, max_batch=64, name="report", progress=None):
+               mem_fraction=0.8, device=0, max_batch=64, name="report", progress=None, autotune=0):
     """Solve [(SpotConfig, weight)] to ``target`` % pot; returns the report
-    dict with ``flops_per_hour``."""
+    dict with ``flops_per_hour``. autotune > 0: each wave's batch first times
+    every launch schedule over that many iterations (plus a warm-up each;
+    these count as iterations of the solves) and keeps the fastest; the
+    chosen schedules are listed under ``schedules``."""
     t0 = time.time()
     meta = {"name": name, "target_pct": target, "max_iterations": max_iterations, "storage": storage}
     rows = []
@@ -94,6 +97,8 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
     st = L.STORAGE[storage]
     k = 0
     waves = 0
+    schedules = []
+    tune_cost = L.SCHEDULE_CANDIDATES * (autotune + 1)
     while k < len(flops):
         # fill a wave up to the memory budget (at least one flop)
         wave, used = [], 0
@@ -111,9 +116,14 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
         gpus = [GpuSolver(s, storage=storage, device=device) for _, _, s in wave]
         active = list(range(len(wave)))
         batch = None
+        tuned = False
         while active:
             if batch is None:
                 batch = Batch([gpus[i] for i in active])
+                if autotune > 0 and not tuned and gpus[active[0]].iteration + tune_cost <= max_iterations:
+                    sched, ms = batch.autotune(autotune)
+                    schedules.append(dict(sched, wave=waves, ms_per_iteration=ms))
+                tuned = True
             it = gpus[active[0]].iteration
             step = min(check_every - it % check_every, max_iterations - it)
             batch.iterate(step)
@@ -140,6 +150,7 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
                 active = [i for i in active if i not in done]
         del gpus
     elapsed = time.time() - t0
-    meta.update(seconds=elapsed, waves=waves, flops_per_hour=len(rows) / elapsed * 3600.0 if elapsed > 0 else 0.0)
+    meta.update(seconds=elapsed, waves=waves, flops_per_hour=len(rows) / elapsed * 3600.0 if elapsed > 0 else 0.0,
+                schedules=schedules)
     _write(path, meta, rows, True)
     return dict(meta, complete=True, flops=rows)

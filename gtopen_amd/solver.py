@@ -27,6 +27,28 @@ class GpuSolver(QueryMixin):
         if showdown != "exact":
             self.set_showdown(showdown)
 
+    def set_schedule(self, show_nodes=16, per_level=False):
+        """Launch schedule of the up sweep (results are identical under every
+        schedule): showdown nodes per workgroup (16, 8 or 4) and whether each
+        level's terminals launch with that level."""
+        sc = L.Schedule(show_nodes, int(bool(per_level)))
+        L.check(L.lib().gto_hip_set_schedule(self._h, ctypes.byref(sc)), "set schedule")
+
+    def schedule(self):
+        sc = L.Schedule()
+        L.check(L.lib().gto_hip_get_schedule(self._h, ctypes.byref(sc)), "get schedule")
+        return {"show_nodes": sc.show_nodes, "per_level": bool(sc.per_level)}
+
+    def autotune(self, iters=2):
+        """Time every candidate schedule over `iters` iterations (plus one
+        warm-up each; these are ordinary iterations of the solve), keep the
+        fastest. Returns (schedule, [ms per iteration per candidate])."""
+        best = L.Schedule()
+        ms = np.zeros(L.SCHEDULE_CANDIDATES, dtype=np.float32)
+        L.check(L.lib().gto_hip_autotune(self._h, iters, ctypes.byref(best), ms.ctypes.data_as(L.c_f32p)),
+                "autotune")
+        return {"show_nodes": best.show_nodes, "per_level": bool(best.per_level)}, [float(x) for x in ms]
+
     def set_showdown(self, mode):
         """'exact' (f64 prefix sums, the CPU engine's arithmetic), 'bf16' or
         'fp8' (same-board showdowns stacked into matrix-core products with

@@ -279,6 +279,36 @@ int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint3
 #define GTO_SHOWDOWN_FP8 2
 int gto_hip_set_showdown(gto_hip* h, int mode);
 
+/* Launch schedule of the up sweep. Every schedule computes the same values
+ * bit for bit (terminal values depend only on the down pass; a showdown
+ * run's length changes no node's arithmetic), so a schedule is a pure
+ * performance choice. show_nodes: showdown nodes of one river board per
+ * workgroup of the exact showdown kernel (16, 8 or 4). per_level: 0 = the
+ * terminals of every level in one launch each per sweep, before the level
+ * loop (the default); 1 = each level's terminals launched with that level,
+ * just before the chance and action kernels that read their values. No
+ * counterpart in the reference (its kernels have one fixed schedule). */
+typedef struct gto_schedule {
+    uint32_t show_nodes;
+    uint32_t per_level;
+} gto_schedule;
+int gto_hip_set_schedule(gto_hip* h, const gto_schedule* s);
+int gto_hip_get_schedule(const gto_hip* h, gto_schedule* s);
+
+/* Autotuner: for each of the GTO_SCHEDULE_CANDIDATES schedules (candidate
+ * i: show_nodes = {16, 8, 4}[i % 3], per_level = i / 3), one warm-up
+ * iteration (which captures the schedule's graph) and `iters` timed DCFR
+ * iterations; the fastest schedule is then applied. The iterations are
+ * ordinary iterations of the solve: it advances by
+ * GTO_SCHEDULE_CANDIDATES * (iters + 1). ms (may be NULL): per-candidate
+ * milliseconds per iteration (GPU time for an engine, wall time for a
+ * batch); best (may be NULL): the chosen schedule. The batch form applies
+ * each candidate to every engine of the batch and times the batch's
+ * parent-graph iterations. */
+#define GTO_SCHEDULE_CANDIDATES 6
+int gto_hip_autotune(gto_hip* h, uint32_t iters, gto_schedule* best, float* ms);
+int gto_batch_autotune(gto_batch* b, uint32_t iters, gto_schedule* best, float* ms);
+
 /* Range weights per hand (the root reach), replacing the spot's; NULL keeps
  * a player's weights. The hand lists stay as built: a weight of 0 leaves the
  * hand in place with zero reach (the reference's builder would drop it,
