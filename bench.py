@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the f16 and i16/u16 arena comparison")
+    ap.add_argument("--autotune", type=int, default=2,
+                    help="iterations per candidate launch schedule for the autotuner (0: default schedule)")
     ap.add_argument("--showdown", default="exact", choices=["exact", "bf16", "fp8"],
                     help="showdown evaluation: exact f64 prefix sums or the BF16 / FP8 matrix-core products")
     ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half"])
@@ -90,6 +92,12 @@ def main():
     gpu.iterate(args.warmup)
     gpu.synchronize()
     log(f"bench: warmup {args.warmup} done")
+    sched = None
+    if args.autotune > 0:
+        chosen, cand = gpu.autotune(args.autotune)
+        sched = {"chosen": chosen, "candidates_ms_per_iter": [round(x, 3) for x in cand],
+                 "candidates": "show_nodes 16/8/4 x terminals all-levels-first / per-level"}
+        log(f"bench: autotuner chose {chosen}; candidates ms/iter {sched['candidates_ms_per_iter']}")
     t0 = time.perf_counter()
     gpu.iterate(args.steps)
     gpu.synchronize()
@@ -129,7 +137,7 @@ def main():
     sec_to_target, iters_to_target, e_final = None, None, None
     if not args.no_target:
         sec_to_target, iters_to_target, e_final = time_to_target(spot, args.storage, args.target, 2000,
-                                                                 args.showdown)
+                                                                 args.showdown, sched)
         log(f"bench: {iters_to_target} iterations to {e_final:.4f}% pot in {sec_to_target:.3f}s (checks every 10)")
     gpu.close()
 
@@ -148,7 +156,7 @@ def main():
             g3.synchronize()
             ms = (time.perf_counter() - t0) * 1e3 / args.steps
             g3.close()
-            sec, its_t, e_t = time_to_target(spot, st, args.target, cap, sd)
+            sec, its_t, e_t = time_to_target(spot, st, args.target, cap, sd, sched)
             reached = e_t <= args.target
             mixed[key] = {"arenas": {"half": "f16 x per-node power of two", "compressed": "i16/u16 (reference)"}[st],
                           "showdown": {"exact": "f64 prefix sums", "bf16": "BF16 matrix-core products, f32 accumulate",
@@ -205,16 +213,20 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "schedule": sched,
     }
     print(json.dumps(out), flush=True)
 
 
-def time_to_target(spot, storage, target, cap, showdown="exact"):
+def time_to_target(spot, storage, target, cap, showdown="exact", sched=None):
     """Wall time of a fresh engine to `target` % pot, best-response check
-    every 10 iterations, at most `cap` iterations: (seconds, iterations, %pot)."""
+    every 10 iterations, at most `cap` iterations: (seconds, iterations, %pot).
+    sched: the autotuner's choice on this spot, applied to the fresh engine."""
     from gtopen_amd import GpuSolver
 
     g = GpuSolver(spot, storage=storage, showdown=showdown)
+    if sched is not None:
+        g.set_schedule(sched["chosen"]["show_nodes"], sched["chosen"]["per_level"])
     t0 = time.perf_counter()
     e = None
     while g.iteration < cap:
@@ -246,7 +258,11 @@ def bench_batched(args):
 
     t0 = time.perf_counter()
     rep = run_report([(c, 1) for c in cfgs], target=0.0, max_iterations=args.flop_iters,
-                     check_every=args.flop_iters, progress=progress, name="bench")
+                     check_every=args.flop_iters, progress=progress, name="bench", autotune=args.autotune,
+                     storage=args.storage)
+    for sc in rep.get("schedules", []):
+        log(f"bench: wave {sc['wave']}: autotuner chose show_nodes {sc['show_nodes']}, per_level {sc['per_level']}; "
+            f"candidates ms/iter {['%.1f' % x for x in sc['ms_per_iteration']]}")
     dt = time.perf_counter() - t0
     per_hour = len(rep["flops"]) / dt * 3600.0
     log(f"bench: {len(rep['flops'])} flops x {args.flop_iters} iterations in {dt:.1f}s ({rep['waves']} waves)")
@@ -269,14 +285,17 @@ def bench_batched(args):
                          f"scaled to {args.flop_iters} iterations per flop"}
         log(f"bench: cpu oracle {n} iterations of flop 1 in {cdt:.2f}s on {threads} threads")
     out = {
-        "metric": f"flops/hour, {len(cfgs)} batched flops x {args.flop_iters} DCFR iterations (f32 arenas)",
+        "metric": f"flops/hour, {len(cfgs)} batched flops x {args.flop_iters} DCFR iterations ({args.storage} arenas)",
         "value": round(per_hour, 2), "unit": "flops/hour", "n_gpus": 1, "steps": args.flop_iters, "warmup": 0,
         "ms_per_step": round(dt * 1e3 / max(1, args.flop_iters), 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math"}[args.storage],
+        "data": "synthetic",
         "config": {"workload": f"batched_flops: {len(cfgs)} seeded canonical flops, per-flop menus from "
                                "{25,33,50,75,125,a}, 1176-combo Beta(0.7,0.7) ranges",
                    "nodes_total": int(sum(r["nodes"] for r in rep["flops"])), "waves": rep["waves"],
-                   "parallelism": "single GPU, engines of a wave in one parent hipGraph"},
+                   "parallelism": "single GPU, engines of a wave in one parent hipGraph",
+                   "schedules": [{k: sc[k] for k in ("wave", "show_nodes", "per_level")} for sc in rep.get("schedules", [])]},
         "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
