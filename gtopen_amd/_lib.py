@@ -23,7 +23,18 @@ KIND_ACTION, KIND_CHANCE, KIND_FOLD, KIND_SHOWDOWN = 0, 1, 2, 3
 SENTINEL = 0xFFFFFFFF
 STORAGE_F32, STORAGE_COMPRESSED, STORAGE_HALF = 0, 1, 2
 STORAGE = {"f32": STORAGE_F32, "compressed": STORAGE_COMPRESSED, "half": STORAGE_HALF}
-ALGO_DCFR, ALGO_CFRPLUS = 0, 1
+ALGO_DCFR, ALGO_CFRPLUS, ALGO_PCFRPLUS = 0, 1, 2
+
+
+def algo_code(name):
+    """Algorithm::parse (cfr.rs:26-34): case-insensitive, '-', '_' and '+'
+    ignored; dcfr, cfr / cfrplus, pcfr / pcfrplus."""
+    key = name.lower().replace("-", "").replace("_", "").replace("+", "")
+    codes = {"dcfr": ALGO_DCFR, "cfr": ALGO_CFRPLUS, "cfrplus": ALGO_CFRPLUS, "pcfr": ALGO_PCFRPLUS,
+             "pcfrplus": ALGO_PCFRPLUS}
+    if key not in codes:
+        raise ValueError(f"unknown algorithm {name!r} (use dcfr, cfr+ or pcfr+)")
+    return codes[key]
 STRICT, LENIENT_LOAD = 0, 1
 PROF_KINDS = 8
 

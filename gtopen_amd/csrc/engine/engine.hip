@@ -85,6 +85,8 @@ class Engine {
             throw std::runtime_error("unknown storage mode");
         compressed_ = o.storage != GTO_STORAGE_F32;
         half_ = o.storage == GTO_STORAGE_HALF;
+        pred_ = o.algo == GTO_ALGO_PCFRPLUS;
+        if (pred_ && compressed_) throw std::runtime_error("PCFR+ runs with f32 arenas on the GPU");
         HIP_OK(hipSetDevice(o.device));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         plan_ = build_plan(v);
@@ -384,6 +386,14 @@ class Engine {
         if (!compressed_) {
             HIP_OK(hipMemcpyAsync(which == 0 ? S_.R[p] : S_.S[p], src, sizeof(float) * data_size_[p],
                                   hipMemcpyHostToDevice, stream_));
+            if (pred_ && which == 0) {
+                // regrets from outside carry no prediction (the save format
+                // has none; the CPU engine starts its predictions at 0 too)
+                std::vector<float> qv(data_size_[p]);
+                for (size_t k = 0; k < qv.size(); ++k) qv[k] = std::max(src[k], 0.0f);
+                HIP_OK(hipMemcpyAsync(S_.Q[p], qv.data(), sizeof(float) * qv.size(), hipMemcpyHostToDevice, stream_));
+                HIP_OK(hipStreamSynchronize(stream_));  // qv is freed on return
+            }
             HIP_OK(hipStreamSynchronize(stream_));
             return;
         }
@@ -651,6 +661,7 @@ class Engine {
    private:
     gto_hip_options opts_;
     bool compressed_ = false;  // 16-bit arenas (i16/u16 quants or f16)
+    bool pred_ = false;        // PCFR+: current strategy matched on S_.Q
     bool half_ = false;        // f16 arenas with per-node power-of-two scales
     std::vector<uint8_t> node_kind_, node_player_, node_na_;
     std::vector<uint64_t> node_off_;
@@ -832,6 +843,13 @@ class Engine {
                 S_.S[q] = (float*)dalloc(sizeof(float) * ds);
                 HIP_OK(hipMemset(S_.R[q], 0, sizeof(float) * ds));
                 HIP_OK(hipMemset(S_.S[q], 0, sizeof(float) * ds));
+                S_.cur[q] = S_.R[q];
+                if (pred_) {
+                    // Q = max(R,0) + prediction; R = 0 and no prediction yet
+                    S_.Q[q] = (float*)dalloc(sizeof(float) * ds);
+                    HIP_OK(hipMemset(S_.Q[q], 0, sizeof(float) * ds));
+                    S_.cur[q] = S_.Q[q];
+                }
             }
             S_.reach[q] = (float*)dalloc(sizeof(float) * (size_t)plan_.num_slots[q] * plan_.nh[q]);
             S_.alive[q] = (uint8_t*)dalloc(plan_.num_slots[q]);
@@ -1002,7 +1020,10 @@ class Engine {
                     }
                 } else {
                     switch (mode) {
-                        case 0: launch(PK_UP_ACTION, k_up_action<0, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 0:
+                            if (pred_) launch(PK_UP_ACTION, k_up_action<0, kStoreF32P>, g, 0, T_, S_, recs, n, p, q);
+                            else launch(PK_UP_ACTION, k_up_action<0, kStoreF32>, g, 0, T_, S_, recs, n, p, q);
+                            break;
                         case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
                         case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
                         case 4: launch(PK_UP_ACTION, k_up_action<4, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
@@ -1209,8 +1230,8 @@ int gto_hip_create(const gto_spot_view* spot, const gto_hip_options* opts, gto_h
     *out = nullptr;
     gto_hip_options o = {GTO_STORAGE_F32, GTO_ALGO_DCFR, 1, 0};
     if (opts) o = *opts;
-    if (o.algo != GTO_ALGO_DCFR && o.algo != GTO_ALGO_CFRPLUS) {
-        set_last_error("GPU engine supports dcfr and cfr+ only");
+    if (o.algo != GTO_ALGO_DCFR && o.algo != GTO_ALGO_CFRPLUS && o.algo != GTO_ALGO_PCFRPLUS) {
+        set_last_error("unknown algorithm");
         return GTO_ERR_INVALID;
     }
     return guarded([&] {

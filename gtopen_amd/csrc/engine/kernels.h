@@ -16,7 +16,8 @@ constexpr int kMaxActions = 64;   // engine limit on actions per node (one ballo
 
 // Arena storage inside the kernels: f32; the reference's i16/u16 quants
 // against a per-node block max; f16 values times a per-node power of two.
-enum Store { kStoreF32 = 0, kStoreQ = 1, kStoreH = 2 };
+// kStoreF32P: f32 arenas under PCFR+ (the update also writes Q, below)
+enum Store { kStoreF32 = 0, kStoreQ = 1, kStoreH = 2, kStoreF32P = 3 };
 
 // Power-of-two scale for f16 storage of a node whose values are bounded by
 // b: b / scale <= 2^15, well inside f16's range (65504), so the bound may
@@ -136,6 +137,11 @@ __host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp, uint3
 // Mutable solver state.
 struct DevState {
     float* R[2];          // f32 regrets (storage F32)
+    // f32 arena that current-strategy regret matching reads: R, or under
+    // PCFR+ Q = max(R,0) + last instantaneous regret (the value cfr.rs:
+    // 259-266 matches on, stored by the update that produced both)
+    float* cur[2];
+    float* Q[2];          // PCFR+ only (null otherwise)
     float* S[2];          // f32 strategy sums
     int16_t* Rq[2];       // compressed regrets
     uint16_t* Sq[2];      // compressed strategy sums

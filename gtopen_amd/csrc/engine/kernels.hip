@@ -441,7 +441,7 @@ template <int KIND>
 struct ArenaOf;
 template <>
 struct ArenaOf<kCurrentF32> {
-    static __device__ const float* get(const DevState& S, int q) { return S.R[q]; }
+    static __device__ const float* get(const DevState& S, int q) { return S.cur[q]; }
 };
 template <>
 struct ArenaOf<kAverageF32> {
@@ -1570,9 +1570,14 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
         const uint32_t t = *S.iter;
         const float pos = S.disc[3 * t + 0], neg = S.disc[3 * t + 1], ds = S.disc[3 * t + 2];
         const float* __restrict__ rp = S.reach[p] + (size_t)pick(r.own, p) * nhp;
-        if constexpr (STORE == kStoreF32) {
+        if constexpr (STORE == kStoreF32 || STORE == kStoreF32P) {
+            // PCFR+ (kStoreF32P): sigma from Q = max(R,0) + prediction, and
+            // the update also stores Q' = max(R',0) + (v - out), the next
+            // iteration's matching input (cfr.rs:259-266, 337-348)
+            constexpr bool PRED = STORE == kStoreF32P;
             float* __restrict__ R = S.R[p] + off;
             float* __restrict__ Sm = S.S[p] + off;
+            float* __restrict__ Qp = PRED ? S.Q[p] + off : nullptr;
             dispatch_na(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
@@ -1583,7 +1588,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     // the first store
                     for (uint32_t i0 = lane; i0 < nhp; i0 += 2 * kWave) {
                         HandSigma<NA, kCurrentF32> hs[2];
-                        float v[2][NA], sm[2][NA], reach[2];
+                        float v[2][NA], sm[2][NA], reach[2], rr[2][PRED ? NA : 1];
                         bool ok[2];
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
@@ -1592,7 +1597,12 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             const uint32_t ii = ok[h] ? i : i0;
 #pragma unroll
                             for (int a = 0; a < NA; ++a) {
-                                hs[h].raw[a] = R[(size_t)a * nhp + ii];
+                                if constexpr (PRED) {
+                                    hs[h].raw[a] = Qp[(size_t)a * nhp + ii];
+                                    rr[h][a] = R[(size_t)a * nhp + ii];
+                                } else {
+                                    hs[h].raw[a] = R[(size_t)a * nhp + ii];
+                                }
                                 v[h][a] = rw[a][ii];
                                 sm[h][a] = Sm[(size_t)a * nhp + ii];
                             }
@@ -1608,10 +1618,12 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             for (int a = 0; a < NA; ++a) acc += hs[h].sig[a] * v[h][a];
 #pragma unroll
                             for (int a = 0; a < NA; ++a) {
-                                const float rv = hs[h].raw[a];
+                                const float rv = PRED ? rr[h][PRED ? a : 0] : hs[h].raw[a];
                                 const float d = rv > 0.0f ? pos : neg;
-                                R[(size_t)a * nhp + i] = rv * d + (v[h][a] - acc);
+                                const float nr = rv * d + (v[h][a] - acc);
+                                R[(size_t)a * nhp + i] = nr;
                                 Sm[(size_t)a * nhp + i] = sm[h][a] * ds + reach[h] * hs[h].sig[a];
+                                if constexpr (PRED) Qp[(size_t)a * nhp + i] = fmaxf(nr, 0.0f) + (v[h][a] - acc);
                             }
                             out[i] = acc;
                     vm = fmaxf(vm, fabsf(acc));
@@ -1620,7 +1632,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                 } else {
                     for (uint32_t i = lane; i < nhp; i += kWave) {
                         SigmaHand<kCurrentF32> sh;
-                        sh.init(R, nhp, i, na);
+                        sh.init(PRED ? Qp : R, nhp, i, na);
                         float acc = 0.0f;
                         for (int a = 0; a < na; ++a) acc += sh.template sigma<false>(a) * row(a)[i];
                         const float reach = rp[i];
@@ -1629,8 +1641,10 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             const float rv = R[idx];
                             const float sg = sh.template sigma<false>(a);
                             const float d = rv > 0.0f ? pos : neg;
-                            R[idx] = rv * d + (row(a)[i] - acc);
+                            const float nr = rv * d + (row(a)[i] - acc);
+                            R[idx] = nr;
                             Sm[idx] = Sm[idx] * ds + reach * sg;
+                            if constexpr (PRED) Qp[idx] = fmaxf(nr, 0.0f) + (row(a)[i] - acc);
                         }
                         out[i] = acc;
                     vm = fmaxf(vm, fabsf(acc));

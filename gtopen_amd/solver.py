@@ -15,7 +15,7 @@ class GpuSolver(QueryMixin):
         self.spot = spot  # keeps the view's arrays alive
         opts = L.HipOptions(
             L.STORAGE[storage],
-            L.ALGO_DCFR if algo == "dcfr" else L.ALGO_CFRPLUS,
+            L.algo_code(algo),
             1 if use_graphs else 0,
             device,
         )

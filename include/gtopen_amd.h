@@ -53,9 +53,13 @@ extern "C" {
  * and downloads are f32 as always). */
 #define GTO_STORAGE_HALF 2
 
-/* Algorithms (crates/solver/src/cfr.rs:13-35); the engine runs DCFR and CFR+. */
+/* Algorithms (crates/solver/src/cfr.rs:13-35). PCFR+ (predictive CFR+:
+ * CFR+ discounts, the current strategy matched on max(R,0) plus the last
+ * instantaneous regret) runs on the GPU with f32 arenas; the reference runs
+ * it on the CPU only (gpu/mod.rs:147-149). */
 #define GTO_ALGO_DCFR 0
 #define GTO_ALGO_CFRPLUS 1
+#define GTO_ALGO_PCFRPLUS 2
 
 /* Tree build strictness (crates/solver/src/tree.rs:213-243). */
 #define GTO_STRICT 0
@@ -192,7 +196,7 @@ typedef struct gto_hip gto_hip;
 
 typedef struct gto_hip_options {
     int storage;        /* GTO_STORAGE_F32 or GTO_STORAGE_COMPRESSED (device arenas) */
-    int algo;           /* GTO_ALGO_DCFR or GTO_ALGO_CFRPLUS */
+    int algo;           /* GTO_ALGO_DCFR, GTO_ALGO_CFRPLUS or GTO_ALGO_PCFRPLUS (f32 arenas) */
     int use_graphs;     /* capture each sweep into a hipGraph after the first iteration */
     int device;         /* HIP device ordinal */
 } gto_hip_options;

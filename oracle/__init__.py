@@ -91,8 +91,11 @@ class Oracle:
             for q in range(2):
                 self._view.weights[q] = self._w[q].ctypes.data_as(G.c_f32p)
             view_ptr = ctypes.pointer(self._view)
-        self._h = lib().oracle_create(view_ptr, precision, 0 if storage == "f32" else 1,
-                                      0 if algo == "dcfr" else 1)
+        key = algo.lower().replace("-", "").replace("_", "").replace("+", "")
+        codes = {"dcfr": 0, "cfr": 1, "cfrplus": 1, "pcfr": 2, "pcfrplus": 2}  # Algorithm::parse, cfr.rs:26-34
+        if key not in codes:
+            raise ValueError(f"unknown algorithm {algo!r} (use dcfr, cfr+ or pcfr+)")
+        self._h = lib().oracle_create(view_ptr, precision, 0 if storage == "f32" else 1, codes[key])
 
     def set_weights(self, w0, w1):
         if not hasattr(self, "_w"):

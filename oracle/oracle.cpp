@@ -359,7 +359,13 @@ struct Engine : EngineBase {
     std::vector<int16_t> Rq[2];
     std::vector<uint16_t> Sq[2];
     std::vector<float> Rs[2], Ss[2];
+    // PCFR+ predictions, the last instantaneous regrets (cfr.rs:76-78,
+    // 157-168), in the regrets' format
+    std::vector<T> P[2];
+    std::vector<int16_t> Pq[2];
+    std::vector<float> Ps[2];
     bool compressed() const { return storage == GTO_STORAGE_COMPRESSED; }
+    bool predictive() const { return algo == GTO_ALGO_PCFRPLUS; }
 
     Engine(const gto_spot_view* view, int st, int al) : v(view), storage(st), algo(al) {
         for (int p = 0; p < 2; ++p) {
@@ -375,6 +381,14 @@ struct Engine : EngineBase {
             } else {
                 R[p].assign(v->data_size[p], T(0));
                 S[p].assign(v->data_size[p], T(0));
+            }
+            if (predictive()) {
+                if (compressed()) {
+                    Pq[p].assign(v->data_size[p], 0);
+                    Ps[p].assign(v->num_nodes, 0.0f);
+                } else {
+                    P[p].assign(v->data_size[p], T(0));
+                }
             }
         }
         lock_off.assign(v->num_nodes, -1);
@@ -392,6 +406,33 @@ struct Engine : EngineBase {
             return;
         }
         uint64_t off = v->node_data_offset[n];
+        if (predictive()) {
+            // PCFR+: sigma = max(R,0) + prediction, then regret_match_inplace
+            // (cfr.rs:259-266, 276-288, 923-942)
+            if (!compressed()) {
+                const T* r = R[pl].data() + off;
+                const T* pr = P[pl].data() + off;
+                for (size_t k = 0; k < na * h; ++k) sigma[k] = std::max(r[k], T(0)) + pr[k];
+            } else {
+                const int16_t* qr = Rq[pl].data() + off;
+                const int16_t* qp = Pq[pl].data() + off;
+                const float kr = Rs[pl][n] / 32767.0f, kp = Ps[pl][n] / 32767.0f;
+                for (size_t k = 0; k < na * h; ++k)
+                    sigma[k] = (T)((float)std::max<int16_t>(qr[k], 0) * kr + (float)qp[k] * kp);
+            }
+            for (size_t i = 0; i < h; ++i) {
+                T sum = 0;
+                for (size_t a = 0; a < na; ++a) sum += std::max(sigma[a * h + i], T(0));
+                if (sum > (T)1e-12f) {
+                    T inv = T(1) / sum;
+                    for (size_t a = 0; a < na; ++a) sigma[a * h + i] = std::max(sigma[a * h + i], T(0)) * inv;
+                } else {
+                    T u = T(1) / (T)na;
+                    for (size_t a = 0; a < na; ++a) sigma[a * h + i] = u;
+                }
+            }
+            return;
+        }
         if (!compressed()) {
             const T* r = R[pl].data() + off;
             for (size_t i = 0; i < h; ++i) {
@@ -421,7 +462,8 @@ struct Engine : EngineBase {
         }
     }
 
-    // update_node, cfr.rs:297-409 (DCFR/CFR+ branches)
+    // update_node, cfr.rs:297-409 (DCFR/CFR+ branches; PCFR+ also stores
+    // the instantaneous regret as the next prediction, 337-348 and 386-406)
     void update_node(uint32_t n, int p, const T* sigma, const T* vals, const T* out, const T* reach_p,
                      const Discounts<T>& d) {
         size_t na = v->node_num_children[n], h = nh[p];
@@ -437,6 +479,11 @@ struct Engine : EngineBase {
                     r[b + i] = rv * dd + (vals[b + i] - out[i]);
                     s[b + i] = s[b + i] * d.strat + reach_p[i] * sigma[b + i];
                 }
+            }
+            if (predictive()) {
+                T* pr = P[p].data() + off;
+                for (size_t a = 0; a < na; ++a)
+                    for (size_t i = 0; i < h; ++i) pr[a * h + i] = vals[a * h + i] - out[i];
             }
             return;
         }
@@ -464,6 +511,16 @@ struct Engine : EngineBase {
         }
         Rs[p][n] = quantize_i16(rbuf.data(), na * h, rmax, rq);
         Ss[p][n] = quantize_u16(sbuf.data(), na * h, smax, sq);
+        if (predictive()) {
+            float pmax = 0.0f;
+            for (size_t a = 0; a < na; ++a)
+                for (size_t i = 0; i < h; ++i) {
+                    const float delta = (float)vals[a * h + i] - (float)out[i];
+                    rbuf[a * h + i] = delta;
+                    pmax = std::fmax(pmax, std::fabs(delta));
+                }
+            Ps[p][n] = quantize_i16(rbuf.data(), na * h, pmax, Pq[p].data() + off);
+        }
     }
 
     static bool all_nonpositive(const T* r, size_t n) {

@@ -204,6 +204,48 @@ def test_cfr_plus_matches_oracle(built):
         assert abs(eg - ec) <= POT_TOL * pot
 
 
+def test_pcfr_plus_matches_oracle(built):
+    """PCFR+ (cfr.rs:250-289, 337-348: CFR+ discounts, the current strategy
+    matched on max(R,0) plus the last instantaneous regret) on the GPU
+    engine, which stores that sum per entry, against the oracle; river and
+    turn spots, exploitability every 10 iterations, and the arenas after
+    two iterations, bit for bit on the river spot."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: river spot, PCFR+, arenas after 2 iterations")
+    spot = Spot(workloads.river_spot())
+    gpu, cpu = GpuSolver(spot, algo="pcfr+"), oracle.Oracle(spot, algo="pcfr+")
+    gpu.iterate(2)
+    cpu.iterate(2)
+    (gr, gs), (cr, cs) = gpu.download_arenas(), cpu.arenas()
+    for g, c in zip((*gr, *gs), (*cr, *cs)):
+        scale = max(1.0, float(np.abs(c).max()))
+        assert np.allclose(g, c, rtol=0, atol=1e-5 * scale)
+    gpu.close()
+    for name, cfg, n in (("river", workloads.river_spot(), 60), ("turn", workloads.turn_spot(), 40)):
+        _say(f"case: {name} spot, PCFR+, {n} iterations, check every 10")
+        spot = Spot(cfg)
+        gpu, cpu = GpuSolver(spot, algo="pcfr+"), oracle.Oracle(spot, algo="pcfr+")
+        pot = spot.starting_pot
+        for k in range(n // 10):
+            gpu.iterate(10)
+            cpu.iterate(10)
+            eg, ec = gpu.exploitability(), cpu.exploitability()
+            _say(f"  iter {10 * (k + 1)}: gpu {eg:.6f} cpu {ec:.6f}")
+            assert abs(eg - ec) <= POT_TOL * pot
+        gpu.close()
+
+
+def test_pcfr_plus_needs_f32_arenas(built):
+    from gtopen_amd import GpuSolver, GtoError, Spot, workloads
+
+    spot = Spot(workloads.river_spot())
+    for st in ("compressed", "half"):
+        with pytest.raises(GtoError, match="PCFR"):
+            GpuSolver(spot, storage=st, algo="pcfr+")
+
+
 def test_turn_long_trajectory_compressed(built):
     """150 iterations of the compressed engine on the full-range turn spot
     (config 2) against the oracle every 25: quantization snaps both engines
