@@ -169,6 +169,19 @@ def main():
                 f"{its_t} iterations to {e_t:.4f}% pot in {sec:.3f}s" if reached
                 else f"target not reached: {e_t:.4f}% pot after {its_t} iterations"))
 
+    # the other algorithms of the solver crate on the same spot (f32 arenas)
+    algos = None
+    if not args.no_target and not args.no_mixed and args.storage == "f32":
+        algos = {}
+        for name in ("cfr+", "pcfr+"):
+            sec, its_t, e_t = time_to_target(spot, "f32", args.target, 1000, "exact", sched, name)
+            reached = e_t <= args.target
+            algos[name] = {"sec_to_target": round(sec, 3) if reached else None,
+                           "iters_to_target": its_t if reached else None,
+                           "exploitability_pct_at_stop": round(e_t, 4)}
+            log(f"bench: {name}: " + (f"{its_t} iterations to {e_t:.4f}% pot in {sec:.3f}s" if reached
+                                      else f"target not reached: {e_t:.4f}% pot after {its_t} iterations"))
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -213,18 +226,19 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "algorithms": algos,
         "schedule": sched,
     }
     print(json.dumps(out), flush=True)
 
 
-def time_to_target(spot, storage, target, cap, showdown="exact", sched=None):
+def time_to_target(spot, storage, target, cap, showdown="exact", sched=None, algo="dcfr"):
     """Wall time of a fresh engine to `target` % pot, best-response check
     every 10 iterations, at most `cap` iterations: (seconds, iterations, %pot).
     sched: the autotuner's choice on this spot, applied to the fresh engine."""
     from gtopen_amd import GpuSolver
 
-    g = GpuSolver(spot, storage=storage, showdown=showdown)
+    g = GpuSolver(spot, storage=storage, showdown=showdown, algo=algo)
     if sched is not None:
         g.set_schedule(sched["chosen"]["show_nodes"], sched["chosen"]["per_level"])
     t0 = time.perf_counter()
