@@ -35,6 +35,8 @@ def algo_code(name):
     if key not in codes:
         raise ValueError(f"unknown algorithm {name!r} (use dcfr, cfr+ or pcfr+)")
     return codes[key]
+
+
 STRICT, LENIENT_LOAD = 0, 1
 PROF_KINDS = 8
 
@@ -73,6 +75,9 @@ class SpotView(ctypes.Structure):
         ("river_cnt", c_u32p * 2),
         ("river_str", c_u32p * 2),
         ("river_idx", c_u16p * 2),
+        ("num_suit_perms", ctypes.c_uint32),
+        ("suit_perms", c_u8p),
+        ("hand_perm", c_u16p * 2),
     ]
 
 
@@ -82,6 +87,7 @@ class HipOptions(ctypes.Structure):
         ("algo", ctypes.c_int),
         ("use_graphs", ctypes.c_int),
         ("device", ctypes.c_int),
+        ("isomorphism", ctypes.c_int),
     ]
 
 

@@ -89,7 +89,7 @@ class Engine {
         if (pred_ && compressed_) throw std::runtime_error("PCFR+ runs with f32 arenas on the GPU");
         HIP_OK(hipSetDevice(o.device));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        plan_ = build_plan(v);
+        plan_ = build_plan(v, o.isomorphism != 0);
         num_nodes_ = v.num_nodes;
         data_size_[0] = v.data_size[0];
         data_size_[1] = v.data_size[1];
@@ -107,6 +107,7 @@ class Engine {
         upload_tree(v);
         alloc_state();
         reach_valid_[0] = reach_valid_[1] = false;
+        if (plan_.iso) build_iso(v);
     }
 
     ~Engine() {
@@ -224,6 +225,27 @@ class Engine {
         const int p = node_player_[n];
         const size_t na = node_na_[n], nh = plan_.nh[p], cnt = na * nh;
         const uint64_t off = node_off_[n];
+        if (iso_ && iso_src_[n] >= 0) {
+            // a branch CFR does not traverse: its strategy sums are those of
+            // the source it is copied from, suit-permuted (cfr.rs:806-840)
+            const IsoCopy& c = iso_copies_[(size_t)iso_src_[n]];
+            std::vector<float> base(cnt);
+            if (const float* lk = lock_host(c.src)) {
+                std::vector<float> sums(lk, lk + cnt);
+                if (compressed_ && !half_) {  // the copy is stored through the u16 quantizer
+                    std::vector<uint16_t> q(cnt);
+                    encode_u16(sums.data(), cnt, q.data());
+                    for (size_t e = 0; e < cnt; ++e) sums[e] = (float)q[e];
+                }
+                normalize_sums(sums.data(), na, nh, base.data(), compressed_ && !half_);
+            } else {
+                solved_strategy(c.src, base.data());
+            }
+            const uint16_t* tbl = &hand_perm_[p][(size_t)c.k * nh];
+            for (size_t a = 0; a < na; ++a)
+                for (size_t i = 0; i < nh; ++i) out[a * nh + i] = base[a * nh + tbl[i]];
+            return;
+        }
         if (!compressed_) {
             std::vector<float> b(cnt);
             HIP_OK(hipMemcpyAsync(b.data(), S_.S[p] + off, sizeof(float) * cnt, hipMemcpyDeviceToHost, stream_));
@@ -341,6 +363,12 @@ class Engine {
 
     void download(int which, int p, float* dst) {
         if (!dst || data_size_[p] == 0) return;
+        download_reps(which, p, dst);
+        if (iso_) symmetrize(which, p, dst);
+    }
+    // the arenas as the device holds them: branches CFR does not traverse
+    // (under suit isomorphism) are stale
+    void download_reps(int which, int p, float* dst) {
         if (!compressed_) {
             HIP_OK(hipMemcpyAsync(dst, which == 0 ? S_.R[p] : S_.S[p], sizeof(float) * data_size_[p],
                                   hipMemcpyDeviceToHost, stream_));
@@ -379,6 +407,7 @@ class Engine {
         if (rs) HIP_OK(hipMemcpyAsync(rs, S_.Rs[p], sizeof(float) * num_nodes_, hipMemcpyDeviceToHost, stream_));
         if (ss) HIP_OK(hipMemcpyAsync(ss, S_.Ss[p], sizeof(float) * num_nodes_, hipMemcpyDeviceToHost, stream_));
         HIP_OK(hipStreamSynchronize(stream_));
+        if (iso_ && !half_) symmetrize_raw(p, rq, sq, rs, ss);
     }
     void upload(int which, int p, const float* src) {
         if (!src || data_size_[p] == 0) return;
@@ -479,6 +508,10 @@ class Engine {
     void update_locks(const int64_t* lock_off, const float* sigma, uint64_t n_sigma) {
         HIP_OK(hipStreamSynchronize(stream_));
         drop_graph();
+        // host copies: symmetrize copies a locked source's strategy
+        lock_off_h_.assign(num_nodes_, -1);
+        if (lock_off) lock_off_h_.assign(lock_off, lock_off + num_nodes_);
+        lock_sigma_h_.assign(sigma, sigma + n_sigma);
         if (lock_sigma_buf_) {
             HIP_OK(hipFree(lock_sigma_buf_));
             allocs_.erase(std::find(allocs_.begin(), allocs_.end(), (void*)lock_sigma_buf_));
@@ -659,6 +692,187 @@ class Engine {
     }
 
    private:
+    // ---- suit isomorphism (cfr.rs:568-660, 749-870) -------------------------
+    // CFR traverses one chance branch per suit orbit. The other branches are
+    // copies of their representative under a suit symmetry; iso_copies_ lists
+    // those copies for action nodes in the order symmetrize performs them
+    // (representatives' own subtrees first, so a source is complete before it
+    // is copied), and downloads replay the list.
+    struct IsoCopy {
+        uint32_t dst, src, k;
+    };
+    struct IsoDealt {
+        uint8_t c[2] = {0, 0};
+        int n = 0;
+    };
+    bool iso_ = false;
+    std::vector<uint32_t> children_start_, children_;
+    std::vector<uint8_t> suit_perms_;
+    std::vector<uint16_t> hand_perm_[2];
+    std::vector<IsoCopy> iso_copies_;
+    std::vector<int64_t> iso_src_;  // per node: its entry in iso_copies_, or -1
+    std::vector<int64_t> lock_off_h_;
+    std::vector<float> lock_sigma_h_;
+
+    uint8_t permute_card(uint8_t c, uint32_t k) const { return (uint8_t)((c & ~3u) | suit_perms_[4 * k + (c & 3u)]); }
+    const float* lock_host(uint32_t n) const {
+        return (n < lock_off_h_.size() && lock_off_h_[n] >= 0) ? lock_sigma_h_.data() + lock_off_h_[n] : nullptr;
+    }
+
+    void build_iso(const gto_spot_view& v) {
+        iso_ = true;
+        children_start_.assign(v.node_children_start, v.node_children_start + v.num_nodes);
+        children_.assign(v.children, v.children + v.num_children_total);
+        suit_perms_.assign(v.suit_perms, v.suit_perms + 4 * (size_t)v.num_suit_perms);
+        for (int q = 0; q < 2; ++q)
+            hand_perm_[q].assign(v.hand_perm[q], v.hand_perm[q] + (size_t)v.num_suit_perms * v.num_hands[q]);
+        iso_src_.assign(num_nodes_, -1);
+        iso_walk(0, IsoDealt());
+    }
+    // symmetrize_node, cfr.rs:758-800
+    void iso_walk(uint32_t n, IsoDealt d) {
+        const uint32_t cs = children_start_[n];
+        if (node_kind_[n] == GTO_KIND_ACTION) {
+            for (uint32_t a = 0; a < node_na_[n]; ++a) iso_walk(children_[cs + a], d);
+        } else if (node_kind_[n] == GTO_KIND_CHANCE) {
+            std::vector<uint32_t> valid;
+            for (uint32_t k = 0; k < suit_perms_.size() / 4; ++k) {
+                bool fix = true;
+                for (int i = 0; i < d.n; ++i)
+                    if (permute_card(d.c[i], k) != d.c[i]) fix = false;
+                if (fix) valid.push_back(k);
+            }
+            std::vector<std::array<uint32_t, 3>> orbit;  // card, rep, symmetry
+            for (uint32_t c = 0; c < 52; ++c) {
+                if (children_[cs + c] == 0xFFFFFFFFu) continue;
+                bool dealt = false;
+                for (int i = 0; i < d.n; ++i) dealt |= d.c[i] == c;
+                if (dealt) continue;
+                uint32_t rep = c, k_rep = 0;
+                for (uint32_t k : valid) {
+                    const uint32_t pc = permute_card((uint8_t)c, k);
+                    if (pc < rep) {
+                        rep = pc;
+                        k_rep = k;
+                    }
+                }
+                orbit.push_back({c, rep, k_rep});
+            }
+            for (const auto& o : orbit)
+                if (o[0] == o[1]) {
+                    IsoDealt nd = d;
+                    nd.c[nd.n++] = (uint8_t)o[0];
+                    iso_walk(children_[cs + o[0]], nd);
+                }
+            for (const auto& o : orbit)
+                if (o[0] != o[1]) iso_copy(children_[cs + o[1]], children_[cs + o[0]], o[2]);
+        }
+    }
+    // copy_branch, cfr.rs:806-870
+    void iso_copy(uint32_t src, uint32_t dst, uint32_t k) {
+        if (node_kind_[src] == GTO_KIND_ACTION) {
+            iso_src_[dst] = (int64_t)iso_copies_.size();
+            iso_copies_.push_back({dst, src, k});
+            const uint32_t ss = children_start_[src], ds = children_start_[dst];
+            for (uint32_t a = 0; a < node_na_[src]; ++a) iso_copy(children_[ss + a], children_[ds + a], k);
+        } else if (node_kind_[src] == GTO_KIND_CHANCE) {
+            const uint32_t ss = children_start_[src], ds = children_start_[dst];
+            for (uint32_t x = 0; x < 52; ++x) {
+                const uint32_t dc = children_[ds + x];
+                if (dc == 0xFFFFFFFFu) continue;
+                iso_copy(children_[ss + permute_card((uint8_t)x, k)], dc, k);
+            }
+        }
+    }
+    // u16 quantizer of the strategy arenas (store.rs:199-215); returns the scale
+    static float encode_u16(const float* src, size_t cnt, uint16_t* q) {
+        float m = 0.0f;
+        for (size_t e = 0; e < cnt; ++e) m = std::fmax(m, src[e]);
+        if (!(m > 0.0f && std::isfinite(m))) {
+            std::fill(q, q + cnt, (uint16_t)0);
+            return 0.0f;
+        }
+        const float k = 65535.0f / m;
+        for (size_t e = 0; e < cnt; ++e) {
+            float r = std::nearbyint(std::fmax(src[e], 0.0f) * k);
+            r = r != r ? 0.0f : std::fmin(std::fmax(r, 0.0f), 65535.0f);
+            q[e] = (uint16_t)r;
+        }
+        return m;
+    }
+    // solved_strategy's normalisation of one block of sums: f64 sums
+    // (f32 store) or integer sums of u16 quants (cfr.rs:679-722)
+    static void normalize_sums(const float* b, size_t na, size_t nh, float* out, bool quants) {
+        for (size_t i = 0; i < nh; ++i) {
+            if (quants) {
+                uint32_t sum = 0;
+                for (size_t a = 0; a < na; ++a) sum += (uint32_t)b[a * nh + i];
+                const float inv = sum > 0 ? 1.0f / (float)sum : 0.0f;
+                for (size_t a = 0; a < na; ++a) out[a * nh + i] = sum > 0 ? b[a * nh + i] * inv : 1.0f / (float)na;
+            } else {
+                double sum = 0.0;
+                for (size_t a = 0; a < na; ++a) sum += (double)std::max(b[a * nh + i], 0.0f);
+                for (size_t a = 0; a < na; ++a)
+                    out[a * nh + i] =
+                        sum > 1e-12 ? (float)((double)std::max(b[a * nh + i], 0.0f) / sum) : 1.0f / (float)na;
+            }
+        }
+    }
+    // ensure_symmetric on a downloaded f32 arena of player p (which 0 =
+    // regrets, 1 = strategy sums; a locked source gives its strategy)
+    void symmetrize(int which, int p, float* a) const {
+        const size_t nh = plan_.nh[p];
+        std::vector<float> tmp;
+        for (const IsoCopy& c : iso_copies_) {
+            if (node_player_[c.dst] != p) continue;
+            const size_t na = node_na_[c.dst], cnt = na * nh;
+            const uint16_t* tbl = &hand_perm_[p][(size_t)c.k * nh];
+            const float* src = a + node_off_[c.src];
+            const float* lk = which == 1 ? lock_host(c.src) : nullptr;
+            if (lk) {
+                tmp.assign(lk, lk + cnt);
+                if (compressed_ && !half_) {  // stored through the u16 quantizer, then decoded
+                    std::vector<uint16_t> q(cnt);
+                    const float m = encode_u16(tmp.data(), cnt, q.data());
+                    for (size_t e = 0; e < cnt; ++e) tmp[e] = (float)q[e] * (m / 65535.0f);
+                }
+                src = tmp.data();
+            }
+            float* d = a + node_off_[c.dst];
+            for (size_t x = 0; x < na; ++x)
+                for (size_t i = 0; i < nh; ++i) d[x * nh + i] = src[x * nh + tbl[i]];
+        }
+    }
+    // the same on raw i16/u16 blocks: a permuted block keeps its max, so the
+    // quants are permuted and the scale copied
+    void symmetrize_raw(int p, int16_t* rq, uint16_t* sq, float* rs, float* ss) const {
+        const size_t nh = plan_.nh[p];
+        for (const IsoCopy& c : iso_copies_) {
+            if (node_player_[c.dst] != p) continue;
+            const size_t na = node_na_[c.dst], cnt = na * nh;
+            const uint16_t* tbl = &hand_perm_[p][(size_t)c.k * nh];
+            const uint64_t so = node_off_[c.src], dof = node_off_[c.dst];
+            if (rq)
+                for (size_t x = 0; x < na; ++x)
+                    for (size_t i = 0; i < nh; ++i) rq[dof + x * nh + i] = rq[so + x * nh + tbl[i]];
+            if (rs) rs[c.dst] = rs[c.src];
+            if (const float* lk = lock_host(c.src)) {
+                std::vector<float> perm(cnt);
+                for (size_t x = 0; x < na; ++x)
+                    for (size_t i = 0; i < nh; ++i) perm[x * nh + i] = lk[x * nh + tbl[i]];
+                std::vector<uint16_t> q(cnt);
+                const float m = encode_u16(perm.data(), cnt, q.data());
+                if (sq) std::copy(q.begin(), q.end(), sq + dof);
+                if (ss) ss[c.dst] = m;
+            } else {
+                if (sq)
+                    for (size_t x = 0; x < na; ++x)
+                        for (size_t i = 0; i < nh; ++i) sq[dof + x * nh + i] = sq[so + x * nh + tbl[i]];
+                if (ss) ss[c.dst] = ss[c.src];
+            }
+        }
+    }
+
     gto_hip_options opts_;
     bool compressed_ = false;  // 16-bit arenas (i16/u16 quants or f16)
     bool pred_ = false;        // PCFR+: current strategy matched on S_.Q
@@ -732,6 +946,11 @@ class Engine {
         T_.kids = upload_vec(plan_.kids);
         T_.chr_card = upload_vec(plan_.chr_card);
         T_.chr_cfv = upload_vec(plan_.chr_cfv);
+        T_.chr_perm = upload_vec(plan_.chr_perm);
+        if (plan_.iso)
+            for (int q = 0; q < 2; ++q)
+                T_.hand_perm[q] = upload_vec(std::vector<uint16_t>(
+                    v.hand_perm[q], v.hand_perm[q] + (size_t)v.num_suit_perms * v.num_hands[q]));
         T_.nh_max = plan_.nh_max;
         for (int q = 0; q < 2; ++q) {
             const uint32_t nh = v.num_hands[q];
@@ -1228,7 +1447,7 @@ int gto_hip_create(const gto_spot_view* spot, const gto_hip_options* opts, gto_h
         return GTO_ERR_INVALID;
     }
     *out = nullptr;
-    gto_hip_options o = {GTO_STORAGE_F32, GTO_ALGO_DCFR, 1, 0};
+    gto_hip_options o = {GTO_STORAGE_F32, GTO_ALGO_DCFR, 1, 0, 1};
     if (opts) o = *opts;
     if (o.algo != GTO_ALGO_DCFR && o.algo != GTO_ALGO_CFRPLUS && o.algo != GTO_ALGO_PCFRPLUS) {
         set_last_error("unknown algorithm");

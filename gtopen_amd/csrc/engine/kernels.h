@@ -37,7 +37,9 @@ struct DevTree {
     const float* lock_sigma;
     const uint32_t* kids;      // ActRec children: value rows, then reach slots
     const uint8_t* chr_card;   // ChanceRec children: dealt card
-    const uint32_t* chr_cfv;   // ChanceRec children: value row
+    const uint32_t* chr_cfv;   // ChanceRec children: value row (the orbit representative's)
+    const uint8_t* chr_perm;   // ChanceRec children: suit symmetry onto the representative (0 = identity)
+    const uint16_t* hand_perm[2];  // [k * nh_q + i]: hand i's image under symmetry k
     uint32_t nh[2];
     uint32_t nh_max;
     const uint8_t* c1[2];

@@ -1418,21 +1418,44 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
     float vm = 0.0f;
     const uint8_t* __restrict__ cards = T.chr_card + r.off;
     const uint32_t* __restrict__ rows = T.chr_cfv + r.off;
+    const uint8_t* __restrict__ perms = T.chr_perm + r.off;
+    const uint16_t* __restrict__ hperm = T.hand_perm[p];
     const float* __restrict__ cfv = S.cfv;
+    // under suit isomorphism entry t reads its representative's row at the
+    // image of hand i under the entry's symmetry (identity: i itself;
+    // cfr.rs:641-656); without it every entry reads its own child's row at i
+    auto at = [&](uint32_t t, uint32_t i) -> uint32_t {
+        if (!hperm) return i;
+        return perms[t] ? hperm[(size_t)perms[t] * nhp + i] : i;
+    };
     for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
         const uint64_t hm = T.hmask[p][i];
         float acc = 0.0f;
         uint32_t t = 0;
-        for (; t + 8 <= r.cnt; t += 8) {
-            float v[8];
+        if (!hperm) {
+            for (; t + 8 <= r.cnt; t += 8) {
+                float v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = cfv[(size_t)rows[t + u] * T.nh_max + i];
+                for (int u = 0; u < 8; ++u) v[u] = cfv[(size_t)rows[t + u] * T.nh_max + i];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (!((hm >> cards[t + u]) & 1ull)) acc += v[u];
+                for (int u = 0; u < 8; ++u)
+                    if (!((hm >> cards[t + u]) & 1ull)) acc += v[u];
+            }
+        } else {
+            for (; t + 8 <= r.cnt; t += 8) {
+                uint32_t ix[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) ix[u] = at(t + u, i);
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = cfv[(size_t)rows[t + u] * T.nh_max + ix[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (!((hm >> cards[t + u]) & 1ull)) acc += v[u];
+            }
         }
         for (; t < r.cnt; ++t)
-            if (!((hm >> cards[t]) & 1ull)) acc += cfv[(size_t)rows[t] * T.nh_max + i];
+            if (!((hm >> cards[t]) & 1ull)) acc += cfv[(size_t)rows[t] * T.nh_max + at(t, i)];
         out[i] = acc * r.inv;
         vm = fmaxf(vm, fabsf(acc * r.inv));
     }

@@ -29,10 +29,14 @@ uint32_t river_key_of(const gto_spot_view& v, const Item& it) {
     uint32_t a = std::min(it.dealt[0], it.dealt[1]), b = std::max(it.dealt[0], it.dealt[1]);
     return a * 52 + b;
 }
+// permute_card, cards.rs:94-96 (card = 4 * rank + suit)
+inline uint8_t permute_card(uint8_t c, const uint8_t* pm) { return (uint8_t)((c & ~3u) | pm[c & 3u]); }
 }  // namespace
 
-Plan build_plan(const gto_spot_view& v) {
+Plan build_plan(const gto_spot_view& v, bool iso) {
     Plan pl;
+    iso = iso && v.num_suit_perms > 1;
+    pl.iso = iso;
     const uint32_t N = v.num_nodes;
     pl.num_nodes = N;
     pl.nh[0] = v.num_hands[0];
@@ -81,6 +85,17 @@ Plan build_plan(const gto_spot_view& v) {
             }
         } else if (kind == GTO_KIND_CHANCE) {
             pl.ch_off[n] = (uint32_t)pl.ch_child.size();
+            // suit isomorphism (plan.rs:154-191, cfr.rs:568-603): symmetries
+            // fixing the dealt cards; each card's orbit representative is its
+            // smallest image, and only representatives are traversed
+            std::vector<uint32_t> valid;
+            if (iso)
+                for (uint32_t k = 0; k < v.num_suit_perms; ++k) {
+                    bool fix = true;
+                    for (int d = 0; d < it.ndealt; ++d)
+                        if (permute_card(it.dealt[d], v.suit_perms + 4 * k) != it.dealt[d]) fix = false;
+                    if (fix) valid.push_back(k);
+                }
             std::vector<Item> kids;
             for (uint32_t c = 0; c < 52; ++c) {
                 uint32_t child = v.children[cs + c];
@@ -88,8 +103,21 @@ Plan build_plan(const gto_spot_view& v) {
                 bool dealt = false;
                 for (int d = 0; d < it.ndealt; ++d) dealt |= it.dealt[d] == c;
                 if (dealt) continue;
-                pl.ch_child.push_back(child);
+                uint8_t rep = (uint8_t)c;
+                uint32_t k_rep = 0;
+                for (uint32_t k : valid) {
+                    const uint8_t pc = permute_card((uint8_t)c, v.suit_perms + 4 * k);
+                    if (pc < rep) {
+                        rep = pc;
+                        k_rep = k;
+                    }
+                }
+                // the value row of the representative, read through the hand
+                // table of the symmetry that maps this card onto it
+                pl.ch_child.push_back(v.children[cs + rep]);
                 pl.ch_card.push_back((uint8_t)c);
+                pl.ch_perm.push_back((uint8_t)k_rep);
+                if (rep != c) continue;
                 Item k = it;
                 k.node = child;
                 k.level = it.level + 1;
@@ -107,7 +135,7 @@ Plan build_plan(const gto_spot_view& v) {
                 edges_c[it.level].push_back(child);
                 edges_card[it.level].push_back(c);
             }
-            pl.ch_cnt[n] = (uint32_t)kids.size();
+            pl.ch_cnt[n] = (uint32_t)(pl.ch_child.size() - pl.ch_off[n]);
             for (size_t k = kids.size(); k-- > 0;) stack.push_back(kids[k]);
         } else if (kind == GTO_KIND_SHOWDOWN) {
             key[n] = river_key_of(v, it);
@@ -350,6 +378,7 @@ Plan build_plan(const gto_spot_view& v) {
             for (uint32_t t = 0; t < pl.ch_cnt[n]; ++t) {
                 pl.chr_card.push_back(pl.ch_card[pl.ch_off[n] + t]);
                 pl.chr_cfv.push_back(pl.cfv_slot[pl.ch_child[pl.ch_off[n] + t]]);
+                pl.chr_perm.push_back(pl.ch_perm[pl.ch_off[n] + t]);
             }
             ll.chance_rec.push_back(r);
         }

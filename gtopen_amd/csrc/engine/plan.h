@@ -51,8 +51,14 @@ struct Plan {
 
     // chance nodes: valid (card, child) lists in card order
     std::vector<uint32_t> ch_off, ch_cnt;  // per node
+    // per valid card of a chance node: the value row's node (the card's
+    // orbit representative under suit isomorphism, else its own child), the
+    // card, and the symmetry mapping the card onto its representative
+    // (0 = identity)
     std::vector<uint32_t> ch_child;
     std::vector<uint8_t> ch_card;
+    std::vector<uint8_t> ch_perm;
+    bool iso = false;  // suit isomorphism active (symmetries beyond the identity)
 
     // showdown nodes -> dense river-board slot
     std::vector<uint32_t> show_slot;  // per node (kSentinel elsewhere)
@@ -94,6 +100,7 @@ struct Plan {
     // chance children referenced by ChanceRec: dealt card and value row
     std::vector<uint8_t> chr_card;
     std::vector<uint32_t> chr_cfv;
+    std::vector<uint8_t> chr_perm;  // symmetry index per entry (0 = identity)
     // per board slot and hand of q (hand order), four words: rank bounds,
     // per-card list segments, per-card counts, list lengths + identical
     // combo position (layout documented in build_plan)
@@ -115,6 +122,7 @@ struct Plan {
     uint64_t reached_entries_by_player[2] = {0, 0};
 };
 
-Plan build_plan(const gto_spot_view& v);
+// iso: traverse one chance branch per suit orbit (cfr.rs:84-87)
+Plan build_plan(const gto_spot_view& v, bool iso = true);
 
 }  // namespace gto

@@ -116,6 +116,9 @@ struct Spot {
     std::vector<uint32_t> river_str[2];
     std::vector<uint16_t> river_idx[2];
     std::vector<std::array<uint8_t, 4>> suit_perms;
+    std::vector<uint8_t> suit_perms_flat;  // 4 per symmetry
+    // [k * nh + i]: index of hand i's image under suit_perms[k] (game.rs:294-306)
+    std::vector<uint16_t> hand_perm[2];
     size_t num_hands(int p) const { return hand_c1[p].size(); }
 };
 std::unique_ptr<Spot> build_spot(const SpotConfig& cfg, uint64_t max_nodes, Strictness strictness);

@@ -173,6 +173,16 @@ std::unique_ptr<Spot> build_spot(const SpotConfig& cfg, uint64_t max_nodes, Stri
                 }
     for (size_t k = 0; k < sp->suit_perms.size(); ++k)
         if (sp->suit_perms[k] == std::array<uint8_t, 4>{0, 1, 2, 3}) std::swap(sp->suit_perms[0], sp->suit_perms[k]);
+    // hand tables of each symmetry (every image is in the range: checked above)
+    for (int p = 0; p < 2; ++p) {
+        std::vector<int32_t> idx_of(1326, -1);
+        for (size_t i = 0; i < nh[p]; ++i) idx_of[combo_index(sp->hand_c1[p][i], sp->hand_c2[p][i])] = (int32_t)i;
+        for (const auto& pm : sp->suit_perms)
+            for (size_t i = 0; i < nh[p]; ++i)
+                sp->hand_perm[p].push_back(
+                    (uint16_t)idx_of[combo_index(permute(sp->hand_c1[p][i], pm), permute(sp->hand_c2[p][i], pm))]);
+    }
+    for (const auto& pm : sp->suit_perms) sp->suit_perms_flat.insert(sp->suit_perms_flat.end(), pm.begin(), pm.end());
     return sp;
 }
 
@@ -241,6 +251,9 @@ int gto_spot_build(const char* config_json, int strictness, uint64_t max_nodes, 
             v.river_idx[p] = s.river_idx[p].data();
         }
         v.num_river_keys = s.num_river_keys;
+        v.num_suit_perms = (uint32_t)s.suit_perms.size();
+        v.suit_perms = s.suit_perms_flat.data();
+        for (int p = 0; p < 2; ++p) v.hand_perm[p] = s.hand_perm[p].data();
         *out = h;
         return GTO_OK;
     } catch (const std::bad_alloc&) {

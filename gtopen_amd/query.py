@@ -140,9 +140,45 @@ class QueryMixin:
 
     # --- paths -------------------------------------------------------------
     def canonical_path(self, path):
-        """Card steps mapped to their suit-orbit representatives
-        (query.rs:645-700). Every runout is its own branch here: identity."""
-        return list(path)
+        """Card steps mapped to their suit-orbit representatives, so
+        operations land on the branch CFR traverses (query.rs:643-647)."""
+        return self.canonical_path_with_perm(path)[0]
+
+    def canonical_path_with_perm(self, path):
+        """Canonical path plus the composed suit map from the browsed branch
+        to the canonical one (query.rs:649-700): each card is first moved by
+        the map so far, then replaced by its smallest image under the
+        symmetries fixing the canonical cards dealt before it (the same
+        strictly-less scan as the engine's symmetrize)."""
+        composed = [0, 1, 2, 3]
+        perms = self.spot.suit_perms
+        if not getattr(self, "isomorphism", False) or len(perms) < 2:
+            return list(path), composed
+
+        def permute(c, pm):
+            return (c & ~3) | pm[c & 3]
+
+        dealt, out = [], []
+        for st in path:
+            kind, val = _step(st)
+            cs = parse_cards(val) if kind == "card" else []
+            if kind != "card" or len(cs) != 1:
+                out.append(st)
+                continue
+            cm = permute(cs[0], composed)
+            rep, k_rep = cm, None
+            for k, pm in enumerate(perms):
+                if any(permute(d, pm) != d for d in dealt):
+                    continue
+                pc = permute(cm, pm)
+                if pc < rep:
+                    rep, k_rep = pc, k
+            if k_rep is not None:
+                pm = perms[k_rep]
+                composed = [pm[s] for s in composed]
+            dealt.append(rep)
+            out.append({"type": "card", "card": card_str(rep)} if isinstance(st, dict) else card_str(rep))
+        return out, composed
 
     def walk_path(self, path):
         """Follow a path from the root, multiplying the acting player's reach

@@ -11,13 +11,20 @@ from .query import QueryMixin
 
 
 class GpuSolver(QueryMixin):
-    def __init__(self, spot, storage="f32", algo="dcfr", use_graphs=True, device=0, showdown="exact"):
+    def __init__(self, spot, storage="f32", algo="dcfr", use_graphs=True, device=0, showdown="exact",
+                 isomorphism=True):
+        """isomorphism: traverse one chance branch per suit orbit (the
+        reference's default use_isomorphism, cfr.rs:84-87, 143); downloads
+        and strategies of the other branches are their suit-permuted
+        copies, as after the reference's ensure_symmetric."""
         self.spot = spot  # keeps the view's arrays alive
+        self.isomorphism = bool(isomorphism)
         opts = L.HipOptions(
             L.STORAGE[storage],
             L.algo_code(algo),
             1 if use_graphs else 0,
             device,
+            1 if isomorphism else 0,
         )
         self.storage = storage
         self.locks, self.lock_labels = {}, {}

@@ -155,6 +155,13 @@ class Spot:
     def num_suit_perms(self):
         return int(L.lib().gto_spot_num_suit_perms(self._h))
 
+    @property
+    def suit_perms(self):
+        """Suit symmetries of board and ranges, identity first (game.rs:262-317):
+        symmetry k maps suit s to suit_perms[k][s]."""
+        v = self.view
+        return [tuple(int(v.suit_perms[4 * k + s]) for s in range(4)) for k in range(int(v.num_suit_perms))]
+
     def child(self, node, a):
         return int(self.children[int(self.children_start[node]) + a])
 

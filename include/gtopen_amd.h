@@ -119,6 +119,11 @@ typedef struct gto_spot_view {
     const uint32_t* river_cnt[2];
     const uint32_t* river_str[2];
     const uint16_t* river_idx[2];
+    /* suit symmetries fixing the board pointwise with both ranges invariant
+     * (game.rs:262-317), identity first: suit s -> suit_perms[4k + s] */
+    uint32_t num_suit_perms;
+    const uint8_t* suit_perms;
+    const uint16_t* hand_perm[2]; /* [k * nh_q + i]: index of hand i's image under symmetry k */
 } gto_spot_view;
 
 /* Build a spot from the reference's SpotConfig JSON (game.rs:21-27, serde
@@ -199,6 +204,8 @@ typedef struct gto_hip_options {
     int algo;           /* GTO_ALGO_DCFR, GTO_ALGO_CFRPLUS or GTO_ALGO_PCFRPLUS (f32 arenas) */
     int use_graphs;     /* capture each sweep into a hipGraph after the first iteration */
     int device;         /* HIP device ordinal */
+    int isomorphism;    /* 1: one chance branch per suit orbit, as the reference's default
+                         * use_isomorphism (cfr.rs:84-87, 143); 0: every branch */
 } gto_hip_options;
 
 /* Create an engine over `spot` (copied to the device; the view may be freed

@@ -48,6 +48,7 @@ def lib():
             ("oracle_get_arenas_f64", None, [vp, f64p, f64p, f64p, f64p]),
             ("oracle_set_locks", None, [vp, i64p, f32p, ctypes.c_uint64]),
             ("oracle_average_strategy", None, [vp, ctypes.c_uint32, f32p]),
+            ("oracle_set_isomorphism", None, [vp, ctypes.c_int]),
             ("oracle_fold_cfv", None, [sp, ctypes.c_int, f32p, ctypes.c_float, f32p]),
             ("oracle_showdown_cfv", None, [sp, ctypes.c_int, ctypes.c_uint32, f32p, ctypes.c_float, ctypes.c_float,
                                            ctypes.c_float, f32p]),
@@ -77,7 +78,7 @@ class Oracle:
     storage 'f32' or 'compressed' (i16/u16 arenas, precision 32 only).
     """
 
-    def __init__(self, spot, precision=32, storage="f32", algo="dcfr", weights=None):
+    def __init__(self, spot, precision=32, storage="f32", algo="dcfr", weights=None, isomorphism=True):
         """weights=(w0, w1): per-hand range weights replacing the spot's (the
         engine reads them through a private copy of the spot's view, so
         set_weights can change them later)."""
@@ -96,6 +97,8 @@ class Oracle:
         if key not in codes:
             raise ValueError(f"unknown algorithm {algo!r} (use dcfr, cfr+ or pcfr+)")
         self._h = lib().oracle_create(view_ptr, precision, 0 if storage == "f32" else 1, codes[key])
+        if not isomorphism:
+            lib().oracle_set_isomorphism(self._h, 0)
 
     def set_weights(self, w0, w1):
         if not hasattr(self, "_w"):

@@ -179,6 +179,9 @@ struct Dealt {  // game.rs:65-80
     }
 };
 
+// permute_card, cards.rs:94-96 (card = 4 * rank + suit)
+inline uint8_t permute_card(uint8_t c, const uint8_t* pm) { return (uint8_t)((c & ~3u) | pm[c & 3u]); }
+
 uint32_t river_key(const gto_spot_view* v, const Dealt& d) {  // game.rs:43-56
     if (v->root_street == 2) return 0;
     if (v->root_street == 1) return d.cards[0];
@@ -344,6 +347,8 @@ struct EngineBase {
     virtual void get_arenas_f64(int which, int p, double* out) = 0;
     virtual void set_arenas(int which, int p, const float* in) = 0;
     virtual void average_strategy(uint32_t node, float* out) = 0;
+    virtual void set_isomorphism(bool on) = 0;
+    virtual void mark_sym_dirty() = 0;
     uint32_t iteration = 0;
     std::vector<int64_t> lock_off;
     std::vector<float> lock_sigma;
@@ -366,6 +371,38 @@ struct Engine : EngineBase {
     std::vector<float> Ps[2];
     bool compressed() const { return storage == GTO_STORAGE_COMPRESSED; }
     bool predictive() const { return algo == GTO_ALGO_PCFRPLUS; }
+    // suit isomorphism (cfr.rs:84-87, 143): traverse one chance branch per
+    // orbit; the other branches' arenas are stale until symmetrize()
+    bool use_iso = true;
+    bool sym_dirty = false;
+    bool iso() const { return use_iso && v->num_suit_perms > 1; }
+    const uint8_t* perm(size_t k) const { return v->suit_perms + 4 * k; }
+
+    // Spot::perms_fixing, game.rs:335-344
+    std::vector<size_t> perms_fixing(const Dealt& d) const {
+        std::vector<size_t> ks;
+        for (size_t k = 0; k < v->num_suit_perms; ++k) {
+            bool ok = true;
+            for (int i = 0; i < d.len; ++i)
+                if (permute_card(d.cards[i], perm(k)) != d.cards[i]) ok = false;
+            if (ok) ks.push_back(k);
+        }
+        return ks;
+    }
+    // orbit representative (smallest image) and the symmetry that maps the
+    // card onto it, strictly-less scan over the valid symmetries
+    // (cfr.rs:575-589; symmetrize_node 776-787)
+    void orbit(uint8_t c, const std::vector<size_t>& valid, uint8_t* rep, size_t* k_rep) const {
+        *rep = c;
+        *k_rep = 0;
+        for (size_t k : valid) {
+            uint8_t pc = permute_card(c, perm(k));
+            if (pc < *rep) {
+                *rep = pc;
+                *k_rep = k;
+            }
+        }
+    }
 
     Engine(const gto_spot_view* view, int st, int al) : v(view), storage(st), algo(al) {
         for (int p = 0; p < 2; ++p) {
@@ -529,39 +566,141 @@ struct Engine : EngineBase {
         return true;
     }
 
-    // chance_node, cfr.rs:539-664 (identity orbits: every valid card is its
-    // own representative). `rec(child, rp, ro, dealt, out)`.
+    // chance_node, cfr.rs:539-664: one child per suit orbit is traversed
+    // (every valid card is its own orbit when iso is off); the others take
+    // the representative's values through the hand table of the symmetry
+    // that maps them onto it. `rec(child, rp, ro, dealt, out)`.
     template <class F>
     void chance_node(uint32_t n, int p, const T* reach_p, const T* reach_o, Dealt dealt, T* out, const F& rec) {
         const int o = 1 - p;
         const size_t h = nh[p], ho = nh[o];
         T divisor = (T)(46 - (int)v->node_street[n]);
-        uint8_t cards[52];
-        size_t nc = 0;
+        uint8_t cards[52], rep_of[52], reps[52];
+        size_t perm_of[52], slot_of[52];
+        size_t nc = 0, nr = 0;
         uint32_t cs = v->node_children_start[n];
         for (int c = 0; c < 52; ++c)
             if (v->children[cs + c] != kSent && !dealt.contains((uint8_t)c)) cards[nc++] = (uint8_t)c;
-        bool parallel = nc > 0 && v->node_kind[v->children[cs + cards[0]]] == GTO_KIND_ACTION;
-        Buf<T> results(nc * h, true);
-        auto run_child = [&](size_t k) {
+        const std::vector<size_t> valid = iso() ? perms_fixing(dealt) : std::vector<size_t>{};
+        for (size_t k = 0; k < nc; ++k) {
             uint8_t c = cards[k];
+            rep_of[c] = c;
+            perm_of[c] = 0;
+            if (iso()) orbit(c, valid, &rep_of[c], &perm_of[c]);
+            if (rep_of[c] == c) {
+                slot_of[c] = nr;
+                reps[nr++] = c;
+            }
+        }
+        bool parallel = nc > 0 && v->node_kind[v->children[cs + cards[0]]] == GTO_KIND_ACTION;
+        Buf<T> results(nr * h, true);
+        auto run_child = [&](size_t k) {
+            uint8_t c = reps[k];
             uint64_t cm = 1ull << c;
             Buf<T> rp(h, false), ro(ho, false);
             for (size_t i = 0; i < h; ++i) rp[i] = (hmask[p][i] & cm) ? T(0) : reach_p[i];
             for (size_t j = 0; j < ho; ++j) ro[j] = (hmask[o][j] & cm) ? T(0) : reach_o[j];
             rec(v->children[cs + c], rp.data(), ro.data(), dealt.push(c), results.data() + k * h);
         };
-        if (parallel) pool().parallel_for(nc, run_child);
+        if (parallel) pool().parallel_for(nr, run_child);
         else
-            for (size_t k = 0; k < nc; ++k) run_child(k);
+            for (size_t k = 0; k < nr; ++k) run_child(k);
         for (size_t k = 0; k < nc; ++k) {
-            uint64_t cm = 1ull << cards[k];
-            const T* co = results.data() + k * h;
-            for (size_t i = 0; i < h; ++i)
-                if (!(hmask[p][i] & cm)) out[i] += co[i];
+            uint8_t c = cards[k];
+            uint64_t cm = 1ull << c;
+            const T* co = results.data() + slot_of[rep_of[c]] * h;
+            if (rep_of[c] == c) {
+                for (size_t i = 0; i < h; ++i)
+                    if (!(hmask[p][i] & cm)) out[i] += co[i];
+            } else {
+                const uint16_t* tbl = v->hand_perm[p] + perm_of[c] * h;
+                for (size_t i = 0; i < h; ++i)
+                    if (!(hmask[p][i] & cm)) out[i] += co[tbl[i]];
+            }
         }
         T inv = T(1) / divisor;
         for (size_t i = 0; i < h; ++i) out[i] *= inv;
+    }
+
+    // ensure_symmetric / symmetrize_node / copy_branch, cfr.rs:749-870:
+    // representatives' subtrees first, then each orbit member takes a
+    // suit-permuted copy of its representative's subtree (regrets, strategy
+    // sums, or a locked source's strategy; predictions are not copied)
+    void ensure_symmetric() {
+        if (!sym_dirty || !iso()) {
+            sym_dirty = false;
+            return;
+        }
+        symmetrize_node(0, Dealt());
+        sym_dirty = false;
+    }
+    void symmetrize_node(uint32_t n, Dealt dealt) {
+        uint32_t cs = v->node_children_start[n];
+        if (v->node_kind[n] == GTO_KIND_ACTION) {
+            for (size_t a = 0; a < v->node_num_children[n]; ++a) symmetrize_node(child(n, a), dealt);
+        } else if (v->node_kind[n] == GTO_KIND_CHANCE) {
+            const std::vector<size_t> valid = perms_fixing(dealt);
+            std::vector<uint8_t> cards, reps;
+            std::vector<size_t> ks;
+            for (int c = 0; c < 52; ++c)
+                if (v->children[cs + c] != kSent && !dealt.contains((uint8_t)c)) {
+                    uint8_t rep;
+                    size_t k;
+                    orbit((uint8_t)c, valid, &rep, &k);
+                    cards.push_back((uint8_t)c);
+                    reps.push_back(rep);
+                    ks.push_back(k);
+                }
+            for (size_t j = 0; j < cards.size(); ++j)
+                if (cards[j] == reps[j]) symmetrize_node(v->children[cs + cards[j]], dealt.push(cards[j]));
+            for (size_t j = 0; j < cards.size(); ++j)
+                if (cards[j] != reps[j]) copy_branch(v->children[cs + reps[j]], v->children[cs + cards[j]], ks[j]);
+        }
+    }
+    void copy_branch(uint32_t src, uint32_t dst, size_t k) {
+        if (v->node_kind[src] == GTO_KIND_ACTION) {
+            const int p = v->node_player[src];
+            const size_t h = nh[p], na = v->node_num_children[src], cnt = na * h;
+            const uint16_t* tbl = v->hand_perm[p] + k * h;
+            const uint64_t so = v->node_data_offset[src], dof = v->node_data_offset[dst];
+            const float* lk = lock_of(src);
+            if (!compressed()) {
+                for (size_t a = 0; a < na; ++a)
+                    for (size_t i = 0; i < h; ++i) {
+                        const size_t j = tbl[i];
+                        R[p][dof + a * h + i] = R[p][so + a * h + j];
+                        S[p][dof + a * h + i] = lk ? (T)lk[a * h + j] : S[p][so + a * h + j];
+                    }
+            } else {
+                // permuting a block leaves its max, hence its scale and the
+                // re-encoded quants, unchanged (store.rs:169-215)
+                for (size_t a = 0; a < na; ++a)
+                    for (size_t i = 0; i < h; ++i) Rq[p][dof + a * h + i] = Rq[p][so + a * h + tbl[i]];
+                Rs[p][dst] = Rs[p][src];
+                if (lk) {
+                    std::vector<float> sb(cnt);
+                    float smax = 0.0f;
+                    for (size_t a = 0; a < na; ++a)
+                        for (size_t i = 0; i < h; ++i) {
+                            sb[a * h + i] = lk[a * h + tbl[i]];
+                            smax = std::fmax(smax, sb[a * h + i]);
+                        }
+                    Ss[p][dst] = quantize_u16(sb.data(), cnt, smax, Sq[p].data() + dof);
+                } else {
+                    for (size_t a = 0; a < na; ++a)
+                        for (size_t i = 0; i < h; ++i) Sq[p][dof + a * h + i] = Sq[p][so + a * h + tbl[i]];
+                    Ss[p][dst] = Ss[p][src];
+                }
+            }
+            for (size_t a = 0; a < na; ++a) copy_branch(child(src, a), child(dst, a), k);
+        } else if (v->node_kind[src] == GTO_KIND_CHANCE) {
+            const uint32_t ss = v->node_children_start[src], ds = v->node_children_start[dst];
+            for (int x = 0; x < 52; ++x) {
+                const uint32_t dc = v->children[ds + x];
+                if (dc == kSent) continue;
+                copy_branch(v->children[ss + permute_card((uint8_t)x, perm(k))], dc, k);
+            }
+        }
     }
 
     // cfr, cfr.rs:414-534. `out` is zero on entry.
@@ -616,7 +755,14 @@ struct Engine : EngineBase {
         }
     }
 
+    void set_isomorphism(bool on) override {
+        ensure_symmetric();
+        use_iso = on;
+    }
+    void mark_sym_dirty() override { sym_dirty = true; }
+
     void iterate() override {  // cfr.rs:170-190
+        sym_dirty = true;
         iteration += 1;
         Discounts<T> d = discounts<T>(algo, iteration);
         for (int p = 0; p < 2; ++p) {
@@ -776,6 +922,7 @@ struct Engine : EngineBase {
     }
 
     void get_arenas(int which, int p, float* out) override {
+        ensure_symmetric();
         size_t len = v->data_size[p];
         if (!compressed()) {
             const std::vector<T>& a = which == 0 ? R[p] : S[p];
@@ -797,6 +944,7 @@ struct Engine : EngineBase {
         }
     }
     void get_arenas_f64(int which, int p, double* out) override {
+        ensure_symmetric();
         size_t len = v->data_size[p];
         if (!compressed()) {
             const std::vector<T>& a = which == 0 ? R[p] : S[p];
@@ -823,6 +971,7 @@ struct Engine : EngineBase {
         }
     }
     void average_strategy(uint32_t n, float* out) override {
+        ensure_symmetric();
         int pl = v->node_player[n];
         size_t cnt = (size_t)v->node_num_children[n] * nh[pl];
         std::vector<T> s(cnt);
@@ -883,7 +1032,9 @@ void oracle_set_locks(oracle* o, const int64_t* lock_off, const float* sigma, ui
     if (lock_off) o->e->lock_off.assign(lock_off, lock_off + nn);
     else o->e->lock_off.assign(nn, -1);
     o->e->lock_sigma.assign(sigma, sigma + n_sigma);
+    o->e->mark_sym_dirty();
 }
+void oracle_set_isomorphism(oracle* o, int on) { o->e->set_isomorphism(on != 0); }
 void oracle_average_strategy(oracle* o, uint32_t node, float* out) { o->e->average_strategy(node, out); }
 
 void oracle_fold_cfv(const gto_spot_view* spot, int p, const float* reach_o, float amount, float* out) {

@@ -8,8 +8,10 @@
  * quantizers of store.rs:153-215) over a spot given as a gto_spot_view.
  * Precision 32 follows the reference's f32 arithmetic operation for
  * operation (f64 running sums where the reference uses them); precision 64
- * runs every vector in f64 and is the FP64 oracle. Suit isomorphism is not
- * restated (the engine's semantics are those of use_isomorphism = false).
+ * runs every vector in f64 and is the FP64 oracle. Suit isomorphism
+ * (cfr.rs:568-660, 749-870) is on by default, as in the reference
+ * (cfr.rs:143): chance nodes traverse one card per orbit, and arenas are
+ * symmetrized before they are read.
  */
 #ifndef GTO_ORACLE_H
 #define GTO_ORACLE_H
@@ -43,6 +45,8 @@ void oracle_get_arenas_f64(oracle* o, double* r0, double* r1, double* s0, double
 void oracle_set_locks(oracle* o, const int64_t* lock_off, const float* sigma, uint64_t n_sigma);
 /* Average strategy at a node (na*nh floats), lock-aware (cfr.rs:668-726). */
 void oracle_average_strategy(oracle* o, uint32_t node, float* out);
+/* use_isomorphism (cfr.rs:84-87); on by default. */
+void oracle_set_isomorphism(oracle* o, int on);
 
 /* Terminal math in isolation (game.rs:470-577), for the unit tests. */
 void oracle_fold_cfv(const gto_spot_view* spot, int p, const float* reach_o, float amount, float* out);
