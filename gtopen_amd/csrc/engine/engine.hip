@@ -330,6 +330,9 @@ class Engine {
     // chance weight * value of child a (k_grad_sigma).
     void game_value(int p, double* U, float* gw[2], float* gs[2]) {
         if (compressed_) throw std::runtime_error("game values need an engine with f32 arenas");
+        // the strategy is an input per node; under suit isomorphism copied
+        // branches would silently take their representatives' strategies
+        if (iso_) throw std::runtime_error("game values need an engine created with isomorphism off");
         const int o = 1 - p;
         launch_root(3);
         launch_down(3, kRawF32);

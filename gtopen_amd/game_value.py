@@ -70,7 +70,8 @@ class GameValue:
 
     def __init__(self, spot, device=0):
         self.spot = spot
-        self.engine = GpuSolver(spot, storage="f32", use_graphs=False, device=device)
+        # every node's strategy is an input: no suit-orbit sharing
+        self.engine = GpuSolver(spot, storage="f32", use_graphs=False, device=device, isomorphism=False)
 
     def set_weights(self, w0=None, w1=None):
         keep = [None if w is None else np.ascontiguousarray(w, dtype=np.float32) for w in (w0, w1)]

@@ -341,7 +341,8 @@ int gto_hip_set_weights(gto_hip* h, const float* w0, const float* w1);
  *     reach_q(node,i) * chance weight of the path * value of child a; 0 at
  *     locked nodes and nodes outside the reachable tree.
  * The reference has no backward; its CFR values are the reference gradient.
- * F32 storage only. */
+ * F32 storage only, engine created with isomorphism 0 (every node's strategy
+ * is an input). */
 int gto_hip_game_value(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
                        float* grad_s0, float* grad_s1);
 

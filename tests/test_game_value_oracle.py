@@ -23,7 +23,7 @@ def _setup(rake=0.0, iters=40):
     solver.iterate(iters)
     _, strat = solver.arenas()
     sigma = behavioural_strategy(spot, strat)
-    o64 = oracle.Oracle(spot, precision=64, weights=(spot.weights[0], spot.weights[1]))
+    o64 = oracle.Oracle(spot, precision=64, weights=(spot.weights[0], spot.weights[1]), isomorphism=False)
     o64.set_arenas([np.zeros(spot.data_size[q], dtype=np.float32) for q in range(2)], sigma)
     return spot, sigma, o64
 

@@ -29,7 +29,7 @@ def _solved(cfg, iters):
     sigma = behavioural_strategy(spot, strat)
     gv = GameValue(spot)
     gv.set_strategy(*sigma)
-    o64 = oracle.Oracle(spot, precision=64, weights=(spot.weights[0], spot.weights[1]))
+    o64 = oracle.Oracle(spot, precision=64, weights=(spot.weights[0], spot.weights[1]), isomorphism=False)
     o64.set_arenas([np.zeros(spot.data_size[q], dtype=np.float32) for q in range(2)], sigma)
     return spot, sigma, gv, o64
 
