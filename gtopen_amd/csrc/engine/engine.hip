@@ -684,14 +684,36 @@ class Engine {
 
     uint64_t reached_entries() const { return plan_.reached_entries; }
 
-    static uint64_t estimate(const gto_spot_view& v) {
-        // arenas + reach slots + value rows + tables (upper bound)
-        uint64_t n = v.num_nodes;
-        uint64_t nh0 = v.num_hands[0], nh1 = v.num_hands[1], nhm = std::max(nh0, nh1);
-        uint64_t arenas = (v.data_size[0] + v.data_size[1]) * 8;
-        uint64_t reach = n * (nh0 + nh1) * 4;
-        uint64_t cfv = n * nhm * 4;
-        return arenas + reach + cfv + n * 64 + (256ull << 20);
+    // An upper bound of the device memory an engine allocates for `v` with
+    // DCFR / CFR+ (PCFR+ adds 4 bytes per arena entry), the exact showdown,
+    // counted buffer by buffer over every node (reached or not).
+    static uint64_t estimate(const gto_spot_view& v, int storage) {
+        const uint64_t n = v.num_nodes, nh[2] = {v.num_hands[0], v.num_hands[1]};
+        const uint64_t nhm = std::max(nh[0], nh[1]);
+        // reach slots: the root, every action child for its actor, every
+        // chance child for both players
+        uint64_t slots[2] = {1, 1};
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t cs = v.node_children_start[i];
+            if (v.node_kind[i] == GTO_KIND_ACTION) {
+                slots[v.node_player[i]] += v.node_num_children[i];
+            } else if (v.node_kind[i] == GTO_KIND_CHANCE) {
+                uint64_t c = 0;
+                for (int x = 0; x < 52; ++x) c += v.children[cs + x] != 0xFFFFFFFFu;
+                slots[0] += c;
+                slots[1] += c;
+            }
+        }
+        const uint64_t D = v.data_size[0] + v.data_size[1];
+        uint64_t arenas = storage == GTO_STORAGE_F32 ? D * 8 : D * 4 + n * 16;
+        if (storage == GTO_STORAGE_HALF) arenas += n * 16;                 // per-node maxima
+        uint64_t reach = 0;
+        for (int q = 0; q < 2; ++q) reach += slots[q] * (nh[q] * 4 + 9);    // row, alive, mass, max
+        const uint64_t cfv = n * (nhm * 4 + 4);                               // value rows and their maxima
+        const uint64_t records = n * 96;                                      // work records, child lists
+        const uint64_t keys = v.num_river_keys;
+        const uint64_t boards = keys * ((nh[0] + nh[1]) * 30 + 2 * 52 * 2 * 48 + 2 * 53 * 4);  // showdown tables
+        return arenas + reach + cfv + records + boards + (64ull << 20);
     }
 
    private:
@@ -1675,7 +1697,7 @@ int gto_hip_enable_timing(gto_hip*, int) { return GTO_OK; }
 
 uint64_t gto_hip_reached_entries(const gto_hip* h) { return h->e->reached_entries(); }
 
-uint64_t gto_hip_estimate_vram(const gto_spot_view* spot, int) { return Engine::estimate(*spot); }
+uint64_t gto_hip_estimate_vram(const gto_spot_view* spot, int storage) { return spot ? Engine::estimate(*spot, storage) : 0; }
 
 int gto_hip_vram_info(int device, uint64_t* free_mb, uint64_t* total_mb) {
     return guarded([&] {
