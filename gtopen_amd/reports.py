@@ -99,16 +99,30 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
     waves = 0
     schedules = []
     tune_cost = L.SCHEDULE_CANDIDATES * (autotune + 1)
+    needs = []
+    pending = None
     while k < len(flops):
-        # fill a wave up to the memory budget (at least one flop)
+        # waves of even size: the memory budget sets how many waves the
+        # remaining flops need (from the mean estimate so far), each wave
+        # takes its share, and the budget is never exceeded (at least one
+        # flop per wave); a short last wave would leave the device idle
+        if pending is None:
+            pending = (flops[k][0], flops[k][1], Spot(flops[k][0]))
+            needs.append(int(L.lib().gto_hip_estimate_vram(pending[2].view_ptr, st)))
+        remaining = len(flops) - k
+        mean = sum(needs) / len(needs)
+        n_waves = max(1, int(np.ceil(mean * remaining / budget)), int(np.ceil(remaining / max_batch)))
+        cap = int(np.ceil(remaining / n_waves))
         wave, used = [], 0
-        while k < len(flops) and len(wave) < max_batch:
-            cfg, weight = flops[k]
-            spot = Spot(cfg)
-            need = int(L.lib().gto_hip_estimate_vram(spot.view_ptr, st))
+        while k < len(flops) and len(wave) < cap:
+            if pending is None:
+                pending = (flops[k][0], flops[k][1], Spot(flops[k][0]))
+                needs.append(int(L.lib().gto_hip_estimate_vram(pending[2].view_ptr, st)))
+            need = needs[-1]
             if wave and used + need > budget:
                 break
-            wave.append((cfg, weight, spot))
+            wave.append(pending)
+            pending = None
             used += need
             k += 1
         waves += 1
