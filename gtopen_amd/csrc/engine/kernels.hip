@@ -437,6 +437,21 @@ __device__ __forceinline__ void dispatch_na(int na, F&& f) {
     }
 }
 
+// The same for the up sweep's kernels with at most 4 actions specialised:
+// those kernels hold two hands' values per lane, and the register demand of
+// the wider cases would lower every launch's occupancy; nodes with more
+// actions take the generic loop.
+template <class F>
+__device__ __forceinline__ void dispatch_na4(int na, F&& f) {
+    switch (na) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        default: f(std::integral_constant<int, 0>{}); break;
+    }
+}
+
 template <int KIND>
 struct ArenaOf;
 template <>
@@ -1506,7 +1521,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
         // opponent node: sum; locked node: lock-weighted; best response: max
         const float* sig = r.lock_off >= 0 ? T.lock_sigma + r.lock_off : nullptr;
         const int kind = q != p ? 0 : (sig ? 1 : 2);
-        dispatch_na(na, [&](auto NAc) {
+        dispatch_na4(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
             if constexpr (NA > 0) {
                 const float* rw[NA];
@@ -1565,7 +1580,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                                  ? (STORE == kStoreQ ? kAverageQ : STORE == kStoreH ? kAverageH : kAverageF32)
                                  : (STORE == kStoreQ ? kCurrentQ : STORE == kStoreH ? kCurrentH : kCurrentF32);
         const auto* arena = ArenaOf<KIND>::get(S, p) + off;
-        dispatch_na(na, [&](auto NAc) {
+        dispatch_na4(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
             if constexpr (NA > 0) {
                 for (uint32_t i = lane; i < nhp; i += kWave) {
@@ -1601,7 +1616,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             float* __restrict__ R = S.R[p] + off;
             float* __restrict__ Sm = S.S[p] + off;
             float* __restrict__ Qp = PRED ? S.Q[p] + off : nullptr;
-            dispatch_na(na, [&](auto NAc) {
+            dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
                     const float* rw[NA];
@@ -1697,7 +1712,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             const float kS = half_scale(S.Sm[p][r.node] * ds + mr);
             const float iR = 1.0f / kR, iS = 1.0f / kS;
             float xr = 0.0f, xs = 0.0f;
-            dispatch_na(na, [&](auto NAc) {
+            dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
                     const float* rw[NA];
@@ -1786,7 +1801,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             float rmax = 0.0f, smax = 0.0f;
             float RM = 0.0f, SM = 0.0f, rk = 0.0f, sk = 0.0f;
             bool rok = false, sok = false;
-            dispatch_na(na, [&](auto NAc) {
+            dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 auto hand = [&](uint32_t i, auto&& sink) -> float {
                     if constexpr (NA > 0) {

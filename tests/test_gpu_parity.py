@@ -204,6 +204,39 @@ def test_cfr_plus_matches_oracle(built):
         assert abs(eg - ec) <= POT_TOL * pot
 
 
+def test_wide_action_menus_match_oracle(built):
+    """Nodes with 5 to 8 actions (the update kernels' generic path): a river
+    spot with six bet sizes and two raise sizes, f32 and f16 arenas, against
+    the oracle (f16: against FP64, within the f16 engine's tolerance)."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, SpotConfig, StreetSizing, TreeConfig, workloads, _lib as L
+
+    base = workloads.river_spot()
+    s = [StreetSizing(), StreetSizing(), StreetSizing.of("25 33 50 75 125 a", "2.5x 4x")]
+    cfg = SpotConfig(base.board, base.range_oop, base.range_ip,
+                     TreeConfig(starting_pot=20.0, effective_stack=90.0, oop=s, ip=s, max_raises=1))
+    spot = Spot(cfg)
+    na = np.asarray(spot.num_children)[np.asarray(spot.kind) == L.KIND_ACTION]
+    _say(f"case: river spot, action counts {np.bincount(na).tolist()}, 40 iterations")
+    assert na.max() >= 7
+    gpu, cpu = GpuSolver(spot), oracle.Oracle(spot)
+    pot = spot.starting_pot
+    for k in range(4):
+        gpu.iterate(10)
+        cpu.iterate(10)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {10 * (k + 1)}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) <= POT_TOL * pot
+    gpu.close()
+    gh, o64 = GpuSolver(spot, storage="half"), oracle.Oracle(spot, precision=64)
+    gh.iterate(40)
+    o64.iterate(40)
+    eh, e64 = gh.exploitability(), o64.exploitability()
+    _say(f"  f16 arenas, iter 40: gpu {eh:.6f} fp64 {e64:.6f}")
+    assert abs(eh - e64) <= 1e-3 * pot
+    gh.close()
+
+
 def test_pcfr_plus_matches_oracle(built):
     """PCFR+ (cfr.rs:250-289, 337-348: CFR+ discounts, the current strategy
     matched on max(R,0) plus the last instantaneous regret) on the GPU
