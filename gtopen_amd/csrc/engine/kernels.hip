@@ -756,15 +756,15 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     if (w >= count) return;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
-    // packed hand words (cards, identical combo) of both players: static
+    // packed hand words (cards, identical combo) of both players: static;
+    // the opponent's now, the traverser's once the sums are done (the two
+    // sets are never live together)
     const uint32_t* __restrict__ hpp = T.hpack[p];
     const uint32_t* __restrict__ hpo = T.hpack[o];
-    uint32_t hw[kMaxH], ho[kMaxH];
+    uint32_t ho[kMaxH];
 #pragma unroll
-    for (int u = 0; u < kMaxH; ++u) {
-        if (u * kWave < (int)nhp) hw[u] = hpp[min(lane + u * kWave, nhp - 1)];
+    for (int u = 0; u < kMaxH; ++u)
         if (u * kWave < (int)nho) ho[u] = hpo[min(lane + u * kWave, nho - 1)];
-    }
     const FoldRec r = recs[w];
     float* out = S.cfv + (size_t)r.cfv * T.nh_max;
     const uint32_t oslot = pick(r.own, o);
@@ -816,6 +816,10 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         }
         s_card[c] = acc;
     }
+    uint32_t hw[kMaxH];
+#pragma unroll
+    for (int u = 0; u < kMaxH; ++u)
+        if (u * kWave < (int)nhp) hw[u] = hpp[min(lane + u * kWave, nhp - 1)];
     wave_sync();
     // swap: the opponent's payoff (win and lose exchanged), for game-value
     // gradients with respect to the opponent's range and strategy
