@@ -17,6 +17,7 @@ the end, with "complete" false until every flop is in.
 import json
 import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -101,13 +102,26 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
     tune_cost = L.SCHEDULE_CANDIDATES * (autotune + 1)
     needs = []
     pending = None
+    # spots are built ahead on host threads (the C++ builder runs outside the
+    # interpreter lock), at most `window` beyond the flop being placed, so
+    # building overlaps the device's iterations of the current wave
+    pool = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    window = max_batch
+    built = {}
+
+    def spot_of(j):
+        for t in range(j, min(len(flops), j + window)):
+            if t not in built:
+                built[t] = pool.submit(Spot, flops[t][0])
+        return built.pop(j).result()
+
     while k < len(flops):
         # waves of even size: the memory budget sets how many waves the
         # remaining flops need (from the mean estimate so far), each wave
         # takes its share, and the budget is never exceeded (at least one
         # flop per wave); a short last wave would leave the device idle
         if pending is None:
-            pending = (flops[k][0], flops[k][1], Spot(flops[k][0]))
+            pending = (flops[k][0], flops[k][1], spot_of(k))
             needs.append(int(L.lib().gto_hip_estimate_vram(pending[2].view_ptr, st)))
         remaining = len(flops) - k
         mean = sum(needs) / len(needs)
@@ -116,7 +130,7 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
         wave, used = [], 0
         while k < len(flops) and len(wave) < cap:
             if pending is None:
-                pending = (flops[k][0], flops[k][1], Spot(flops[k][0]))
+                pending = (flops[k][0], flops[k][1], spot_of(k))
                 needs.append(int(L.lib().gto_hip_estimate_vram(pending[2].view_ptr, st)))
             need = needs[-1]
             if wave and used + need > budget:
@@ -127,7 +141,9 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
             k += 1
         waves += 1
         tw = time.time()
-        gpus = [GpuSolver(s, storage=storage, device=device) for _, _, s in wave]
+        # the wave's engines are created on the same threads (plan building
+        # and uploads of independent spots)
+        gpus = list(pool.map(lambda w: GpuSolver(w[2], storage=storage, device=device), wave))
         active = list(range(len(wave)))
         batch = None
         tuned = False
@@ -163,6 +179,7 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
                     gpus[i].close()
                 active = [i for i in active if i not in done]
         del gpus
+    pool.shutdown()
     elapsed = time.time() - t0
     meta.update(seconds=elapsed, waves=waves, flops_per_hour=len(rows) / elapsed * 3600.0 if elapsed > 0 else 0.0,
                 schedules=schedules)
