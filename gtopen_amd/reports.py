@@ -159,11 +159,15 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
             batch.iterate(step)
             it += step
             done = []
-            for i in active:
+            # the engines' checks run concurrently (each engine has its own
+            # stream; a small tree's pass alone leaves most of the device idle)
+            pcts = list(pool.map(lambda i: gpus[i].exploitability() / wave[i][2].starting_pot * 100.0, active))
+            fin = [i for i, pct in zip(active, pcts) if pct <= target or it >= max_iterations]
+            stats = dict(zip(fin, pool.map(lambda i: _root_stats(gpus[i], wave[i][2]), fin)))
+            for i, pct in zip(active, pcts):
                 cfg, weight, spot = wave[i]
-                pct = gpus[i].exploitability() / spot.starting_pot * 100.0
-                if pct <= target or it >= max_iterations:
-                    players, root, vs_check = _root_stats(gpus[i], spot)
+                if i in stats:
+                    players, root, vs_check = stats[i]
                     rows.append({"board": cfg.board, "weight": weight, "iterations": it, "exploit_pct": pct,
                                  "seconds": time.time() - tw, "nodes": spot.num_nodes, "players": players,
                                  "root": root, "vs_check": vs_check})
