@@ -52,16 +52,19 @@ struct DevLevel {
     uint32_t n_edge = 0;
 };
 
-// Workgroups for a list of `count` one-wavefront items, padded to a
-// multiple of the 8 XCDs (see xcd_slice).
 // IEEE half <-> float on the host (round to nearest even)
 inline float half_to_float(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
 inline uint16_t float_to_half(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
+// Workgroups for a list of `count` one-wavefront items, padded to a
+// multiple of the 8 XCDs (see xcd_slice).
 uint32_t wave_grid(uint32_t count, uint32_t waves = kWaves) {
     uint32_t blocks = (count + waves - 1) / waves;
     return (blocks + 7u) & ~7u;
 }
+
+// wavefronts of the fold kernel for n nodes (kFoldNodes per wavefront)
+uint32_t fold_waves(uint32_t n) { return (n + kFoldNodes - 1) / kFoldNodes; }
 
 // Discounts::for_iteration (cfr.rs:44-63), computed in f64 and stored as f32.
 void discounts(int algo, uint32_t it, float* out3) {
@@ -1024,8 +1027,9 @@ class Engine {
         size_t lds_max = std::max(std::max(show_lds_[0], show_lds_[1]), std::max(fold_lds_[0], fold_lds_[1]));
         if (lds_max > kLdsBudget)
             throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
-        HIP_OK(hipFuncSetAttribute((const void*)k_up_show, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)std::max(show_lds_[0], show_lds_[1])));
+        for (const void* f : {(const void*)k_up_show<3>, (const void*)k_up_show<kRowPer>})
+            HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)std::max(show_lds_[0], show_lds_[1])));
         HIP_OK(hipFuncSetAttribute((const void*)k_up_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)std::max(fold_lds_[0], fold_lds_[1])));
         n_fold_all_ = (uint32_t)plan_.fold_all.size();
@@ -1215,6 +1219,10 @@ class Engine {
     // precision form serves the solve's own sweeps (0, 3); best response,
     // average-strategy values and the game value always take the exact
     // showdown, so exploitability judges every form with the same arithmetic.
+    // the exact showdown kernel sized to the opponent's hand count (3 rows
+    // per thread cover up to 768 hands)
+    auto show_kernel(int p) const { return plan_.nh[1 - p] <= 3u * kBlock ? k_up_show<3> : k_up_show<kRowPer>; }
+
     void launch_up(int p, int mode, int swap = 0) {
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
         if (mma)
@@ -1224,10 +1232,10 @@ class Engine {
         if (!sched_.per_level) {
             // terminals of every level first: they read only the down pass
             if (n_fold_all_)
-                launch(PK_UP_FOLD, k_up_fold, wave_grid(n_fold_all_), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
+                launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
                        n_fold_all_, p, swap);
             if (!mma && n_show_chunks_[cap_])
-                launch(PK_UP_SHOW, k_up_show, wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
+                launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
                        (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_], p, swap);
         }
         for (size_t l = lv_.size(); l-- > 0;) {
@@ -1235,10 +1243,10 @@ class Engine {
             if (sched_.per_level) {
                 // this level's terminals, just before the kernels that read them
                 if (D.n_fold)
-                    launch(PK_UP_FOLD, k_up_fold, wave_grid(D.n_fold), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
+                    launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(D.n_fold)), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
                            D.n_fold, p, swap);
                 if (!mma && D.n_show_chunks[cap_])
-                    launch(PK_UP_SHOW, k_up_show, wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
+                    launch(PK_UP_SHOW, show_kernel(p), wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
                            (const ShowRec*)D.show, (const uint4*)D.show_chunks[cap_], D.n_show_chunks[cap_], p, swap);
             }
             if (D.n_chance)

@@ -502,9 +502,6 @@ __device__ __forceinline__ uint32_t wave_item() {
 // Hands per lane whose loads are issued together in the hand loops (a
 // load's first use otherwise waits for it before the next one is issued).
 constexpr int kBatch = 4;
-// Terminal nodes per wavefront in the fold and showdown kernels: static
-// per-player / per-board tables stay in registers across the run.
-constexpr uint32_t kNodesPerWave = 8;
 
 // Orders this wavefront's LDS writes before its later LDS reads.
 __device__ __forceinline__ void wave_sync() {
@@ -734,15 +731,21 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
 // ---------------------------------------------------------------------------
 // Up sweep, fold terminals (game.rs:470-496), all levels in one launch:
 // value = amount * (opponent reach compatible with the hand), with f64
-// totals. One wavefront per node: the opponent's row is scattered into a
-// combo-indexed LDS row (zero where the opponent has no such hand), one
-// lane per card sums the card's 51 partner combos in combo order (the
-// CPU's order, game.rs:478-485), and each hand reads its two card sums and
-// its own combo. LDS per wavefront: 52 doubles + 1328 floats.
+// totals. One wavefront takes kFoldNodes consecutive nodes. The opponent's
+// row is scattered into a symmetric 52 x 52 card matrix in LDS, M[a][b] =
+// M[b][a] = reach of the hand {a, b} (0 where the opponent has no such
+// hand, and on the diagonal; those zeros are written once per wavefront,
+// since every node writes the same hand set). Card c's sum is row c read
+// in column order, i.e. its hands in ascending order of the other card,
+// which is hand (combo) order: the order the CPU accumulates them in
+// (game.rs:478-485). Rows are 52 floats, so lane c's 16-byte reads of row
+// c fall in distinct banks. Each hand then reads its two card sums and
+// its own combo's entry. LDS per wavefront: 52 x 52 floats + 52 doubles.
 // ---------------------------------------------------------------------------
 constexpr int kMaxH = 21;         // hand rows per lane: 21 * 64 >= 1326
 constexpr int kRowPer = (1326 + kBlock - 1) / kBlock;  // hand rows per thread of a workgroup
-constexpr size_t kFoldWaveBytes = 52 * 8 + 1328 * 4;  // per-card sums + combo-indexed row
+constexpr uint32_t kFoldNodes = 4;                      // fold nodes per wavefront
+constexpr size_t kFoldWaveBytes = 52 * 52 * 4 + 52 * 8;  // card matrix + per-card sums
 
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
 __device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return hi * (hi - 1) / 2 + lo; }
@@ -752,95 +755,113 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     extern __shared__ double lds[];
     const int o = 1 - p;
     const uint32_t nho = T.nh[o], nhp = T.nh[p];
-    const uint32_t w = wave_item();
-    if (w >= count) return;
+    const uint32_t first = wave_item() * kFoldNodes;
+    if (first >= count) return;
+    const uint32_t last = min(first + kFoldNodes, count);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
-    // packed hand words (cards, identical combo) of both players: static;
-    // the opponent's now, the traverser's once the sums are done (the two
-    // sets are never live together)
+    char* base = (char*)lds + wave * kFoldWaveBytes;
+    float* M = (float*)base;
+    double* s_card = (double*)(base + 52 * 52 * 4);
+    // packed hand words (c1 | c2 << 8 | ...) of both players: static for the
+    // wavefront's nodes; the opponent's as the two matrix positions of the
+    // hand (a * 52 + b | (b * 52 + a) << 16)
     const uint32_t* __restrict__ hpp = T.hpack[p];
     const uint32_t* __restrict__ hpo = T.hpack[o];
-    uint32_t ho[kMaxH];
+    uint32_t mo_pos[kMaxH], hw[kMaxH];
 #pragma unroll
     for (int u = 0; u < kMaxH; ++u)
-        if (u * kWave < (int)nho) ho[u] = hpo[min(lane + u * kWave, nho - 1)];
-    const FoldRec r = recs[w];
-    float* out = S.cfv + (size_t)r.cfv * T.nh_max;
-    const uint32_t oslot = pick(r.own, o);
-    if (!S.alive[o][oslot]) {
-        for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
-        if (S.vmax && lane == 0) S.vmax[r.cfv] = 0.0f;
-        return;
-    }
-    const float* __restrict__ ro = S.reach[o] + (size_t)oslot * nho;
-    float rh[kMaxH];
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u)
-        if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
-    // LDS per wavefront: the opponent row indexed by combo (0 where the
-    // opponent has no such hand) and the 52 per-card sums
-    const size_t per_wave = kFoldWaveBytes;
-    char* base = (char*)lds + wave * per_wave;
-    double* s_card = (double*)base;
-    float* rsc = (float*)(base + 52 * 8);
-    for (uint32_t k = lane; k < 1326; k += kWave) rsc[k] = 0.0f;
-    wave_sync();
-    double part = 0.0;
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u)
-        if (lane + u * kWave < nho) {
-            const uint32_t h = ho[u];
-            rsc[combo_of(h & 0xFFu, (h >> 8) & 0xFFu)] = rh[u];
-            part += (double)rh[u];
+        if (u * kWave < (int)nho) {
+            const uint32_t h = hpo[min(lane + u * kWave, nho - 1)];
+            const uint32_t a = h & 0xFFu, b = (h >> 8) & 0xFFu;
+            mo_pos[u] = (a * 52 + b) | ((b * 52 + a) << 16);
         }
-    const double tot = wave_sum_f64(part);
-    wave_sync();
-    // card c's hands in hand (combo) order: (c, x) for x < c, then (x, c)
-    // for x > c -- the order the CPU accumulates them in (game.rs:478-485);
-    // absent combos contribute +0.0, which leaves an f64 sum unchanged
-    if (lane < 52) {
-        const uint32_t c = lane;
-        double acc = 0.0;
-#pragma unroll
-        for (int t0 = 0; t0 < 56; t0 += 8) {
-            float v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const uint32_t t = min((uint32_t)(t0 + u), 50u);
-                const uint32_t x = t < c ? t : t + 1;
-                v[u] = rsc[x < c ? c * (c - 1) / 2 + x : x * (x - 1) / 2 + c];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) acc += (t0 + u) < 51 ? (double)v[u] : 0.0;
-        }
-        s_card[c] = acc;
-    }
-    uint32_t hw[kMaxH];
 #pragma unroll
     for (int u = 0; u < kMaxH; ++u)
         if (u * kWave < (int)nhp) hw[u] = hpp[min(lane + u * kWave, nhp - 1)];
-    wave_sync();
-    // swap: the opponent's payoff (win and lose exchanged), for game-value
-    // gradients with respect to the opponent's range and strategy
-    const float amount = (r.folder == (uint32_t)p) != (swap != 0) ? r.t_lose : r.t_win;
-    float vm = 0.0f;
-#pragma unroll
-    for (int u = 0; u < kMaxH; ++u) {
-        const uint32_t i = lane + u * kWave;
-        if (u * kWave >= (int)nhp) break;
-        if (i < nhp) {
-            const uint32_t h = hw[u];
-            const uint32_t a1 = h & 0xFFu, a2 = (h >> 8) & 0xFFu;
-            const float same_r = rsc[combo_of(a1, a2)];
-            const float valid = (float)(tot - s_card[a1] - s_card[a2]) + same_r;
-            out[i] = amount * valid;
-            vm = fmaxf(vm, fabsf(amount * valid));
-        }
+    {
+        float4* m4 = (float4*)M;
+        for (uint32_t k = lane; k < 52 * 52 / 4; k += kWave) m4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    if (S.vmax) {
-        vm = wave_max_f32(vm);
-        if (lane == 0) S.vmax[r.cfv] = vm;
+    float rh[kMaxH];
+    {
+        const float* __restrict__ ro = S.reach[o] + (size_t)pick(recs[first].own, o) * nho;
+#pragma unroll
+        for (int u = 0; u < kMaxH; ++u)
+            if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
+    }
+#pragma unroll 1
+    for (uint32_t n = first; n < last; ++n) {
+        const FoldRec r = recs[n];
+        float* out = S.cfv + (size_t)r.cfv * T.nh_max;
+        const bool live = S.alive[o][pick(r.own, o)] != 0;
+        double part = 0.0;
+        if (live) {
+            wave_sync();  // the previous node's reads of M and s_card are done
+#pragma unroll
+            for (int u = 0; u < kMaxH; ++u)
+                if (lane + u * kWave < nho) {
+                    M[mo_pos[u] & 0xFFFFu] = rh[u];
+                    M[mo_pos[u] >> 16] = rh[u];
+                    part += (double)rh[u];
+                }
+        }
+        // the next node's row, in flight during the rest of this node
+        if (n + 1 < last) {
+            const float* __restrict__ ro = S.reach[o] + (size_t)pick(recs[n + 1].own, o) * nho;
+#pragma unroll
+            for (int u = 0; u < kMaxH; ++u)
+                if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
+        }
+        if (!live) {
+            for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
+            if (S.vmax && lane == 0) S.vmax[r.cfv] = 0.0f;
+            continue;
+        }
+        const double tot = wave_sum_f64(part);
+        wave_sync();
+        if (lane < 52) {
+            const float4* row = (const float4*)(M + lane * 52);
+            double acc = 0.0;
+#pragma unroll
+            for (int t0 = 0; t0 < 13; t0 += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (t0 + t < 13) v[t] = row[t0 + t];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (t0 + t < 13) {
+                        acc += (double)v[t].x;
+                        acc += (double)v[t].y;
+                        acc += (double)v[t].z;
+                        acc += (double)v[t].w;
+                    }
+            }
+            s_card[lane] = acc;
+        }
+        wave_sync();
+        // swap: the opponent's payoff (win and lose exchanged), for game-value
+        // gradients with respect to the opponent's range and strategy
+        const float amount = (r.folder == (uint32_t)p) != (swap != 0) ? r.t_lose : r.t_win;
+        float vm = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kMaxH; ++u) {
+            const uint32_t i = lane + u * kWave;
+            if (u * kWave >= (int)nhp) break;
+            if (i < nhp) {
+                const uint32_t h = hw[u];
+                const uint32_t a1 = h & 0xFFu, a2 = (h >> 8) & 0xFFu;
+                const float same_r = M[a1 * 52 + a2];
+                const float valid = (float)(tot - s_card[a1] - s_card[a2]) + same_r;
+                out[i] = amount * valid;
+                vm = fmaxf(vm, fabsf(amount * valid));
+            }
+        }
+        if (S.vmax) {
+            vm = wave_max_f32(vm);
+            if (lane == 0) S.vmax[r.cfv] = vm;
+        }
     }
 }
 
@@ -862,6 +883,9 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 //      V = total - card totals + identical combo.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
+// kR: opponent hand rows per thread, kR * kBlock >= the opponent's hand count
+// (also the strength-order chunk per thread in phase 2)
+template <int kR>
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
                                                     int swap) {
@@ -913,11 +937,11 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     }
     __syncthreads();
     // reach row of the first node
-    float rh[kRowPer];
+    float rh[kR];
     {
         const float* __restrict__ ro = S.reach[o] + (size_t)nodes[0].oslot * nho;
 #pragma unroll
-        for (int u = 0; u < kRowPer; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
+        for (int u = 0; u < kR; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
     }
     // per-card segment of this thread (cards 0..51, four lanes each)
     const uint32_t card = tid >> 2, seg = tid & 3;
@@ -940,14 +964,14 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         const bool live = nd.live != 0;
         if (live) {
 #pragma unroll
-            for (int u = 0; u < kRowPer; ++u)
+            for (int u = 0; u < kR; ++u)
                 if (tid + u * kBlock < nho) rhs[tid + u * kBlock] = rh[u];
         }
         // 1. the next node's reach row, in flight during this node
         if (n + 1 < cnt) {
             const float* __restrict__ ro = S.reach[o] + (size_t)nodes[n + 1].oslot * nho;
 #pragma unroll
-            for (int u = 0; u < kRowPer; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
+            for (int u = 0; u < kR; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
         }
         if (!live) {  // no reach, or no opponent hand on this board: all 0
             for (uint32_t i = tid; i < nhp; i += kBlock) out[i] = 0.0f;
@@ -957,15 +981,15 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         __syncthreads();
         // 2. strength order (also scattered into the per-card arrays) and the
         //    chunk sums of the exclusive prefix
-        float v[kRowPer];
+        float v[kR];
         double part = 0.0;
 #pragma unroll
-        for (int u = 0; u < kRowPer; ++u) {
+        for (int u = 0; u < kR; ++u) {
             const uint32_t k = min(lo + u, mo - 1);
             v[u] = rhs[oidx[k]];
         }
 #pragma unroll
-        for (int u = 0; u < kRowPer; ++u) {
+        for (int u = 0; u < kR; ++u) {
             const uint32_t k = lo + u;
             if (k < hi) {
                 const uint32_t ks = kslot[k];
@@ -993,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         }
         double run = before + excl;
 #pragma unroll
-        for (int u = 0; u < kRowPer; ++u) {
+        for (int u = 0; u < kR; ++u) {
             const uint32_t k = lo + u;
             if (k < hi) {
                 P[k] = run;
