@@ -76,7 +76,7 @@ struct DevTree {
 // order; the hand-order reach row is staged in the per-card region first)
 // are rebuilt per node.
 struct ShowLds {
-    uint32_t ent, P, cpre, wsum, rs, rcard, oidx, kslot, coff, nodes, bytes;
+    uint32_t ent, P, cpre, wsum, rs, rcard, okey, coff, nodes, bytes;
 };
 // per node of a run, staged once: value row, opponent reach slot, live
 // flag, win, lose, tie
@@ -93,9 +93,8 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, ui
     L.wsum = L.cpre + 8 * creg;                 // 4 doubles (per-wave scan totals)
     L.rs = L.wsum + 32;                         // mo floats: reach in strength order
     L.rcard = L.rs + 4 * mo;                    // 2 mo floats: reach in per-card list order
-    L.oidx = L.rcard + 8 * mo;                  // mo u16
-    L.kslot = L.oidx + ((2 * mo + 3) & ~3u);    // 2 mo u16
-    L.coff = L.kslot + ((4 * mo + 3) & ~3u);    // 53 u32
+    L.okey = (L.rcard + 8 * mo + 7) & ~7u;      // mo uint2: hand index, per-card slots
+    L.coff = L.okey + 8 * mo;                   // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
     L.bytes = (L.nodes + 16 * sizeof(ShowNode) + 15) & ~15u;
     return L;

@@ -510,14 +510,55 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ double wave_sum_f64(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
+// Cross-lane moves on the VALU (DPP) rather than through the LDS crossbar
+// (ds_bpermute), which the showdown and fold kernels keep busy. Controls:
+// row_shr:n = 0x110 + n (within 16-lane rows), row_bcast:15 = 0x142 and
+// row_bcast:31 = 0x143 (the last lane of a row, or of lanes 0-31, into the
+// rows that ROWS enables), wave_shr:1 = 0x138, quad_perm = 2 bits per lane.
+// dpp_keep: lanes without a source (or in rows not enabled) keep x.
+// dpp_zero: they get 0.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_keep(float x) {
+    const int u = __builtin_bit_cast(int, x);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(u, u, CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_zero(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ float wave_max_f32(float v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-    return v;
+// Inclusive prefix sum over the wavefront's 64 lanes (all lanes active).
+__device__ __forceinline__ double wave_incl_sum_f64(double x) {
+    x += dpp_zero<0x111>(x);
+    x += dpp_zero<0x112>(x);
+    x += dpp_zero<0x114>(x);
+    x += dpp_zero<0x118>(x);
+    x += dpp_zero<0x142, 0xA>(x);
+    x += dpp_zero<0x143, 0xC>(x);
+    return x;
+}
+
+// Sum over the wavefront (all lanes active), the same in every lane.
+__device__ __forceinline__ double wave_sum_f64(double v) { return readlane_f64(wave_incl_sum_f64(v), kWave - 1); }
+
+// Max over the wavefront (all lanes active), the same in every lane.
+__device__ __forceinline__ float wave_max_f32(float x) {
+    x = fmaxf(x, dpp_keep<0x111>(x));
+    x = fmaxf(x, dpp_keep<0x112>(x));
+    x = fmaxf(x, dpp_keep<0x114>(x));
+    x = fmaxf(x, dpp_keep<0x118>(x));
+    x = fmaxf(x, dpp_keep<0x142, 0xA>(x));
+    x = fmaxf(x, dpp_keep<0x143, 0xC>(x));
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), kWave - 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -744,7 +785,7 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
 // ---------------------------------------------------------------------------
 constexpr int kMaxH = 21;         // hand rows per lane: 21 * 64 >= 1326
 constexpr int kRowPer = (1326 + kBlock - 1) / kBlock;  // hand rows per thread of a workgroup
-constexpr uint32_t kFoldNodes = 4;                      // fold nodes per wavefront
+constexpr uint32_t kFoldNodes = 8;                      // fold nodes per wavefront
 constexpr size_t kFoldWaveBytes = 52 * 52 * 4 + 52 * 8;  // card matrix + per-card sums
 
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
@@ -907,8 +948,9 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     float* rs = (float*)(base + L.rs);
     float* rcard = (float*)(base + L.rcard);
     float* rhs = (float*)cpre;
-    uint16_t* oidx = (uint16_t*)(base + L.oidx);
-    uint32_t* kslot = (uint32_t*)(base + L.kslot);  // two u16 per sorted hand
+    // per sorted opponent hand: its hand index and its two slots in the
+    // per-card arrays (x: index, y: slot | slot << 16)
+    uint2* okey = (uint2*)(base + L.okey);
     uint32_t* coff = (uint32_t*)(base + L.coff);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
 
@@ -930,9 +972,8 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         for (uint32_t i = tid; i < nhp; i += kBlock) ent[i] = tab[i];
         const uint32_t so = T.sd_off[o][b];
         const uint16_t* __restrict__ gi = T.sd_idx[o] + so;
-        for (uint32_t k = tid; k < mo; k += kBlock) oidx[k] = gi[k];
         const uint32_t* __restrict__ gk = (const uint32_t*)(T.sd_kslot[o] + 2 * (size_t)so);
-        for (uint32_t k = tid; k < mo; k += kBlock) kslot[k] = gk[k];
+        for (uint32_t k = tid; k < mo; k += kBlock) okey[k] = make_uint2(gi[k], gk[k]);
         if (tid < 53) coff[tid] = T.sd_coff[o][(size_t)b * 53 + tid];
     }
     __syncthreads();
@@ -982,30 +1023,26 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         // 2. strength order (also scattered into the per-card arrays) and the
         //    chunk sums of the exclusive prefix
         float v[kR];
+        uint32_t ks[kR];
         double part = 0.0;
 #pragma unroll
         for (int u = 0; u < kR; ++u) {
-            const uint32_t k = min(lo + u, mo - 1);
-            v[u] = rhs[oidx[k]];
+            const uint2 e = okey[min(lo + u, mo - 1)];
+            ks[u] = e.y;
+            v[u] = rhs[e.x];
         }
 #pragma unroll
         for (int u = 0; u < kR; ++u) {
             const uint32_t k = lo + u;
             if (k < hi) {
-                const uint32_t ks = kslot[k];
                 rs[k] = v[u];
-                rcard[ks & 0xFFFFu] = v[u];
-                rcard[ks >> 16] = v[u];
+                rcard[ks[u] & 0xFFFFu] = v[u];
+                rcard[ks[u] >> 16] = v[u];
                 part += (double)v[u];
             }
         }
-        double incl = part;
-        for (int off = 1; off < kWave; off <<= 1) {
-            const double y = __shfl_up(incl, off);
-            if ((int)lane >= off) incl += y;
-        }
-        double excl = __shfl_up(incl, 1);
-        if (lane == 0) excl = 0.0;
+        const double incl = wave_incl_sum_f64(part);
+        const double excl = dpp_zero<0x138>(incl);  // the previous lane's; 0 in lane 0
         if (lane == kWave - 1) wsum[wave] = incl;
         __syncthreads();
         double before = 0.0, total = 0.0;
@@ -1033,9 +1070,9 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             double a = 0.0;
 #pragma unroll
             for (int u = 0; u < (int)kSeg; ++u) a += (uint32_t)u < sn ? (double)x[u] : 0.0;
-            // exclusive scan over the card's four segments (lanes 4*card..+3)
-            const uint32_t l0 = lane & ~3u;
-            const double a0 = __shfl(a, l0), a1 = __shfl(a, l0 + 1), a2 = __shfl(a, l0 + 2), a3 = __shfl(a, l0 + 3);
+            // exclusive scan over the card's four segments (lanes 4*card..+3,
+            // one quad: quad_perm broadcasts each lane's sum to the quad)
+            const double a0 = dpp_zero<0x00>(a), a1 = dpp_zero<0x55>(a), a2 = dpp_zero<0xAA>(a), a3 = dpp_zero<0xFF>(a);
             double y = 0.0;
             if (seg > 0) y += a0;
             if (seg > 1) y += a1;
