@@ -997,16 +997,10 @@ class Engine {
             }
             T_.hpack[q] = upload_vec(hp);
             T_.weights[q] = upload_arr(v.weights[q], nh);
-            T_.hc_off[q] = upload_vec(plan_.hc_off[q]);
-            T_.hc_idx[q] = upload_vec(plan_.hc_idx[q]);
-            T_.hc_t[q] = upload_vec(plan_.hc_t[q]);
-            T_.sd_cpos_t[q] = upload_vec(plan_.sd_cpos_t[q]);
             T_.sd_off[q] = upload_vec(plan_.sd_off[q]);
             T_.sd_cnt[q] = upload_vec(plan_.sd_cnt[q]);
             T_.sd_idx[q] = upload_vec(plan_.sd_idx[q]);
             T_.sd_coff[q] = upload_vec(plan_.sd_coff[q]);
-            T_.sd_cpos_base[q] = upload_vec(plan_.sd_cpos_base[q]);
-            T_.sd_cpos[q] = upload_vec(plan_.sd_cpos[q]);
             T_.sd_tab[q] = (const uint4*)upload_vec(plan_.sd_tab[q]);
             T_.sd_kslot[q] = upload_vec(plan_.sd_kslot[q]);
         }

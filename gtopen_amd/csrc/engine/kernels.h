@@ -48,17 +48,11 @@ struct DevTree {
     const uint32_t* same[2];
     const uint32_t* hpack[2];    // c1 | c2 << 8 | identical opponent combo << 16 (0xFFFF: none)
     const float* weights[2];
-    const uint32_t* hc_off[2];   // per-card hand lists (53 offsets)
-    const uint16_t* hc_idx[2];
-    const uint16_t* hc_t[2];          // entry-major hand lists: [t * 52 + card]
     // showdown tables, per river-board slot
     const uint32_t* sd_off[2];        // start of q's sorted list
     const uint32_t* sd_cnt[2];        // its length
     const uint16_t* sd_idx[2];        // hand index, ascending strength
     const uint32_t* sd_coff[2];       // 53 per board: per-card list offsets
-    const uint32_t* sd_cpos_base[2];  // per board: base of the per-card lists
-    const uint16_t* sd_cpos[2];       // sorted positions holding card c
-    const uint16_t* sd_cpos_t[2];     // entry-major: [board][t * 52 + card]
     const uint4* sd_tab[2];           // [board][hand]: packed rank/segment table
     const uint16_t* sd_kslot[2];      // [sorted hand]: slots in the per-card arrays (2 per hand)
     // MFMA showdown: per board, for each sorted hand of q, a bitset over the

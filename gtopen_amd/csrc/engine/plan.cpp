@@ -415,41 +415,6 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
     }
     runs(pl.show_all, kMmaNodes, pl.show_chunks_mma);
 
-    // per-card hand lists (fold terminals)
-    for (int q = 0; q < 2; ++q) {
-        std::vector<std::vector<uint16_t>> by_card(52);
-        for (uint32_t h = 0; h < v.num_hands[q]; ++h) {
-            by_card[v.hand_c1[q][h]].push_back((uint16_t)h);
-            by_card[v.hand_c2[q][h]].push_back((uint16_t)h);
-        }
-        pl.hc_off[q].resize(53);
-        uint32_t acc = 0;
-        for (int c = 0; c < 52; ++c) {
-            pl.hc_off[q][c] = acc;
-            for (uint16_t h : by_card[c]) pl.hc_idx[q].push_back(h);
-            acc += (uint32_t)by_card[c].size();
-        }
-        pl.hc_off[q][52] = acc;
-        pl.hc_t[q].assign((size_t)kHandListCap * 52, 0);
-        for (int c = 0; c < 52; ++c) {
-            if (by_card[c].size() > kHandListCap) throw std::runtime_error("hand list longer than capacity");
-            for (size_t t = 0; t < by_card[c].size(); ++t) pl.hc_t[q][t * 52 + c] = by_card[c][t];
-        }
-    }
-    // entry-major per-card position lists of every board
-    for (int q = 0; q < 2; ++q) {
-        pl.sd_cpos_t[q].assign((size_t)pl.num_boards * kCardListCap * 52, 0);
-        for (uint32_t b = 0; b < pl.num_boards; ++b) {
-            const uint32_t* coff = &pl.sd_coff[q][(size_t)b * 53];
-            const uint16_t* cpos = pl.sd_cpos[q].data() + pl.sd_cpos_base[q][b];
-            uint16_t* dst = &pl.sd_cpos_t[q][(size_t)b * kCardListCap * 52];
-            for (int c = 0; c < 52; ++c) {
-                uint32_t len = coff[c + 1] - coff[c];
-                if (len > kCardListCap) throw std::runtime_error("card list longer than capacity");
-                for (uint32_t t = 0; t < len; ++t) dst[t * 52 + c] = cpos[coff[c] + t];
-            }
-        }
-    }
     return pl;
 }
 

@@ -11,8 +11,6 @@
 
 namespace gto {
 
-constexpr uint32_t kHandListCap = 56;  // hands holding one card: <= 51
-constexpr uint32_t kCardListCap = 48;  // one card's hands on a river board: <= 46
 
 struct LevelLists {
     // action nodes split by actor, fold, showdown and chance nodes at this depth
@@ -80,16 +78,6 @@ struct Plan {
     std::vector<uint16_t> sd_cpos[2];
     uint32_t max_board_cnt[2] = {0, 0};
 
-    // per player: hand indices holding card c, ascending (fold card sums)
-    std::vector<uint32_t> hc_off[2];  // 53 entries
-    std::vector<uint16_t> hc_idx[2];
-    // the same lists entry-major: [t * 52 + c] = t-th hand holding card c
-    // (0 past the list's end), kHandListCap rows, so the lanes of a
-    // wavefront (one per card) read one contiguous row per entry
-    std::vector<uint16_t> hc_t[2];
-    // per board, q's per-card position lists entry-major likewise:
-    // [board][t * 52 + c], kCardListCap rows
-    std::vector<uint16_t> sd_cpos_t[2];
     // per board, for each of q's sorted hands: its slot in the per-card
     // arrays of its two cards (coff[c] + index within card c's list), two
     // u16 per hand, at sd_off[q][b] * 2
