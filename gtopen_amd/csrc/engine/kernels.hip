@@ -438,7 +438,7 @@ __device__ __forceinline__ void dispatch_na(int na, F&& f) {
 }
 
 // The same for the up sweep's kernels with at most 4 actions specialised:
-// those kernels hold two hands' values per lane, and the register demand of
+// those kernels hold four hands' values per lane, and the register demand of
 // the wider cases would lower every launch's occupancy; nodes with more
 // actions take the generic loop.
 template <class F>
@@ -502,6 +502,9 @@ __device__ __forceinline__ uint32_t wave_item() {
 // Hands per lane whose loads are issued together in the hand loops (a
 // load's first use otherwise waits for it before the next one is issued).
 constexpr int kBatch = 4;
+// Hands per lane in one step of the up-sweep action kernels: every load
+// of all of them is issued before the first store.
+constexpr int kStepHands = 4;
 
 // Orders this wavefront's LDS writes before its later LDS reads.
 __device__ __forceinline__ void wave_sync() {
@@ -1592,16 +1595,16 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                 const float* rw[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                for (uint32_t i0 = lane; i0 < nhp; i0 += 2 * kWave) {
-                    float v[2][NA];
+                for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
+                    float v[kStepHands][NA];
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
+                    for (int h = 0; h < kStepHands; ++h) {
                         const uint32_t ii = min(i0 + h * kWave, nhp - 1);
 #pragma unroll
                         for (int a = 0; a < NA; ++a) v[h][a] = rw[a][ii];
                     }
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
+                    for (int h = 0; h < kStepHands; ++h) {
                         const uint32_t i = i0 + h * kWave;
                         if (i >= nhp) break;
                         float acc = 0.0f;
@@ -1687,14 +1690,12 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     const float* rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                    // two hands per step: every load of both is issued before
-                    // the first store
-                    for (uint32_t i0 = lane; i0 < nhp; i0 += 2 * kWave) {
-                        HandSigma<NA, kCurrentF32> hs[2];
-                        float v[2][NA], sm[2][NA], reach[2], rr[2][PRED ? NA : 1];
-                        bool ok[2];
+                    for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
+                        HandSigma<NA, kCurrentF32> hs[kStepHands];
+                        float v[kStepHands][NA], sm[kStepHands][NA], reach[kStepHands], rr[kStepHands][PRED ? NA : 1];
+                        bool ok[kStepHands];
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) {
+                        for (int h = 0; h < kStepHands; ++h) {
                             const uint32_t i = i0 + h * kWave;
                             ok[h] = i < nhp;
                             const uint32_t ii = ok[h] ? i : i0;
@@ -1712,7 +1713,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             reach[h] = rp[ii];
                         }
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) {
+                        for (int h = 0; h < kStepHands; ++h) {
                             if (!ok[h]) continue;
                             const uint32_t i = i0 + h * kWave;
                             hs[h].finish();
@@ -1783,12 +1784,12 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     const float* rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                    for (uint32_t i0 = lane; i0 < nhp; i0 += 2 * kWave) {
-                        HandSigma<NA, kCurrentH> hs[2];
-                        float v[2][NA], sm[2][NA], reach[2];
-                        bool ok[2];
+                    for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
+                        HandSigma<NA, kCurrentH> hs[kStepHands];
+                        float v[kStepHands][NA], sm[kStepHands][NA], reach[kStepHands];
+                        bool ok[kStepHands];
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) {
+                        for (int h = 0; h < kStepHands; ++h) {
                             const uint32_t i = i0 + h * kWave;
                             ok[h] = i < nhp;
                             const uint32_t ii = ok[h] ? i : i0;
@@ -1801,7 +1802,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             reach[h] = rp[ii];
                         }
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) {
+                        for (int h = 0; h < kStepHands; ++h) {
                             if (!ok[h]) continue;
                             const uint32_t i = i0 + h * kWave;
                             hs[h].finish();
