@@ -505,6 +505,9 @@ constexpr int kBatch = 4;
 // Hands per lane in one step of the up-sweep action kernels: every load
 // of all of them is issued before the first store.
 constexpr int kStepHands = 4;
+// The same in the down-sweep action kernel (two: 2.09 ms per f32 iteration,
+// three: 1.93, four: 1.96).
+constexpr int kDownHands = 3;
 
 // Orders this wavefront's LDS writes before its later LDS reads.
 __device__ __forceinline__ void wave_sync() {
@@ -655,17 +658,17 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
                 float* dst[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) dst[a] = reach + (size_t)kid_own[a] * nh;
-                for (uint32_t i0 = lane; i0 < nh; i0 += 2 * kWave) {
-                    HandSigma<NA, KIND> hs[2];
-                    float pv[2];
+                for (uint32_t i0 = lane; i0 < nh; i0 += kDownHands * kWave) {
+                    HandSigma<NA, KIND> hs[kDownHands];
+                    float pv[kDownHands];
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
+                    for (int h = 0; h < kDownHands; ++h) {
                         const uint32_t ii = min(i0 + h * kWave, nh - 1);
                         hs[h].fetch(arena, nh, ii);
                         pv[h] = pr[ii];
                     }
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
+                    for (int h = 0; h < kDownHands; ++h) {
                         const uint32_t i = i0 + h * kWave;
                         if (i >= nh) break;
                         hs[h].finish();
@@ -788,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
 // ---------------------------------------------------------------------------
 constexpr int kMaxH = 21;         // hand rows per lane: 21 * 64 >= 1326
 constexpr int kRowPer = (1326 + kBlock - 1) / kBlock;  // hand rows per thread of a workgroup
-constexpr uint32_t kFoldNodes = 8;                      // fold nodes per wavefront
+constexpr uint32_t kFoldNodes = 16;                     // fold nodes per wavefront
 constexpr size_t kFoldWaveBytes = 52 * 52 * 4 + 52 * 8;  // card matrix + per-card sums
 
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
