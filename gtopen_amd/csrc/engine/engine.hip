@@ -1244,8 +1244,8 @@ class Engine {
                            (const ShowRec*)D.show, (const uint4*)D.show_chunks[cap_], D.n_show_chunks[cap_], p, swap);
             }
             if (D.n_chance)
-                launch(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), 0, T_, S_, (const ChanceRec*)D.chance,
-                       D.n_chance, p);
+                launch_b(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), kChanceBlock, 0, T_, S_,
+                         (const ChanceRec*)D.chance, D.n_chance, p);
             for (int q = 0; q < 2; ++q) {
                 if (!D.n_act[q]) continue;
                 const ActRec* recs = D.act[q];

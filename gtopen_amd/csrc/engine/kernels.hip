@@ -1484,12 +1484,14 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
 // ---------------------------------------------------------------------------
 // Up sweep, chance nodes (cfr.rs:637-663): sum the children's values over
 // the dealt cards in card order, skipping hands that hold the card, then
-// scale by 1/(46 - street). One workgroup per node (a node reads up to 49
-// child rows); loads are issued in batches of 8 children ahead of the
-// in-order adds.
+// scale by 1/(46 - street). One workgroup of kChanceBlock threads per node
+// (a node reads up to 49 child rows; there are few chance nodes per level,
+// so a hand per thread keeps the most loads in flight); loads are issued in
+// batches of 8 children ahead of the in-order adds.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, const ChanceRec* __restrict__ recs,
-                                                      uint32_t count, int p) {
+constexpr int kChanceBlock = 1024;
+__global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState S, const ChanceRec* __restrict__ recs,
+                                                            uint32_t count, int p) {
     const uint32_t w = xcd_slice(blockIdx.x, gridDim.x);
     if (w >= count) return;
     const int o = 1 - p;
@@ -1497,7 +1499,7 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
     const uint32_t nhp = T.nh[p];
     float* out = S.cfv + (size_t)r.cfv * T.nh_max;
     if (!S.alive[o][pick(r.own, o)]) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) out[i] = 0.0f;
+        for (uint32_t i = threadIdx.x; i < nhp; i += kChanceBlock) out[i] = 0.0f;
         if (S.vmax && threadIdx.x == 0) S.vmax[r.cfv] = 0.0f;
         return;
     }
@@ -1514,7 +1516,7 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
         if (!hperm) return i;
         return perms[t] ? hperm[(size_t)perms[t] * nhp + i] : i;
     };
-    for (uint32_t i = threadIdx.x; i < nhp; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < nhp; i += kChanceBlock) {
         const uint64_t hm = T.hmask[p][i];
         float acc = 0.0f;
         uint32_t t = 0;
@@ -1546,11 +1548,16 @@ __global__ __launch_bounds__(kBlock) void k_up_chance(DevTree T, DevState S, con
         vm = fmaxf(vm, fabsf(acc * r.inv));
     }
     if (S.vmax) {
-        __shared__ float wm[kWaves];
+        constexpr int kW = kChanceBlock / kWave;
+        __shared__ float wm[kW];
         vm = wave_max_f32(vm);
         if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = vm;
         __syncthreads();
-        if (threadIdx.x == 0) S.vmax[r.cfv] = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+        if (threadIdx.x == 0) {
+            float m = wm[0];
+            for (int k = 1; k < kW; ++k) m = fmaxf(m, wm[k]);
+            S.vmax[r.cfv] = m;
+        }
     }
 }
 
