@@ -892,7 +892,8 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         // gradients with respect to the opponent's range and strategy
         const float amount = (r.folder == (uint32_t)p) != (swap != 0) ? r.t_lose : r.t_win;
         float vm = 0.0f;
-#pragma unroll
+        // a rolled loop (the unrolled form, with its early exit removed,
+        // holds every hand's loads at once and runs slower)
         for (int u = 0; u < kMaxH; ++u) {
             const uint32_t i = lane + u * kWave;
             if (u * kWave >= (int)nhp) break;
