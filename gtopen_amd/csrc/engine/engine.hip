@@ -1013,7 +1013,7 @@ class Engine {
                 for (int c = 0; c < 52; ++c)
                     if (cf[c + 1] - cf[c] > (uint32_t)kMaxCardList)
                         throw std::runtime_error("a card appears in more opponent hands than the showdown kernel holds");
-                need = std::max<size_t>(need, show_lds_layout(plan_.sd_cnt[o][b], plan_.nh[p], plan_.nh[o]).bytes);
+                need = std::max<size_t>(need, show_lds_layout(plan_.sd_cnt[o][b], plan_.sd_cnt[p][b], plan_.nh[p], plan_.nh[o]).bytes);
             }
             show_lds_[p] = need;
             fold_lds_[p] = kFoldWaveBytes * kWaves;

@@ -53,7 +53,7 @@ struct DevTree {
     const uint32_t* sd_cnt[2];        // its length
     const uint16_t* sd_idx[2];        // hand index, ascending strength
     const uint32_t* sd_coff[2];       // 53 per board: per-card list offsets
-    const uint4* sd_tab[2];           // [board][hand]: packed rank/segment table
+    const uint4* sd_tab[2];           // [sorted hand] (at sd_off): packed rank/segment table
     const uint16_t* sd_kslot[2];      // [sorted hand]: slots in the per-card arrays (2 per hand)
     // MFMA showdown: per board, for each sorted hand of q, a bitset over the
     // opponent's sorted hands (1 = no shared card), padded to 32-bit words
@@ -64,13 +64,13 @@ struct DevTree {
 };
 
 // LDS layout of the showdown kernel for one board (byte offsets): the
-// board's static tables (per-hand entries, opponent sorted hand indices,
+// board's static tables (entries per sorted hand, opponent sorted hand indices,
 // per-card offsets and positions) stay for a run of nodes; the per-node
 // rows (strength-order prefix P, per-card prefixes, reach in strength
 // order; the hand-order reach row is staged in the per-card region first)
 // are rebuilt per node.
 struct ShowLds {
-    uint32_t ent, P, cpre, wsum, rs, rcard, okey, coff, nodes, bytes;
+    uint32_t ent, P, cpre, wsum, rs, rcard, okey, coff, nodes, O, pidx, bytes;
 };
 // per node of a run, staged once: value row, opponent reach slot, live
 // flag, win, lose, tie
@@ -78,11 +78,12 @@ struct ShowNode {
     uint32_t cfv, oslot, live;
     float win, lose, tie;
 };
-__host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, uint32_t nho) {
+// mo / mp: the opponent's / traverser's hands on the board; nho / nhp: all their hands
+__host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t mp, uint32_t nhp, uint32_t nho) {
     ShowLds L;
     const uint32_t creg = (2 * mo + 52) > ((nho + 1) / 2) ? (2 * mo + 52) : ((nho + 1) / 2);
-    L.ent = 0;                                  // nhp x uint4
-    L.P = L.ent + 16 * nhp;                     // (mo + 1) doubles
+    L.ent = 0;                                  // mp x uint4
+    L.P = L.ent + 16 * mp;                      // (mo + 1) doubles
     L.cpre = L.P + 8 * (mo + 1);                // creg doubles
     L.wsum = L.cpre + 8 * creg;                 // 4 doubles (per-wave scan totals)
     L.rs = L.wsum + 32;                         // mo floats: reach in strength order
@@ -90,7 +91,9 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, ui
     L.okey = (L.rcard + 8 * mo + 7) & ~7u;      // mo uint2: hand index, per-card slots
     L.coff = L.okey + 8 * mo;                   // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
-    L.bytes = (L.nodes + 16 * sizeof(ShowNode) + 15) & ~15u;
+    L.O = L.nodes + 16 * sizeof(ShowNode);      // nhp floats: the value row in hand order
+    L.pidx = L.O + 4 * nhp;                     // mp u16: hand of each sorted position
+    L.bytes = (L.pidx + 2 * mp + 15) & ~15u;
     return L;
 }
 

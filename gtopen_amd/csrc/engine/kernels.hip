@@ -945,8 +945,8 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     const int o = 1 - p;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nhp = T.nh[p], nho = T.nh[o];
-    const uint32_t mo = T.sd_cnt[o][b];
-    const ShowLds L = show_lds_layout(mo, nhp, nho);
+    const uint32_t mo = T.sd_cnt[o][b], mp = T.sd_cnt[p][b];
+    const ShowLds L = show_lds_layout(mo, mp, nhp, nho);
     char* base = (char*)lds;
     uint4* ent = (uint4*)(base + L.ent);
     double* P = (double*)(base + L.P);
@@ -960,6 +960,8 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     uint2* okey = (uint2*)(base + L.okey);
     uint32_t* coff = (uint32_t*)(base + L.coff);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
+    float* O = (float*)(base + L.O);
+    uint16_t* pidx = (uint16_t*)(base + L.pidx);
 
     // the run's records and alive flags, then the board statics
     if (tid < cnt) {
@@ -975,8 +977,15 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         nodes[tid] = nd;
     }
     {
-        const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
-        for (uint32_t i = tid; i < nhp; i += kBlock) ent[i] = tab[i];
+        const uint32_t sp = T.sd_off[p][b];
+        const uint4* __restrict__ tab = T.sd_tab[p] + sp;
+        const uint16_t* __restrict__ gp = T.sd_idx[p] + sp;
+        for (uint32_t t = tid; t < mp; t += kBlock) {
+            ent[t] = tab[t];
+            pidx[t] = gp[t];
+        }
+        // hands not on the board keep 0 in the staged value row
+        for (uint32_t i = tid; i < nhp; i += kBlock) O[i] = 0.0f;
         const uint32_t so = T.sd_off[o][b];
         const uint16_t* __restrict__ gi = T.sd_idx[o] + so;
         const uint32_t* __restrict__ gk = (const uint32_t*)(T.sd_kslot[o] + 2 * (size_t)so);
@@ -1093,18 +1102,16 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             if (seg == 3) cpre[c0 + card + len] = ((a0 + a1) + a2) + a3;
         }
         __syncthreads();
-        // 4. per-hand values
+        // 4. per-hand values, the traverser's hands in strength order (the
+        //    rank bounds, hence the prefix reads, rise with the thread index),
+        //    staged in hand order in LDS
         const float win = nd.win, lose = nd.lose, tie = nd.tie;
         float vm = 0.0f;
 #pragma unroll
         for (int u = 0; u < kRowPer; ++u) {
-            const uint32_t i = tid + u * kBlock;
-            if (i >= nhp) break;
-            const uint4 e = ent[i];
-            if (e.x == 0xFFFFFFFFu) {
-                out[i] = 0.0f;
-                continue;
-            }
+            const uint32_t t = tid + u * kBlock;
+            if (t >= mp) break;
+            const uint4 e = ent[t];
             const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
             const double* cp1 = cpre + (e.y & 0xFFFFu);
             const double* cp2 = cpre + (e.y >> 16);
@@ -1117,7 +1124,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             const float valid = (float)(total - S1 - S2) + same_r;
             const float tied = valid - lower - higher;
             const float val = win * lower + lose * higher + tie * tied;
-            out[i] = val;
+            O[pidx[t]] = val;
             vm = fmaxf(vm, fabsf(val));
         }
         if (S.vmax) {
@@ -1125,6 +1132,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             if (lane == 0) wsum[wave] = (double)vm;
         }
         __syncthreads();  // this node's LDS reads complete before the next node's writes
+        for (uint32_t i = tid; i < nhp; i += kBlock) out[i] = O[i];
         if (S.vmax && tid == 0)
             S.vmax[nd.cfv] = (float)fmax(fmax(wsum[0], wsum[1]), fmax(wsum[2], wsum[3]));
     }

@@ -265,8 +265,8 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
             }
         }
     }
-    // hand-ordered showdown tables: 16 bytes per (board, hand of p)
-    //   x = lb | ub << 16        (0xFFFFFFFF: the hand conflicts with the board)
+    // showdown tables: 16 bytes per (board, sorted hand of p), at sd_off[p][b]
+    //   x = lb | ub << 16
     //   y = seg(c1) | seg(c2) << 16   start of each card's list in the
     //       opponent's per-card arrays (the card offset plus the card index)
     //   z = entries before lb (c1, c2), before ub (c1, c2), one byte each
@@ -274,9 +274,7 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
     //       identical combo << 16 (0xFFFF: none)
     for (int p = 0; p < 2; ++p) {
         const int o = 1 - p;
-        const size_t nhp = v.num_hands[p];
-        pl.sd_tab[p].assign((size_t)pl.num_boards * nhp * 4, 0u);
-        for (size_t e = 0; e < (size_t)pl.num_boards * nhp; ++e) pl.sd_tab[p][4 * e] = 0xFFFFFFFFu;
+        pl.sd_tab[p].assign(pl.sd_idx[p].size() * 4, 0u);
         for (uint32_t b = 0; b < pl.num_boards; ++b) {
             const uint32_t* coff = &pl.sd_coff[o][(size_t)b * 53];
             std::vector<int> pos_of(v.num_hands[o], -1);
@@ -293,7 +291,7 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
                     sp = (uint32_t)pos_of[sm];
                 }
                 if (coff[52] + 52 > 0xFFFFu || l1 > 255 || l2 > 255) throw std::runtime_error("showdown table overflow");
-                uint32_t* e = &pl.sd_tab[p][4 * ((size_t)b * nhp + h)];
+                uint32_t* e = &pl.sd_tab[p][4 * (size_t)k];
                 e[0] = pl.sd_lbub[p][k];
                 e[1] = (coff[a1] + a1) | ((coff[a2] + a2) << 16);
                 e[2] = pl.sd_cnt4[p][k];

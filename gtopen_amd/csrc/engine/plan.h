@@ -89,9 +89,9 @@ struct Plan {
     std::vector<uint8_t> chr_card;
     std::vector<uint32_t> chr_cfv;
     std::vector<uint8_t> chr_perm;  // symmetry index per entry (0 = identity)
-    // per board slot and hand of q (hand order), four words: rank bounds,
-    // per-card list segments, per-card counts, list lengths + identical
-    // combo position (layout documented in build_plan)
+    // per board slot and sorted hand of q (at sd_off[q][b]), four words:
+    // rank bounds, per-card list segments, per-card counts, list lengths +
+    // identical combo position (layout documented in build_plan)
     std::vector<uint32_t> sd_tab[2];
     uint32_t max_actions = 0;
 
