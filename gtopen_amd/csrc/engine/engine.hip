@@ -1002,7 +1002,7 @@ class Engine {
             T_.sd_idx[q] = upload_vec(plan_.sd_idx[q]);
             T_.sd_coff[q] = upload_vec(plan_.sd_coff[q]);
             T_.sd_tab[q] = (const uint4*)upload_vec(plan_.sd_tab[q]);
-            T_.sd_kslot[q] = upload_vec(plan_.sd_kslot[q]);
+            T_.sd_cpos[q] = upload_vec(plan_.sd_cpos[q]);
         }
         // LDS: showdown, per workgroup, for traverser p (largest board)
         for (int p = 0; p < 2; ++p) {

@@ -54,7 +54,7 @@ struct DevTree {
     const uint16_t* sd_idx[2];        // hand index, ascending strength
     const uint32_t* sd_coff[2];       // 53 per board: per-card list offsets
     const uint4* sd_tab[2];           // [sorted hand] (at sd_off): packed rank/segment table
-    const uint16_t* sd_kslot[2];      // [sorted hand]: slots in the per-card arrays (2 per hand)
+    const uint16_t* sd_cpos[2];       // per board at 2 * sd_off: sorted positions holding card 0, 1, ...
     // MFMA showdown: per board, for each sorted hand of q, a bitset over the
     // opponent's sorted hands (1 = no shared card), padded to 32-bit words
     // and stored pair-split (bit 2i of a word at bit i, bit 2i+1 at 16+i)
@@ -65,12 +65,12 @@ struct DevTree {
 
 // LDS layout of the showdown kernel for one board (byte offsets): the
 // board's static tables (entries per sorted hand, opponent sorted hand indices,
-// per-card offsets and positions) stay for a run of nodes; the per-node
+// per-card offsets) stay for a run of nodes; the per-node
 // rows (strength-order prefix P, per-card prefixes, reach in strength
 // order; the hand-order reach row is staged in the per-card region first)
 // are rebuilt per node.
 struct ShowLds {
-    uint32_t ent, P, cpre, wsum, rs, rcard, okey, coff, nodes, O, pidx, bytes;
+    uint32_t ent, P, cpre, wsum, rs, oidx, coff, nodes, O, pidx, bytes;
 };
 // per node of a run, staged once: value row, opponent reach slot, live
 // flag, win, lose, tie
@@ -87,9 +87,8 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t mp, uin
     L.cpre = L.P + 8 * (mo + 1);                // creg doubles
     L.wsum = L.cpre + 8 * creg;                 // 4 doubles (per-wave scan totals)
     L.rs = L.wsum + 32;                         // mo floats: reach in strength order
-    L.rcard = L.rs + 4 * mo;                    // 2 mo floats: reach in per-card list order
-    L.okey = (L.rcard + 8 * mo + 7) & ~7u;      // mo uint2: hand index, per-card slots
-    L.coff = L.okey + 8 * mo;                   // 53 u32
+    L.oidx = L.rs + 4 * mo;                     // mo u32: opponent hand of sorted position k
+    L.coff = L.oidx + 4 * mo;                   // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
     L.O = L.nodes + 16 * sizeof(ShowNode);      // nhp floats: the value row in hand order
     L.pidx = L.O + 4 * nhp;                     // mp u16: hand of each sorted position

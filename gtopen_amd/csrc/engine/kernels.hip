@@ -925,7 +925,8 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 //      chunks and a workgroup scan),
 //   3. per card, the exclusive prefix of the reach of the opponent hands
 //      holding it, in strength order (f64; four lanes per card, each a
-//      quarter of the card's list),
+//      quarter of the card's list, read from the strength-order row at
+//      positions held in registers for the run),
 //   4. per hand, O(1) lookups: L = P[lb] - (card prefixes at lb),
 //      H = (total - P[ub]) - (card totals - card prefixes at ub),
 //      V = total - card totals + identical combo.
@@ -953,11 +954,8 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     double* cpre = (double*)(base + L.cpre);
     double* wsum = (double*)(base + L.wsum);
     float* rs = (float*)(base + L.rs);
-    float* rcard = (float*)(base + L.rcard);
     float* rhs = (float*)cpre;
-    // per sorted opponent hand: its hand index and its two slots in the
-    // per-card arrays (x: index, y: slot | slot << 16)
-    uint2* okey = (uint2*)(base + L.okey);
+    uint32_t* oidx = (uint32_t*)(base + L.oidx);  // opponent hand of sorted position k
     uint32_t* coff = (uint32_t*)(base + L.coff);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
     float* O = (float*)(base + L.O);
@@ -988,8 +986,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         for (uint32_t i = tid; i < nhp; i += kBlock) O[i] = 0.0f;
         const uint32_t so = T.sd_off[o][b];
         const uint16_t* __restrict__ gi = T.sd_idx[o] + so;
-        const uint32_t* __restrict__ gk = (const uint32_t*)(T.sd_kslot[o] + 2 * (size_t)so);
-        for (uint32_t k = tid; k < mo; k += kBlock) okey[k] = make_uint2(gi[k], gk[k]);
+        for (uint32_t k = tid; k < mo; k += kBlock) oidx[k] = gi[k];
         if (tid < 53) coff[tid] = T.sd_coff[o][(size_t)b * 53 + tid];
     }
     __syncthreads();
@@ -1012,6 +1009,16 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         len = coff[card + 1] - c0;
         s0 = min(seg * kSeg, len);
         sn = min(kSeg, len - s0);
+    }
+    // the strength-order positions of this thread's segment of its card's
+    // list (static for the run; entries past the segment repeat an in-range
+    // position and are not summed)
+    uint32_t cpos[kSeg];
+    {
+        const uint16_t* __restrict__ gpos = T.sd_cpos[o] + 2 * (size_t)T.sd_off[o][b];
+        const uint32_t last = mo > 0 ? 2 * mo - 1 : 0;
+#pragma unroll
+        for (int u = 0; u < (int)kSeg; ++u) cpos[u] = mo > 0 ? gpos[min(c0 + s0 + u, last)] : 0u;
     }
 
 #pragma unroll 1
@@ -1036,24 +1043,16 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             continue;
         }
         __syncthreads();
-        // 2. strength order (also scattered into the per-card arrays) and the
-        //    chunk sums of the exclusive prefix
+        // 2. strength order and the chunk sums of its exclusive prefix
         float v[kR];
-        uint32_t ks[kR];
         double part = 0.0;
 #pragma unroll
-        for (int u = 0; u < kR; ++u) {
-            const uint2 e = okey[min(lo + u, mo - 1)];
-            ks[u] = e.y;
-            v[u] = rhs[e.x];
-        }
+        for (int u = 0; u < kR; ++u) v[u] = rhs[oidx[min(lo + u, mo - 1)]];
 #pragma unroll
         for (int u = 0; u < kR; ++u) {
             const uint32_t k = lo + u;
             if (k < hi) {
                 rs[k] = v[u];
-                rcard[ks[u] & 0xFFFFu] = v[u];
-                rcard[ks[u] >> 16] = v[u];
                 part += (double)v[u];
             }
         }
@@ -1082,7 +1081,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         if (card < 52) {
             float x[kSeg];
 #pragma unroll
-            for (int u = 0; u < (int)kSeg; ++u) x[u] = rcard[c0 + min(s0 + u, len > 0 ? len - 1 : 0)];
+            for (int u = 0; u < (int)kSeg; ++u) x[u] = rs[cpos[u]];
             double a = 0.0;
 #pragma unroll
             for (int u = 0; u < (int)kSeg; ++u) a += (uint32_t)u < sn ? (double)x[u] : 0.0;

@@ -215,19 +215,12 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
                 by_card[v.hand_c2[q][h]].push_back((uint16_t)t);
             }
             pl.sd_cpos_base[q][b] = (uint32_t)pl.sd_cpos[q].size();
+            // the device finds a board's lists at 2 * sd_off (two entries per hand)
+            if (pl.sd_cpos_base[q][b] != 2 * pl.sd_off[q][b]) throw std::runtime_error("per-card list offsets out of step");
             uint32_t acc = 0;
-            const size_t ks = pl.sd_kslot[q].size();
-            pl.sd_kslot[q].resize(ks + 2 * (size_t)cnt);
             for (int c = 0; c < 52; ++c) {
                 pl.sd_coff[q][(size_t)b * 53 + c] = acc;
-                for (size_t t = 0; t < by_card[c].size(); ++t) {
-                    const uint16_t pos = by_card[c][t];
-                    pl.sd_cpos[q].push_back(pos);
-                    // first or second card of the hand at sorted position pos
-                    const uint16_t h = v.river_idx[q][off + pos];
-                    const int which = v.hand_c1[q][h] == c ? 0 : 1;
-                    pl.sd_kslot[q][ks + 2 * (size_t)pos + which] = (uint16_t)(acc + t);
-                }
+                for (uint16_t pos : by_card[c]) pl.sd_cpos[q].push_back(pos);
                 acc += (uint32_t)by_card[c].size();
             }
             pl.sd_coff[q][(size_t)b * 53 + 52] = acc;

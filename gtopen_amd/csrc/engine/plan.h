@@ -78,10 +78,6 @@ struct Plan {
     std::vector<uint16_t> sd_cpos[2];
     uint32_t max_board_cnt[2] = {0, 0};
 
-    // per board, for each of q's sorted hands: its slot in the per-card
-    // arrays of its two cards (coff[c] + index within card c's list), two
-    // u16 per hand, at sd_off[q][b] * 2
-    std::vector<uint16_t> sd_kslot[2];
 
     // children of action nodes referenced by ActRec::kid
     std::vector<uint32_t> kids;
