@@ -1013,7 +1013,7 @@ class Engine {
                 for (int c = 0; c < 52; ++c)
                     if (cf[c + 1] - cf[c] > (uint32_t)kMaxCardList)
                         throw std::runtime_error("a card appears in more opponent hands than the showdown kernel holds");
-                need = std::max<size_t>(need, show_lds_layout(plan_.sd_cnt[o][b], plan_.sd_cnt[p][b], plan_.nh[p], plan_.nh[o]).bytes);
+                need = std::max<size_t>(need, show_lds_layout(plan_.sd_cnt[o][b], plan_.nh[p], plan_.nh[o]).bytes);
             }
             show_lds_[p] = need;
             fold_lds_[p] = kFoldWaveBytes * kWaves;
@@ -1213,9 +1213,9 @@ class Engine {
     // precision form serves the solve's own sweeps (0, 3); best response,
     // average-strategy values and the game value always take the exact
     // showdown, so exploitability judges every form with the same arithmetic.
-    // the exact showdown kernel sized to the opponent's hand count (3 rows
-    // per thread cover up to 768 hands)
-    auto show_kernel(int p) const { return plan_.nh[1 - p] <= 3u * kBlock ? k_up_show<3> : k_up_show<kRowPer>; }
+    // the exact showdown kernel sized to the larger hand count (3 rows per
+    // thread cover up to 768 hands)
+    auto show_kernel(int) const { return std::max(plan_.nh[0], plan_.nh[1]) <= 3u * kBlock ? k_up_show<3> : k_up_show<kRowPer>; }
 
     void launch_up(int p, int mode, int swap = 0) {
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);

@@ -64,13 +64,12 @@ struct DevTree {
 };
 
 // LDS layout of the showdown kernel for one board (byte offsets): the
-// board's static tables (entries per sorted hand, opponent sorted hand indices,
-// per-card offsets) stay for a run of nodes; the per-node
-// rows (strength-order prefix P, per-card prefixes, reach in strength
-// order; the hand-order reach row is staged in the per-card region first)
-// are rebuilt per node.
+// per-card offsets stay for a run of nodes (the other static tables are
+// held in registers); the per-node rows (strength-order prefix P, per-card
+// prefixes, reach in strength order; the hand-order reach row is staged in
+// the per-card region first) are rebuilt per node.
 struct ShowLds {
-    uint32_t ent, P, cpre, wsum, rs, oidx, coff, nodes, O, pidx, bytes;
+    uint32_t P, cpre, wsum, rs, coff, nodes, O, bytes;
 };
 // per node of a run, staged once: value row, opponent reach slot, live
 // flag, win, lose, tie
@@ -78,21 +77,18 @@ struct ShowNode {
     uint32_t cfv, oslot, live;
     float win, lose, tie;
 };
-// mo / mp: the opponent's / traverser's hands on the board; nho / nhp: all their hands
-__host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t mp, uint32_t nhp, uint32_t nho) {
+// mo: the opponent's hands on the board; nho / nhp: all hands of each player
+__host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, uint32_t nho) {
     ShowLds L;
     const uint32_t creg = (2 * mo + 52) > ((nho + 1) / 2) ? (2 * mo + 52) : ((nho + 1) / 2);
-    L.ent = 0;                                  // mp x uint4
-    L.P = L.ent + 16 * mp;                      // (mo + 1) doubles
+    L.P = 0;                                    // (mo + 1) doubles
     L.cpre = L.P + 8 * (mo + 1);                // creg doubles
     L.wsum = L.cpre + 8 * creg;                 // 4 doubles (per-wave scan totals)
     L.rs = L.wsum + 32;                         // mo floats: reach in strength order
-    L.oidx = L.rs + 4 * mo;                     // mo u32: opponent hand of sorted position k
-    L.coff = L.oidx + 4 * mo;                   // 53 u32
+    L.coff = L.rs + 4 * mo;                     // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
     L.O = L.nodes + 16 * sizeof(ShowNode);      // nhp floats: the value row in hand order
-    L.pidx = L.O + 4 * nhp;                     // mp u16: hand of each sorted position
-    L.bytes = (L.pidx + 2 * mp + 15) & ~15u;
+    L.bytes = (L.O + 4 * nhp + 15) & ~15u;
     return L;
 }
 

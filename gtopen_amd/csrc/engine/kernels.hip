@@ -932,8 +932,8 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 //      V = total - card totals + identical combo.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
-// kR: opponent hand rows per thread, kR * kBlock >= the opponent's hand count
-// (also the strength-order chunk per thread in phase 2)
+// kR: hand rows per thread, kR * kBlock >= either player's hand count (also
+// the strength-order chunk per thread in phase 2)
 template <int kR>
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
@@ -947,19 +947,16 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nhp = T.nh[p], nho = T.nh[o];
     const uint32_t mo = T.sd_cnt[o][b], mp = T.sd_cnt[p][b];
-    const ShowLds L = show_lds_layout(mo, mp, nhp, nho);
+    const ShowLds L = show_lds_layout(mo, nhp, nho);
     char* base = (char*)lds;
-    uint4* ent = (uint4*)(base + L.ent);
     double* P = (double*)(base + L.P);
     double* cpre = (double*)(base + L.cpre);
     double* wsum = (double*)(base + L.wsum);
     float* rs = (float*)(base + L.rs);
     float* rhs = (float*)cpre;
-    uint32_t* oidx = (uint32_t*)(base + L.oidx);  // opponent hand of sorted position k
     uint32_t* coff = (uint32_t*)(base + L.coff);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
     float* O = (float*)(base + L.O);
-    uint16_t* pidx = (uint16_t*)(base + L.pidx);
 
     // the run's records and alive flags, then the board statics
     if (tid < cnt) {
@@ -974,19 +971,30 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         nd.tie = rr.tie;
         nodes[tid] = nd;
     }
+    // static for the run, in registers: this thread's sorted traverser hands
+    // (table entry, hand index) and its strength-order chunk's opponent hands
+    const uint32_t cs = (mo + kBlock - 1) / kBlock;
+    const uint32_t lo = min(tid * cs, mo), hi = min(lo + cs, mo);
+    uint4 ent[kR];
+    uint32_t pidx[kR], oidx[kR];
     {
         const uint32_t sp = T.sd_off[p][b];
         const uint4* __restrict__ tab = T.sd_tab[p] + sp;
         const uint16_t* __restrict__ gp = T.sd_idx[p] + sp;
-        for (uint32_t t = tid; t < mp; t += kBlock) {
-            ent[t] = tab[t];
-            pidx[t] = gp[t];
+        const uint32_t mlast = mp > 0 ? mp - 1 : 0;
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const uint32_t t = min(tid + u * kBlock, mlast);
+            ent[u] = mp > 0 ? tab[t] : make_uint4(0u, 0u, 0u, 0u);
+            pidx[u] = mp > 0 ? gp[t] : 0u;
         }
-        // hands not on the board keep 0 in the staged value row
-        for (uint32_t i = tid; i < nhp; i += kBlock) O[i] = 0.0f;
         const uint32_t so = T.sd_off[o][b];
         const uint16_t* __restrict__ gi = T.sd_idx[o] + so;
-        for (uint32_t k = tid; k < mo; k += kBlock) oidx[k] = gi[k];
+        const uint32_t olast = mo > 0 ? mo - 1 : 0;
+#pragma unroll
+        for (int u = 0; u < kR; ++u) oidx[u] = mo > 0 ? gi[min(lo + u, olast)] : 0u;
+        // hands not on the board keep 0 in the staged value row
+        for (uint32_t i = tid; i < nhp; i += kBlock) O[i] = 0.0f;
         if (tid < 53) coff[tid] = T.sd_coff[o][(size_t)b * 53 + tid];
     }
     __syncthreads();
@@ -1000,9 +1008,6 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     // per-card segment of this thread (cards 0..51, four lanes each)
     const uint32_t card = tid >> 2, seg = tid & 3;
     constexpr uint32_t kSeg = kMaxCardList / 4;
-    // strength-order chunk of this thread
-    const uint32_t cs = (mo + kBlock - 1) / kBlock;
-    const uint32_t lo = min(tid * cs, mo), hi = min(lo + cs, mo);
     uint32_t c0 = 0, len = 0, s0 = 0, sn = 0;
     if (card < 52) {
         c0 = coff[card];
@@ -1047,7 +1052,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         float v[kR];
         double part = 0.0;
 #pragma unroll
-        for (int u = 0; u < kR; ++u) v[u] = rhs[oidx[min(lo + u, mo - 1)]];
+        for (int u = 0; u < kR; ++u) v[u] = rhs[oidx[u]];
 #pragma unroll
         for (int u = 0; u < kR; ++u) {
             const uint32_t k = lo + u;
@@ -1107,10 +1112,10 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         const float win = nd.win, lose = nd.lose, tie = nd.tie;
         float vm = 0.0f;
 #pragma unroll
-        for (int u = 0; u < kRowPer; ++u) {
+        for (int u = 0; u < kR; ++u) {
             const uint32_t t = tid + u * kBlock;
             if (t >= mp) break;
-            const uint4 e = ent[t];
+            const uint4 e = ent[u];
             const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
             const double* cp1 = cpre + (e.y & 0xFFFFu);
             const double* cp2 = cpre + (e.y >> 16);
@@ -1123,7 +1128,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             const float valid = (float)(total - S1 - S2) + same_r;
             const float tied = valid - lower - higher;
             const float val = win * lower + lose * higher + tie * tied;
-            O[pidx[t]] = val;
+            O[pidx[u]] = val;
             vm = fmaxf(vm, fabsf(val));
         }
         if (S.vmax) {
