@@ -1016,14 +1016,14 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         sn = min(kSeg, len - s0);
     }
     // the strength-order positions of this thread's segment of its card's
-    // list (static for the run; entries past the segment repeat an in-range
-    // position and are not summed)
+    // list (static for the run; entries past the segment read position 0,
+    // one address for every such lane, and are not summed)
     uint32_t cpos[kSeg];
     {
         const uint16_t* __restrict__ gpos = T.sd_cpos[o] + 2 * (size_t)T.sd_off[o][b];
         const uint32_t last = mo > 0 ? 2 * mo - 1 : 0;
 #pragma unroll
-        for (int u = 0; u < (int)kSeg; ++u) cpos[u] = mo > 0 ? gpos[min(c0 + s0 + u, last)] : 0u;
+        for (int u = 0; u < (int)kSeg; ++u) cpos[u] = (uint32_t)u < sn ? gpos[min(c0 + s0 + u, last)] : 0u;
     }
 
 #pragma unroll 1
