@@ -69,7 +69,7 @@ struct DevTree {
 // prefixes, reach in strength order; the hand-order reach row is staged in
 // the per-card region first) are rebuilt per node.
 struct ShowLds {
-    uint32_t P, cpre, wsum, rs, coff, nodes, O, bytes;
+    uint32_t P, cpre, creg, wsum, rs, coff, nodes, O, bytes;
 };
 // per node of a run, staged once: value row, opponent reach slot, live
 // flag, win, lose, tie
@@ -81,11 +81,14 @@ struct ShowNode {
 __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, uint32_t nho) {
     ShowLds L;
     const uint32_t creg = (2 * mo + 52) > ((nho + 1) / 2) ? (2 * mo + 52) : ((nho + 1) / 2);
-    L.P = 0;                                    // (mo + 1) doubles
-    L.cpre = L.P + 8 * (mo + 1);                // creg doubles
-    L.wsum = L.cpre + 8 * creg;                 // 4 doubles (per-wave scan totals)
-    L.rs = L.wsum + 32;                         // mo floats: reach in strength order
-    L.coff = L.rs + 4 * mo;                     // 53 u32
+    // the last slot of P, cpre and rs is a spare: padded lanes store there
+    // (P, cpre) or read a zero there (rs) instead of branching
+    L.P = 0;                                    // (mo + 2) doubles
+    L.cpre = L.P + 8 * (mo + 2);                // creg + 1 doubles
+    L.creg = creg;
+    L.wsum = L.cpre + 8 * (creg + 1);           // 4 doubles (per-wave scan totals)
+    L.rs = L.wsum + 32;                         // mo + 1 floats: reach in strength order, then 0
+    L.coff = L.rs + 4 * (mo + 1);               // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
     L.O = L.nodes + 16 * sizeof(ShowNode);      // nhp floats: the value row in hand order
     L.bytes = (L.O + 4 * nhp + 15) & ~15u;

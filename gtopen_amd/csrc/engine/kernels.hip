@@ -995,6 +995,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         for (int u = 0; u < kR; ++u) oidx[u] = mo > 0 ? gi[min(lo + u, olast)] : 0u;
         // hands not on the board keep 0 in the staged value row
         for (uint32_t i = tid; i < nhp; i += kBlock) O[i] = 0.0f;
+        if (tid == 0) rs[mo] = 0.0f;
         if (tid < 53) coff[tid] = T.sd_coff[o][(size_t)b * 53 + tid];
     }
     __syncthreads();
@@ -1015,15 +1016,18 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         s0 = min(seg * kSeg, len);
         sn = min(kSeg, len - s0);
     }
-    // the strength-order positions of this thread's segment of its card's
-    // list (static for the run; entries past the segment read position 0,
-    // one address for every such lane, and are not summed)
-    uint32_t cpos[kSeg];
+    // this thread's segment of its card's list, static for the run: the
+    // strength-order position of each entry and the cpre slot of its prefix
+    // (position | slot << 16); slots past the segment read the zero in rs's
+    // spare slot and store to cpre's
+    uint32_t cw[kSeg];
     {
         const uint16_t* __restrict__ gpos = T.sd_cpos[o] + 2 * (size_t)T.sd_off[o][b];
         const uint32_t last = mo > 0 ? 2 * mo - 1 : 0;
 #pragma unroll
-        for (int u = 0; u < (int)kSeg; ++u) cpos[u] = (uint32_t)u < sn ? gpos[min(c0 + s0 + u, last)] : 0u;
+        for (int u = 0; u < (int)kSeg; ++u)
+            cw[u] = (uint32_t)u < sn ? gpos[min(c0 + s0 + u, last)] | ((c0 + card + s0 + u) << 16)
+                                     : mo | (L.creg << 16);
     }
 
 #pragma unroll 1
@@ -1055,11 +1059,11 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         for (int u = 0; u < kR; ++u) v[u] = rhs[oidx[u]];
 #pragma unroll
         for (int u = 0; u < kR; ++u) {
+            // past the chunk: 0, stored to the spare slot (which stays 0)
             const uint32_t k = lo + u;
-            if (k < hi) {
-                rs[k] = v[u];
-                part += (double)v[u];
-            }
+            v[u] = k < hi ? v[u] : 0.0f;
+            rs[k < hi ? k : mo] = v[u];
+            part += (double)v[u];
         }
         const double incl = wave_incl_sum_f64(part);
         const double excl = dpp_zero<0x138>(incl);  // the previous lane's; 0 in lane 0
@@ -1076,20 +1080,18 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
 #pragma unroll
         for (int u = 0; u < kR; ++u) {
             const uint32_t k = lo + u;
-            if (k < hi) {
-                P[k] = run;
-                run += (double)v[u];
-            }
+            P[k < hi ? k : mo + 1] = run;
+            run += (double)v[u];
         }
         if (tid == 0) P[mo] = total;
         // 3. per-card exclusive prefixes in strength order
         if (card < 52) {
             float x[kSeg];
 #pragma unroll
-            for (int u = 0; u < (int)kSeg; ++u) x[u] = rs[cpos[u]];
+            for (int u = 0; u < (int)kSeg; ++u) x[u] = rs[cw[u] & 0xFFFFu];
             double a = 0.0;
 #pragma unroll
-            for (int u = 0; u < (int)kSeg; ++u) a += (uint32_t)u < sn ? (double)x[u] : 0.0;
+            for (int u = 0; u < (int)kSeg; ++u) a += (double)x[u];
             // exclusive scan over the card's four segments (lanes 4*card..+3,
             // one quad: quad_perm broadcasts each lane's sum to the quad)
             const double a0 = dpp_zero<0x00>(a), a1 = dpp_zero<0x55>(a), a2 = dpp_zero<0xAA>(a), a3 = dpp_zero<0xFF>(a);
@@ -1097,11 +1099,10 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             if (seg > 0) y += a0;
             if (seg > 1) y += a1;
             if (seg > 2) y += a2;
-            double* cp = cpre + c0 + card + s0;
 #pragma unroll
             for (int u = 0; u < (int)kSeg; ++u) {
-                if ((uint32_t)u < sn) cp[u] = y;
-                y += (uint32_t)u < sn ? (double)x[u] : 0.0;
+                cpre[cw[u] >> 16] = y;
+                y += (double)x[u];
             }
             if (seg == 3) cpre[c0 + card + len] = ((a0 + a1) + a2) + a3;
         }
