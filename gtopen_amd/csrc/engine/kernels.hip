@@ -505,6 +505,8 @@ constexpr int kBatch = 4;
 // Hands per lane in one step of the up-sweep action kernels: every load
 // of all of them is issued before the first store.
 constexpr int kStepHands = 4;
+// Opponent-node sums (no arena traffic, fewer registers) take six.
+constexpr int kSumHands = 6;
 // The same in the down-sweep action kernel (two: 2.09 ms per f32 iteration,
 // three: 1.93, four: 1.96).
 constexpr int kDownHands = 3;
@@ -1619,16 +1621,16 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                 const float* rw[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
-                    float v[kStepHands][NA];
+                for (uint32_t i0 = lane; i0 < nhp; i0 += kSumHands * kWave) {
+                    float v[kSumHands][NA];
 #pragma unroll
-                    for (int h = 0; h < kStepHands; ++h) {
+                    for (int h = 0; h < kSumHands; ++h) {
                         const uint32_t ii = min(i0 + h * kWave, nhp - 1);
 #pragma unroll
                         for (int a = 0; a < NA; ++a) v[h][a] = rw[a][ii];
                     }
 #pragma unroll
-                    for (int h = 0; h < kStepHands; ++h) {
+                    for (int h = 0; h < kSumHands; ++h) {
                         const uint32_t i = i0 + h * kWave;
                         if (i >= nhp) break;
                         float acc = 0.0f;
