@@ -711,7 +711,7 @@ class Engine {
         uint64_t arenas = storage == GTO_STORAGE_F32 ? D * 8 : D * 4 + n * 16;
         if (storage == GTO_STORAGE_HALF) arenas += n * 16;                 // per-node maxima
         uint64_t reach = 0;
-        for (int q = 0; q < 2; ++q) reach += slots[q] * (nh[q] * 4 + 9);    // row, alive, mass, max
+        for (int q = 0; q < 2; ++q) reach += slots[q] * (nh[q] * 4 + 9);    // row, alive, max (and slack)
         const uint64_t cfv = n * (nhm * 4 + 4);                               // value rows and their maxima
         const uint64_t records = n * 96;                                      // work records, child lists
         const uint64_t keys = v.num_river_keys;
@@ -1095,9 +1095,7 @@ class Engine {
             }
             S_.reach[q] = (float*)dalloc(sizeof(float) * (size_t)plan_.num_slots[q] * plan_.nh[q]);
             S_.alive[q] = (uint8_t*)dalloc(plan_.num_slots[q]);
-            S_.mass[q] = (float*)dalloc(sizeof(float) * plan_.num_slots[q]);
             S_.rmax[q] = (float*)dalloc(sizeof(float) * plan_.num_slots[q]);
-            HIP_OK(hipMemset(S_.mass[q], 0, sizeof(float) * plan_.num_slots[q]));
             HIP_OK(hipMemset(S_.rmax[q], 0, sizeof(float) * plan_.num_slots[q]));
             HIP_OK(hipMemset(S_.alive[q], 0, plan_.num_slots[q]));
         }

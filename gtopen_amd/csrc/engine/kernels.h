@@ -147,7 +147,6 @@ struct DevState {
     float* Sm[2];         // half storage: per-node max strategy sum
     float* reach[2];      // [slot][nh_q]
     uint8_t* alive[2];    // [slot]: any reach > 0
-    float* mass[2];       // [slot]: sum of the reach row (a bound for value magnitudes)
     float* rmax[2];       // [slot]: max of the reach row
     float* vmax;          // [value row]: max |value| of the row (f16 engines; null otherwise)
     float* cfv;           // [cfv_slot][nh_max], traverser's values

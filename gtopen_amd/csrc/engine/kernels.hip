@@ -573,28 +573,22 @@ __device__ __forceinline__ float wave_max_f32(float x) {
 // Root: reach = range weights.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_root(DevTree T, DevState S, int mask) {
-    __shared__ float part[kWaves], pmax[kWaves];
+    __shared__ float pmax[kWaves];
     for (int q = 0; q < 2; ++q) {
         if (!((mask >> q) & 1)) continue;
         bool any = false;
-        float sum = 0.0f, mx = 0.0f;
+        float mx = 0.0f;
         for (uint32_t i = threadIdx.x; i < T.nh[q]; i += kBlock) {
             float w = T.weights[q][i];
             S.reach[q][i] = w;
             any |= w > 0.0f;
-            sum += w;
             mx = fmaxf(mx, w);
         }
-        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
         mx = wave_max_f32(mx);
-        if ((threadIdx.x & 63) == 0) {
-            part[threadIdx.x >> 6] = sum;
-            pmax[threadIdx.x >> 6] = mx;
-        }
+        if ((threadIdx.x & 63) == 0) pmax[threadIdx.x >> 6] = mx;
         any = __syncthreads_or(any);
         if (threadIdx.x == 0) {
             S.alive[q][0] = any ? 1 : 0;
-            S.mass[q][0] = ((part[0] + part[1]) + part[2]) + part[3];
             S.rmax[q][0] = fmaxf(fmaxf(pmax[0], pmax[1]), fmaxf(pmax[2], pmax[3]));
         }
         __syncthreads();
@@ -630,18 +624,17 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
         }
         for (int a = lane; a < na; a += kWave) {
             S.alive[q][kid_own[a]] = 0;
-            S.mass[q][kid_own[a]] = 0.0f;
             S.rmax[q][kid_own[a]] = 0.0f;
         }
         return;
     }
     const float* __restrict__ pr = reach + (size_t)src * nh;
     uint64_t anym = 0;
-    // per-child reach sums (bounds for value magnitudes); the first
-    // kMaxUnroll children in registers, the rest summed from memory below
-    float msum[kMaxUnroll], mmax[kMaxUnroll];
+    // per-child reach maxima (the f16 scale bounds); the first kMaxUnroll
+    // children in registers, the rest from memory below
+    float mmax[kMaxUnroll];
 #pragma unroll
-    for (int a = 0; a < kMaxUnroll; ++a) msum[a] = mmax[a] = 0.0f;
+    for (int a = 0; a < kMaxUnroll; ++a) mmax[a] = 0.0f;
     if (r.lock_off >= 0) {
         const float* sig = T.lock_sigma + r.lock_off;
         for (uint32_t i = lane; i < nh; i += kWave) {
@@ -679,7 +672,6 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
                             float v = pv[h] * hs[h].sig[a];
                             dst[a][i] = v;
                             anym |= (uint64_t)(v > 0.0f) << a;
-                            msum[a] += v;
                             mmax[a] = fmaxf(mmax[a], v);
                         }
                     }
@@ -698,30 +690,22 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
             }
         });
     }
-    const bool fast = r.lock_off < 0 && na <= kMaxUnroll;  // msum filled in the loops above
+    const bool fast = r.lock_off < 0 && na <= kMaxUnroll;  // mmax filled in the loops above
     for (int a = 0; a < na; ++a) {
         const bool any = __ballot((anym >> a) & 1ull) != 0ull;
-        float m = 0.0f, mx = 0.0f;
+        float mx = 0.0f;
         if (fast) {
 #pragma unroll
-            for (int a2 = 0; a2 < kMaxUnroll; ++a2) {
-                m = a2 == a ? msum[a2] : m;
-                mx = a2 == a ? mmax[a2] : mx;
-            }
+            for (int a2 = 0; a2 < kMaxUnroll; ++a2) mx = a2 == a ? mmax[a2] : mx;
         } else {
             // this wavefront's own writes above: read back in order
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             const float* row = reach + (size_t)kid_own[a] * nh;
-            for (uint32_t i = lane; i < nh; i += kWave) {
-                m += row[i];
-                mx = fmaxf(mx, row[i]);
-            }
+            for (uint32_t i = lane; i < nh; i += kWave) mx = fmaxf(mx, row[i]);
         }
-        for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off);
         mx = wave_max_f32(mx);
         if (lane == 0) {
             S.alive[q][kid_own[a]] = any ? 1 : 0;
-            S.mass[q][kid_own[a]] = m;
             S.rmax[q][kid_own[a]] = mx;
         }
     }
@@ -745,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
         const bool alive = S.alive[q][pick(r.src, q)];
         const float* in = S.reach[q] + (size_t)pick(r.src, q) * nh;
         bool any = false;
-        float msum = 0.0f, mmx = 0.0f;
+        float mmx = 0.0f;
         for (uint32_t i0 = lane; i0 < nh; i0 += kBatch * kWave) {
             float v[kBatch];
             uint64_t m[kBatch];
@@ -762,16 +746,13 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
                 const float x = (!alive || (m[u] & cm)) ? 0.0f : v[u];
                 out[i] = x;
                 any |= x > 0.0f;
-                msum += x;
                 mmx = fmaxf(mmx, x);
             }
         }
         any = __ballot(any) != 0ull;
-        for (int off = 32; off > 0; off >>= 1) msum += __shfl_xor(msum, off);
         mmx = wave_max_f32(mmx);
         if (lane == 0) {
             S.alive[q][pick(r.dst, q)] = any ? 1 : 0;
-            S.mass[q][pick(r.dst, q)] = msum;
             S.rmax[q][pick(r.dst, q)] = mmx;
         }
     }

@@ -292,28 +292,6 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
             }
         }
     }
-    // value bounds: |cfv of any hand at n| <= vbound[n] * (opponent reach
-    // mass at n); terminals: the largest payoff; chance nodes: the largest
-    // child bound times (children / divisor), the most a hand can collect
-    std::vector<double> vbound(N, 0.0);
-    for (uint32_t l = L; l-- > 0;) {
-        const LevelLists& ll = pl.levels[l];
-        for (uint32_t n : ll.fold) vbound[n] = std::max(std::fabs(v.node_t_win[n]), std::fabs(v.node_t_lose[n]));
-        for (uint32_t n : ll.show)
-            vbound[n] = std::max({std::fabs(v.node_t_win[n]), std::fabs(v.node_t_lose[n]), std::fabs(v.node_t_tie[n])});
-        for (uint32_t n : ll.chance) {
-            double m = 0.0;
-            for (uint32_t t = 0; t < pl.ch_cnt[n]; ++t) m = std::max(m, vbound[pl.ch_child[pl.ch_off[n] + t]]);
-            vbound[n] = m * (double)pl.ch_cnt[n] / (double)(46 - (int)v.node_street[n]);
-        }
-        for (int q = 0; q < 2; ++q)
-            for (uint32_t n : ll.act[q]) {
-                double m = 0.0;
-                const uint32_t cs = v.node_children_start[n];
-                for (uint32_t a = 0; a < v.node_num_children[n]; ++a) m = std::max(m, vbound[v.children[cs + a]]);
-                vbound[n] = m;
-            }
-    }
     // work records
     for (uint32_t l = 0; l < L; ++l) {
         LevelLists& ll = pl.levels[l];
@@ -329,7 +307,6 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
                 r.na = v.node_num_children[n];
                 r.kid = (uint32_t)pl.kids.size();
                 r.cw = cw[n];
-                r.vb = (float)vbound[n];
                 pl.max_actions = std::max(pl.max_actions, r.na);
                 const uint32_t cs = v.node_children_start[n];
                 for (uint32_t a = 0; a < r.na; ++a) pl.kids.push_back(pl.cfv_slot[v.children[cs + a]]);

@@ -19,7 +19,7 @@ struct ActRec {
     uint32_t na;        // number of actions
     uint32_t kid;       // index into the kids array
     float cw;           // product of the chance probabilities above the node
-    float vb;           // |value of any hand at the node| <= vb * (opponent reach mass)
+    uint32_t pad;       // keeps the record at 48 bytes
 };
 static_assert(sizeof(ActRec) == 48, "ActRec layout");
 
