@@ -65,11 +65,11 @@ struct DevTree {
 
 // LDS layout of the showdown kernel for one board (byte offsets): the
 // per-card offsets stay for a run of nodes (the other static tables are
-// held in registers); the per-node rows (strength-order prefix P, per-card
-// prefixes, reach in strength order; the hand-order reach row is staged in
-// the per-card region first) are rebuilt per node.
+// held in registers); the per-node rows (the hand-order reach row, the
+// strength-order reach and its prefix P, per-card prefixes, the value row)
+// are rebuilt per node.
 struct ShowLds {
-    uint32_t P, cpre, creg, wsum, rs, coff, nodes, O, bytes;
+    uint32_t P, cpre, creg, wsum, rs, coff, nodes, O, rhs, bytes;
 };
 // per node of a run, staged once: value row, opponent reach slot, live
 // flag, win, lose, tie
@@ -80,18 +80,19 @@ struct ShowNode {
 // mo: the opponent's hands on the board; nho / nhp: all hands of each player
 __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, uint32_t nho) {
     ShowLds L;
-    const uint32_t creg = (2 * mo + 52) > ((nho + 1) / 2) ? (2 * mo + 52) : ((nho + 1) / 2);
+    const uint32_t creg = 2 * mo + 52;
     // the last slot of P, cpre and rs is a spare: padded lanes store there
     // (P, cpre) or read a zero there (rs) instead of branching
     L.P = 0;                                    // (mo + 2) doubles
     L.cpre = L.P + 8 * (mo + 2);                // creg + 1 doubles
     L.creg = creg;
-    L.wsum = L.cpre + 8 * (creg + 1);           // 4 doubles (per-wave scan totals)
-    L.rs = L.wsum + 32;                         // mo + 1 floats: reach in strength order, then 0
+    L.wsum = L.cpre + 8 * (creg + 1);           // per wave: scan total (double), row max (float)
+    L.rs = L.wsum + 48;                         // mo + 1 floats: reach in strength order, then 0
     L.coff = L.rs + 4 * (mo + 1);               // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
     L.O = L.nodes + 16 * sizeof(ShowNode);      // nhp floats: the value row in hand order
-    L.bytes = (L.O + 4 * nhp + 15) & ~15u;
+    L.rhs = L.O + 4 * nhp;                      // nho floats: the opponent's reach row in hand order
+    L.bytes = (L.rhs + 4 * nho + 15) & ~15u;
     return L;
 }
 

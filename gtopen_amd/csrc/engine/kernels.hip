@@ -934,9 +934,10 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     char* base = (char*)lds;
     double* P = (double*)(base + L.P);
     double* cpre = (double*)(base + L.cpre);
-    double* wsum = (double*)(base + L.wsum);
+    double* wsum = (double*)(base + L.wsum);   // per wave: prefix scan total
+    float* wmax = (float*)(wsum + kWaves);      // per wave: max |value| of the row
     float* rs = (float*)(base + L.rs);
-    float* rhs = (float*)cpre;
+    float* rhs = (float*)(base + L.rhs);
     uint32_t* coff = (uint32_t*)(base + L.coff);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
     float* O = (float*)(base + L.O);
@@ -1013,6 +1014,15 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
                                      : mo | (L.creg << 16);
     }
 
+    // the last evaluated node's row, staged in O, waiting for a barrier
+    bool pending = false;
+    uint32_t pend_cfv = 0;
+    auto flush = [&]() {
+        float* __restrict__ po = S.cfv + (size_t)pend_cfv * T.nh_max;
+        for (uint32_t i = tid; i < nhp; i += kBlock) po[i] = O[i];
+        if (S.vmax && tid == 0) S.vmax[pend_cfv] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        pending = false;
+    };
 #pragma unroll 1
     for (uint32_t n = 0; n < cnt; ++n) {
         const ShowNode nd = nodes[n];
@@ -1030,11 +1040,17 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             for (int u = 0; u < kR; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
         }
         if (!live) {  // no reach, or no opponent hand on this board: all 0
+            if (pending) {
+                __syncthreads();
+                flush();
+            }
             for (uint32_t i = tid; i < nhp; i += kBlock) out[i] = 0.0f;
             if (S.vmax && tid == 0) S.vmax[nd.cfv] = 0.0f;
             continue;
         }
         __syncthreads();
+        // the previous node's row, complete since every thread passed the barrier
+        if (pending) flush();
         // 2. strength order and the chunk sums of its exclusive prefix
         float v[kR];
         double part = 0.0;
@@ -1117,12 +1133,17 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         }
         if (S.vmax) {
             vm = wave_max_f32(vm);
-            if (lane == 0) wsum[wave] = (double)vm;
+            if (lane == 0) wmax[wave] = vm;
         }
-        __syncthreads();  // this node's LDS reads complete before the next node's writes
-        for (uint32_t i = tid; i < nhp; i += kBlock) out[i] = O[i];
-        if (S.vmax && tid == 0)
-            S.vmax[nd.cfv] = (float)fmax(fmax(wsum[0], wsum[1]), fmax(wsum[2], wsum[3]));
+        // the row goes out after the next barrier (the next node's first,
+        // or the one below): the next node's writes before that barrier
+        // touch only its hand-order row
+        pending = true;
+        pend_cfv = nd.cfv;
+    }
+    if (pending) {
+        __syncthreads();
+        flush();
     }
 }
 
