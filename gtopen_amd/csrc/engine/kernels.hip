@@ -900,8 +900,9 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 // Up sweep, showdown terminals (game.rs:501-577), all levels in one launch:
 // value = win * L + lose * H + tie * (V - L - H) with L/H/V the opponent
 // reach strictly weaker / strictly stronger / compatible, blockers removed.
-// One workgroup per run of nodes sharing a river board: the board's tables
-// are staged in LDS once, then per node (all 256 lanes on every phase):
+// One workgroup per run of nodes sharing a river board: the board's static
+// tables are read once per run (into registers; the per-card offsets into
+// LDS), then per node (all 256 lanes on every phase):
 //   1. the opponent's reach row (hand order) into LDS; the next node's row
 //      is loaded meanwhile,
 //   2. reach in strength order + its exclusive prefix P (f64, per-thread
@@ -910,9 +911,12 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 //      holding it, in strength order (f64; four lanes per card, each a
 //      quarter of the card's list, read from the strength-order row at
 //      positions held in registers for the run),
-//   4. per hand, O(1) lookups: L = P[lb] - (card prefixes at lb),
+//   4. per hand (the traverser's hands in strength order), O(1) lookups:
+//      L = P[lb] - (card prefixes at lb),
 //      H = (total - P[ub]) - (card totals - card prefixes at ub),
-//      V = total - card totals + identical combo.
+//      V = total - card totals + identical combo; the row is assembled in
+//      LDS in hand order and written out after the next node's first
+//      barrier.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
 // kR: hand rows per thread, kR * kBlock >= either player's hand count (also
