@@ -115,75 +115,78 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
                 built[t] = pool.submit(Spot, flops[t][0])
         return built.pop(j).result()
 
-    while k < len(flops):
-        # waves of even size: the memory budget sets how many waves the
-        # remaining flops need (from the mean estimate so far), each wave
-        # takes its share, and the budget is never exceeded (at least one
-        # flop per wave); a short last wave would leave the device idle
-        if pending is None:
-            pending = (flops[k][0], flops[k][1], spot_of(k))
-            needs.append(int(L.lib().gto_hip_estimate_vram(pending[2].view_ptr, st)))
-        remaining = len(flops) - k
-        mean = sum(needs) / len(needs)
-        n_waves = max(1, int(np.ceil(mean * remaining / budget)), int(np.ceil(remaining / max_batch)))
-        cap = int(np.ceil(remaining / n_waves))
-        wave, used = [], 0
-        while k < len(flops) and len(wave) < cap:
+    try:
+        while k < len(flops):
+            # waves of even size: the memory budget sets how many waves the
+            # remaining flops need (from the mean estimate so far), each wave
+            # takes its share, and the budget is never exceeded (at least one
+            # flop per wave); a short last wave would leave the device idle
             if pending is None:
                 pending = (flops[k][0], flops[k][1], spot_of(k))
                 needs.append(int(L.lib().gto_hip_estimate_vram(pending[2].view_ptr, st)))
-            need = needs[-1]
-            if wave and used + need > budget:
-                break
-            wave.append(pending)
-            pending = None
-            used += need
-            k += 1
-        waves += 1
-        tw = time.time()
-        # the wave's engines are created on the same threads (plan building
-        # and uploads of independent spots)
-        gpus = list(pool.map(lambda w: GpuSolver(w[2], storage=storage, device=device), wave))
-        active = list(range(len(wave)))
-        batch = None
-        tuned = False
-        while active:
-            if batch is None:
-                batch = Batch([gpus[i] for i in active])
-                if autotune > 0 and not tuned and gpus[active[0]].iteration + tune_cost <= max_iterations:
-                    sched, ms = batch.autotune(autotune)
-                    schedules.append(dict(sched, wave=waves, ms_per_iteration=ms))
-                tuned = True
-            it = gpus[active[0]].iteration
-            step = min(check_every - it % check_every, max_iterations - it)
-            batch.iterate(step)
-            it += step
-            done = []
-            # the engines' checks run concurrently (each engine has its own
-            # stream; a small tree's pass alone leaves most of the device idle)
-            pcts = list(pool.map(lambda i: gpus[i].exploitability() / wave[i][2].starting_pot * 100.0, active))
-            fin = [i for i, pct in zip(active, pcts) if pct <= target or it >= max_iterations]
-            stats = dict(zip(fin, pool.map(lambda i: _root_stats(gpus[i], wave[i][2]), fin)))
-            for i, pct in zip(active, pcts):
-                cfg, weight, spot = wave[i]
-                if i in stats:
-                    players, root, vs_check = stats[i]
-                    rows.append({"board": cfg.board, "weight": weight, "iterations": it, "exploit_pct": pct,
-                                 "seconds": time.time() - tw, "nodes": spot.num_nodes, "players": players,
-                                 "root": root, "vs_check": vs_check})
-                    done.append(i)
-                    if progress:
-                        progress(rows[-1])
-                    if len(rows) % 8 == 0:
-                        _write(path, meta, rows, False)
-            if done:
-                batch.close()
-                batch = None
-                for i in done:
-                    gpus[i].close()
-                active = [i for i in active if i not in done]
-        del gpus
-    pool.shutdown()
+            remaining = len(flops) - k
+            mean = sum(needs) / len(needs)
+            n_waves = max(1, int(np.ceil(mean * remaining / budget)), int(np.ceil(remaining / max_batch)))
+            cap = int(np.ceil(remaining / n_waves))
+            wave, used = [], 0
+            while k < len(flops) and len(wave) < cap:
+                if pending is None:
+                    pending = (flops[k][0], flops[k][1], spot_of(k))
+                    needs.append(int(L.lib().gto_hip_estimate_vram(pending[2].view_ptr, st)))
+                need = needs[-1]
+                if wave and used + need > budget:
+                    break
+                wave.append(pending)
+                pending = None
+                used += need
+                k += 1
+            waves += 1
+            tw = time.time()
+            # the wave's engines are created on the same threads (plan building
+            # and uploads of independent spots)
+            gpus = list(pool.map(lambda w: GpuSolver(w[2], storage=storage, device=device), wave))
+            active = list(range(len(wave)))
+            batch = None
+            tuned = False
+            while active:
+                if batch is None:
+                    batch = Batch([gpus[i] for i in active])
+                    if autotune > 0 and not tuned and gpus[active[0]].iteration + tune_cost <= max_iterations:
+                        sched, ms = batch.autotune(autotune)
+                        schedules.append(dict(sched, wave=waves, ms_per_iteration=ms))
+                    tuned = True
+                it = gpus[active[0]].iteration
+                step = min(check_every - it % check_every, max_iterations - it)
+                batch.iterate(step)
+                it += step
+                done = []
+                # the engines' checks run concurrently (each engine has its own
+                # stream; a small tree's pass alone leaves most of the device idle)
+                pcts = list(pool.map(lambda i: gpus[i].exploitability() / wave[i][2].starting_pot * 100.0, active))
+                fin = [i for i, pct in zip(active, pcts) if pct <= target or it >= max_iterations]
+                stats = dict(zip(fin, pool.map(lambda i: _root_stats(gpus[i], wave[i][2]), fin)))
+                for i, pct in zip(active, pcts):
+                    cfg, weight, spot = wave[i]
+                    if i in stats:
+                        players, root, vs_check = stats[i]
+                        rows.append({"board": cfg.board, "weight": weight, "iterations": it, "exploit_pct": pct,
+                                     "seconds": time.time() - tw, "nodes": spot.num_nodes, "players": players,
+                                     "root": root, "vs_check": vs_check})
+                        done.append(i)
+                        if progress:
+                            progress(rows[-1])
+                        if len(rows) % 8 == 0:
+                            _write(path, meta, rows, False)
+                if done:
+                    batch.close()
+                    batch = None
+                    for i in done:
+                        gpus[i].close()
+                    active = [i for i in active if i not in done]
+            del gpus
+    finally:
+        # queued spot builds are dropped if a wave fails
+        pool.shutdown(wait=True, cancel_futures=True)
     elapsed = time.time() - t0
     meta.update(seconds=elapsed, waves=waves, flops_per_hour=len(rows) / elapsed * 3600.0 if elapsed > 0 else 0.0,
                 schedules=schedules)
