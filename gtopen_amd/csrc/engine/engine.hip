@@ -92,6 +92,10 @@ class Engine {
         if (pred_ && compressed_) throw std::runtime_error("PCFR+ runs with f32 arenas on the GPU");
         HIP_OK(hipSetDevice(o.device));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
+        lstream_ = stream_;
         plan_ = build_plan(v, o.isomorphism != 0);
         num_nodes_ = v.num_nodes;
         data_size_[0] = v.data_size[0];
@@ -117,6 +121,9 @@ class Engine {
         drop_graph();
         for (void* p : allocs_) (void)hipFree(p);
         if (stream_) (void)hipStreamDestroy(stream_);
+        if (aux_) (void)hipStreamDestroy(aux_);
+        if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+        if (join_ev_) (void)hipEventDestroy(join_ev_);
         for (auto& e : evs_) (void)hipEventDestroy(e);
     }
 
@@ -908,6 +915,11 @@ class Engine {
     std::vector<uint8_t> node_kind_, node_player_, node_na_;
     std::vector<uint64_t> node_off_;
     hipStream_t stream_ = nullptr;
+    // a second stream for the fold terminals beside the showdown launch,
+    // forked from and joined back into stream_ by events (captured graphs
+    // get two parallel branches); lstream_ is the stream launches go to
+    hipStream_t aux_ = nullptr, lstream_ = nullptr;
+    hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
     Plan plan_;
     uint32_t num_nodes_ = 0;
     uint64_t data_size_[2] = {0, 0};
@@ -1152,7 +1164,7 @@ class Engine {
             e0 = event();
             HIP_OK(hipEventRecord(e0, stream_));
         }
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream_, args...);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, lstream_, args...);
         HIP_OK(hipGetLastError());
         if (prof_) {
             e1 = event();
@@ -1222,13 +1234,28 @@ class Engine {
                      wave_grid(n_show_chunks_mma_, 1), kMmaBlock, show_mma_lds_, T_, S_, (const ShowRec*)show_all_,
                      (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap, show_mma_staged_ ? 1 : 0);
         if (!sched_.per_level) {
-            // terminals of every level first: they read only the down pass
-            if (n_fold_all_)
+            // terminals of every level first: they read only the down pass.
+            // Fold and showdown are independent: the fold launch runs on the
+            // second stream beside the showdown (not while kernels are timed
+            // one by one)
+            const bool fold = n_fold_all_ != 0, show = !mma && n_show_chunks_[cap_] != 0;
+            const bool fork = fold && show && !prof_;
+            if (fork) {
+                HIP_OK(hipEventRecord(fork_ev_, stream_));
+                HIP_OK(hipStreamWaitEvent(aux_, fork_ev_, 0));
+                lstream_ = aux_;
+            }
+            if (fold)
                 launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
                        n_fold_all_, p, swap);
-            if (!mma && n_show_chunks_[cap_])
+            lstream_ = stream_;
+            if (show)
                 launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
                        (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_], p, swap);
+            if (fork) {
+                HIP_OK(hipEventRecord(join_ev_, aux_));
+                HIP_OK(hipStreamWaitEvent(stream_, join_ev_, 0));
+            }
         }
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
