@@ -1245,9 +1245,14 @@ class Engine {
                 HIP_OK(hipStreamWaitEvent(aux_, fork_ev_, 0));
                 lstream_ = aux_;
             }
-            if (fold)
-                launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_, (const FoldRec*)fold_all_,
-                       n_fold_all_, p, swap);
+            try {
+                if (fold)
+                    launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_,
+                           (const FoldRec*)fold_all_, n_fold_all_, p, swap);
+            } catch (...) {
+                lstream_ = stream_;
+                throw;
+            }
             lstream_ = stream_;
             if (show)
                 launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
