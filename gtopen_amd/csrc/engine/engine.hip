@@ -915,6 +915,7 @@ class Engine {
     std::vector<uint8_t> node_kind_, node_player_, node_na_;
     std::vector<uint64_t> node_off_;
     hipStream_t stream_ = nullptr;
+    uint32_t max_card_list_ = 0;  // longest per-card opponent list on any river board
     // a second stream for the fold terminals beside the showdown launch,
     // forked from and joined back into stream_ by events (captured graphs
     // get two parallel branches); lstream_ is the stream launches go to
@@ -1022,9 +1023,11 @@ class Engine {
             size_t need = 0;
             for (uint32_t b = 0; b < plan_.num_boards; ++b) {
                 const uint32_t* cf = &plan_.sd_coff[o][(size_t)b * 53];
-                for (int c = 0; c < 52; ++c)
+                for (int c = 0; c < 52; ++c) {
                     if (cf[c + 1] - cf[c] > (uint32_t)kMaxCardList)
                         throw std::runtime_error("a card appears in more opponent hands than the showdown kernel holds");
+                    max_card_list_ = std::max(max_card_list_, cf[c + 1] - cf[c]);
+                }
                 need = std::max<size_t>(need, show_lds_layout(plan_.sd_cnt[o][b], plan_.nh[p], plan_.nh[o]).bytes);
             }
             show_lds_[p] = need;
@@ -1033,7 +1036,8 @@ class Engine {
         size_t lds_max = std::max(std::max(show_lds_[0], show_lds_[1]), std::max(fold_lds_[0], fold_lds_[1]));
         if (lds_max > kLdsBudget)
             throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
-        for (const void* f : {(const void*)k_up_show<3>, (const void*)k_up_show<kRowPer>})
+        for (const void* f : {(const void*)k_up_show<3, kShortSeg>, (const void*)k_up_show<3, kMaxCardList / 4>,
+                              (const void*)k_up_show<kRowPer, kMaxCardList / 4>})
             HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)std::max(show_lds_[0], show_lds_[1])));
         HIP_OK(hipFuncSetAttribute((const void*)k_up_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1224,8 +1228,11 @@ class Engine {
     // average-strategy values and the game value always take the exact
     // showdown, so exploitability judges every form with the same arithmetic.
     // the exact showdown kernel sized to the larger hand count (3 rows per
-    // thread cover up to 768 hands)
-    auto show_kernel(int) const { return std::max(plan_.nh[0], plan_.nh[1]) <= 3u * kBlock ? k_up_show<3> : k_up_show<kRowPer>; }
+    // thread cover up to 768 hands) and to the longest per-card list
+    auto show_kernel(int) const {
+        if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4>;
+        return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg> : k_up_show<3, kMaxCardList / 4>;
+    }
 
     void launch_up(int p, int mode, int swap = 0) {
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);

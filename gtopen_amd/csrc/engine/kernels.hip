@@ -919,9 +919,11 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 //      barrier.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
+constexpr int kShortSeg = 9;      // card-list entries per lane when no list exceeds 36 (partial ranges)
 // kR: hand rows per thread, kR * kBlock >= either player's hand count (also
-// the strength-order chunk per thread in phase 2)
-template <int kR>
+// the strength-order chunk per thread in phase 2); kSeg: entries per lane
+// of a card's list (four lanes per card), 4 kSeg >= the longest list
+template <int kR, int kSeg>
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
                                                     int swap) {
@@ -996,13 +998,12 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     }
     // per-card segment of this thread (cards 0..51, four lanes each)
     const uint32_t card = tid >> 2, seg = tid & 3;
-    constexpr uint32_t kSeg = kMaxCardList / 4;
     uint32_t c0 = 0, len = 0, s0 = 0, sn = 0;
     if (card < 52) {
         c0 = coff[card];
         len = coff[card + 1] - c0;
-        s0 = min(seg * kSeg, len);
-        sn = min(kSeg, len - s0);
+        s0 = min(seg * (uint32_t)kSeg, len);
+        sn = min((uint32_t)kSeg, len - s0);
     }
     // this thread's segment of its card's list, static for the run: the
     // strength-order position of each entry and the cpre slot of its prefix
