@@ -186,6 +186,39 @@ def test_full_range_river_matches_oracle(built):
     _close_arenas(sg, sc, 1.0)
 
 
+def test_long_card_lists_with_few_hands_match_oracle(built):
+    """Ace-heavy ranges on a river board: few hands per player (172 / 316)
+    but up to 43 opponent hands holding one card, longer than the showdown
+    kernel's short card-list form takes (36), so the long form runs with
+    the few-hands row layout."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, SpotConfig, StreetSizing, TreeConfig
+
+    _say("case: river, A2+ vs A2+,Q2+ (long card lists, few hands), 30 iterations")
+    s = [StreetSizing(), StreetSizing(), StreetSizing.of("50 a", "2.5x")]
+    tree = TreeConfig(starting_pot=10.0, effective_stack=40.0, oop=s, ip=s, max_raises=1)
+    spot = Spot(SpotConfig("Kd8s5h2c2d", "A2+", "A2+,Q2+", tree))
+    assert max(spot.num_hands) <= 768
+    longest = 0
+    for q in range(2):
+        cnt = np.zeros(52, dtype=np.int64)
+        np.add.at(cnt, np.asarray(spot.hand_c1[q], dtype=np.int64), 1)
+        np.add.at(cnt, np.asarray(spot.hand_c2[q], dtype=np.int64), 1)
+        longest = max(longest, int(cnt.max()))
+    assert longest > 36
+    gpu, cpu = GpuSolver(spot), oracle.Oracle(spot)
+    for _ in range(3):
+        gpu.iterate(10)
+        cpu.iterate(10)
+        eg, ec = gpu.exploitability(), cpu.exploitability()
+        _say(f"  iter {gpu.iteration}: gpu {eg:.6f} cpu {ec:.6f}")
+        assert abs(eg - ec) / spot.starting_pot < POT_TOL
+    rg, sg = gpu.download_arenas()
+    rc, sc = cpu.arenas()
+    _close_arenas(rg, rc, spot.starting_pot)
+    _close_arenas(sg, sc, 1.0)
+
+
 def test_cfr_plus_matches_oracle(built):
     """CFR+ (cfr.rs:57-61: positive regrets kept, negative ones floored by a
     zero discount, strategy weights (t/(t+1))^2) on the GPU engine."""
