@@ -7,7 +7,7 @@ no CPU fallback.
 """
 from ._lib import GtoError, lib  # noqa: F401
 from .game_value import GameValue, game_value  # noqa: F401
-from .solver import GpuSolver, vram_info  # noqa: F401
+from .solver import GpuSolver, Progress, vram_info  # noqa: F401
 from .spot import (  # noqa: F401
     Spot,
     SpotConfig,

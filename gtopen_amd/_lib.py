@@ -102,6 +102,20 @@ class Schedule(ctypes.Structure):
 SCHEDULE_CANDIDATES = 6
 
 
+class RunOptions(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_uint32), ("target_exploit_pct", ctypes.c_double),
+                ("check_every", ctypes.c_uint32)]
+
+
+class Progress(ctypes.Structure):
+    _fields_ = [("iteration", ctypes.c_uint32), ("exploit_chips", ctypes.c_double),
+                ("exploit_pct_pot", ctypes.c_double), ("elapsed_secs", ctypes.c_double),
+                ("gpu_secs", ctypes.c_double)]
+
+
+PROGRESS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(Progress), ctypes.c_void_p)
+
+
 # (name, restype, argtypes) for every symbol declared in include/gtopen_amd.h
 SIGNATURES = [
     ("gto_last_error", ctypes.c_char_p, []),
@@ -155,8 +169,8 @@ SIGNATURES = [
     ("gto_hip_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("gto_hip_profile_kind_name", ctypes.c_char_p, [ctypes.c_int]),
     ("gto_hip_profile_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(HipProfile)]),
-    ("gto_hip_kernel_stats", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),
-    ("gto_hip_enable_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("gto_hip_run", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RunOptions), ctypes.POINTER(ctypes.c_int),
+                                   PROGRESS_FN, ctypes.c_void_p, ctypes.POINTER(Progress)]),
     ("gto_hip_reached_entries", ctypes.c_uint64, [ctypes.c_void_p]),
     ("gto_hip_estimate_vram", ctypes.c_uint64, [ctypes.POINTER(SpotView), ctypes.c_int]),
     ("gto_hip_vram_info", ctypes.c_int, [ctypes.c_int, c_u64p, c_u64p]),

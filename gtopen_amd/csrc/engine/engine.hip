@@ -8,6 +8,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
+#include <exception>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -26,11 +28,19 @@ namespace {
 
 using namespace gto;
 
-#define HIP_OK(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            throw std::runtime_error(std::string("hip: ") + hipGetErrorString(e_) + " in " #expr); \
+// A device allocation failed for lack of memory: reported as GTO_ERR_MEMORY,
+// so a caller can tell a VRAM shortfall from a device fault.
+struct DeviceOutOfMemory : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(expr)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ == hipErrorOutOfMemory)                                                                     \
+            throw DeviceOutOfMemory(std::string("hip: out of device memory in " #expr));                 \
+        if (e_ != hipSuccess)                                                                              \
+            throw std::runtime_error(std::string("hip: ") + hipGetErrorString(e_) + " in " #expr);     \
     } while (0)
 
 enum ProfKind { PK_ROOT = 0, PK_DOWN_ACTION, PK_DOWN_CHANCE, PK_UP_FOLD, PK_UP_SHOW, PK_UP_CHANCE, PK_UP_ACTION, PK_OTHER };
@@ -90,6 +100,11 @@ class Engine {
         half_ = o.storage == GTO_STORAGE_HALF;
         pred_ = o.algo == GTO_ALGO_PCFRPLUS;
         if (pred_ && compressed_) throw std::runtime_error("PCFR+ runs with f32 arenas on the GPU");
+        // fault injection: GTO_INJECT_OOM=1 reports a device allocation
+        // failure before the device is touched, so a host-only test can
+        // drive a caller's out-of-memory path
+        if (const char* f = std::getenv("GTO_INJECT_OOM"); f && f[0] == '1')
+            throw DeviceOutOfMemory("hip: out of device memory (injected by GTO_INJECT_OOM)");
         HIP_OK(hipSetDevice(o.device));
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
@@ -101,6 +116,7 @@ class Engine {
         data_size_[0] = v.data_size[0];
         data_size_[1] = v.data_size[1];
         rake_ = v.rake_pct;
+        starting_pot_ = v.starting_pot;
         node_kind_.assign(v.node_kind, v.node_kind + v.num_nodes);
         node_player_.assign(v.node_player, v.node_player + v.num_nodes);
         node_na_.assign(v.node_num_children, v.node_num_children + v.num_nodes);
@@ -182,6 +198,37 @@ class Engine {
             }
             iteration_ += 1;
         }
+    }
+
+    // Solver::run (cfr.rs:194-226): iterate until max_iterations, the stop
+    // flag or the target; exploitability every check_every iterations and at
+    // the last one. Iterations between checks are launched back to back and
+    // synchronised once (gpu_secs: launch to completion).
+    gto_progress run(const gto_run_options& o, const volatile int* stop, gto_progress_fn cb, void* user) {
+        using clk = std::chrono::steady_clock;
+        const auto t0 = clk::now();
+        const uint32_t every = std::max(1u, o.check_every);
+        double gpu_secs = 0.0;
+        gto_progress last{iteration_, NAN, NAN, 0.0, 0.0};
+        while (iteration_ < o.max_iterations && !(stop && *stop)) {
+            const auto t = clk::now();
+            do {
+                iterate(1);
+            } while (iteration_ % every != 0 && iteration_ < o.max_iterations && !(stop && *stop));
+            synchronize();
+            gpu_secs += std::chrono::duration<double>(clk::now() - t).count();
+            if (iteration_ % every != 0 && iteration_ < o.max_iterations) break;  // stopped between checks
+            const double e = exploitability();
+            last.iteration = iteration_;
+            last.exploit_chips = e;
+            last.exploit_pct_pot = e / starting_pot_ * 100.0;
+            last.elapsed_secs = std::chrono::duration<double>(clk::now() - t0).count();
+            last.gpu_secs = gpu_secs;
+            if (cb && cb(&last, user) == 0) break;
+            if (last.exploit_pct_pot <= o.target_exploit_pct) break;
+        }
+        last.gpu_secs = gpu_secs;  // the record of the last check, as the reference returns it
+        return last;
     }
 
     double exploitability() {
@@ -925,6 +972,7 @@ class Engine {
     uint32_t num_nodes_ = 0;
     uint64_t data_size_[2] = {0, 0};
     double rake_ = 0.0;
+    double starting_pot_ = 1.0;
     std::vector<float> weights_[2];
     std::vector<uint8_t> c1_[2], c2_[2];
     std::vector<uint32_t> same_[2];
@@ -1252,22 +1300,32 @@ class Engine {
                 HIP_OK(hipStreamWaitEvent(aux_, fork_ev_, 0));
                 lstream_ = aux_;
             }
+            // the branch is joined back whatever fails in between, so a
+            // capture in progress can end and later work on stream_ still
+            // orders after the fold
+            std::exception_ptr err;
             try {
                 if (fold)
                     launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_,
                            (const FoldRec*)fold_all_, n_fold_all_, p, swap);
             } catch (...) {
-                lstream_ = stream_;
-                throw;
+                err = std::current_exception();
             }
             lstream_ = stream_;
-            if (show)
-                launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
-                       (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_], p, swap);
+            if (!err) try {
+                    if (show)
+                        launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
+                               (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_],
+                               p, swap);
+                } catch (...) {
+                    err = std::current_exception();
+                }
             if (fork) {
-                HIP_OK(hipEventRecord(join_ev_, aux_));
-                HIP_OK(hipStreamWaitEvent(stream_, join_ev_, 0));
+                const hipError_t r1 = hipEventRecord(join_ev_, aux_);
+                const hipError_t r2 = r1 == hipSuccess ? hipStreamWaitEvent(stream_, join_ev_, 0) : r1;
+                if (!err) HIP_OK(r2);
             }
+            if (err) std::rethrow_exception(err);
         }
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
@@ -1388,6 +1446,9 @@ int guarded(F&& f) {
         return GTO_OK;
     } catch (const std::bad_alloc&) {
         set_last_error("out of host memory");
+        return GTO_ERR_MEMORY;
+    } catch (const DeviceOutOfMemory& e) {
+        set_last_error(e.what());
         return GTO_ERR_MEMORY;
     } catch (const std::exception& e) {
         set_last_error(e.what());
@@ -1727,12 +1788,17 @@ int gto_hip_profile_iterations(gto_hip* h, uint32_t n, gto_hip_profile* out) {
     return guarded([&] { h->e->profile(n, out); });
 }
 
-int gto_hip_kernel_stats(gto_hip*, double*, double*, double*) {
-    set_last_error("use gto_hip_profile_iterations");
-    return GTO_ERR_INVALID;
+int gto_hip_run(gto_hip* h, const gto_run_options* opts, const volatile int* stop, gto_progress_fn on_progress,
+                void* user, gto_progress* out) {
+    if (!h || !opts) {
+        set_last_error("gto_hip_run: null handle or options");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        const gto_progress p = h->e->run(*opts, stop, on_progress, user);
+        if (out) *out = p;
+    });
 }
-
-int gto_hip_enable_timing(gto_hip*, int) { return GTO_OK; }
 
 uint64_t gto_hip_reached_entries(const gto_hip* h) { return h->e->reached_entries(); }
 

@@ -10,9 +10,12 @@ Paths are lists of steps in the reference's serde form,
 Strategies come from the device strategy arena (gto_hip_solved_strategy),
 per-node best-response and average-strategy values from one device sweep
 each (gto_hip_node_values); walking, lock construction and view assembly
-run on the host, as in the reference. The engine solves every runout
-explicitly (no suit-isomorphism folding), so canonical_path is the identity
-and browsed combos need no remapping.
+run on the host, as in the reference. Under suit isomorphism (the
+engine's default) CFR traverses one chance branch per suit orbit:
+canonical_path maps a browsed path onto the traversed branch
+(query.rs:643-700) and views of other branches are read through the
+composed suit map; with isomorphism off, or a spot without symmetries, it
+is the identity.
 """
 import numpy as np
 

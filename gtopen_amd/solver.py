@@ -3,11 +3,26 @@
 sync (download) / update_locks / synchronize / iteration, over the C ABI.
 """
 import ctypes
+from typing import NamedTuple
 
 import numpy as np
 
 from . import _lib as L
 from .query import QueryMixin
+
+
+class Progress(NamedTuple):
+    """Progress (cfr.rs:91-97), plus the seconds spent in iterations."""
+    iteration: int
+    exploit_chips: float
+    exploit_pct_pot: float
+    elapsed_secs: float
+    gpu_secs: float
+
+    @staticmethod
+    def of(p):
+        return Progress(int(p.iteration), float(p.exploit_chips), float(p.exploit_pct_pot), float(p.elapsed_secs),
+                        float(p.gpu_secs))
 
 
 class GpuSolver(QueryMixin):
@@ -87,6 +102,39 @@ class GpuSolver(QueryMixin):
 
     def iterate(self, n=1):
         L.check(L.lib().gto_hip_iterate(self._h, n), "gpu iterate")
+
+    def run(self, max_iterations=1000, target_exploit_pct=0.3, check_every=25, on_progress=None):
+        """Solver::run (cfr.rs:194-226; RunOptions defaults of cfr.rs:107-115):
+        iterate until max_iterations, the target (% of pot) or request_stop();
+        exploitability every check_every iterations and at the last one.
+        on_progress(Progress) runs after each check; returning False stops.
+        Returns the last check's Progress (iteration, exploit_chips,
+        exploit_pct_pot, elapsed_secs, gpu_secs)."""
+        opts = L.RunOptions(max_iterations, target_exploit_pct, check_every)
+        self._stop = ctypes.c_int(0)
+        err = []
+
+        def cb(p, _user):
+            if on_progress is None:
+                return 1
+            try:
+                return 0 if on_progress(Progress.of(p.contents)) is False else 1
+            except Exception as e:  # a Python error stops the run and is raised after it
+                err.append(e)
+                return 0
+
+        fn = L.PROGRESS_FN(cb)
+        out = L.Progress()
+        L.check(L.lib().gto_hip_run(self._h, ctypes.byref(opts), ctypes.byref(self._stop), fn, None,
+                                    ctypes.byref(out)), "gpu run")
+        if err:
+            raise err[0]
+        return Progress.of(out)
+
+    def request_stop(self):
+        """Stop a run() in progress (from another thread) after its current iteration."""
+        if getattr(self, "_stop", None) is not None:
+            self._stop.value = 1
 
     def exploitability(self):
         """Exploitability in chips (best_response.rs:255-287)."""

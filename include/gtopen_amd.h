@@ -34,6 +34,8 @@ extern "C" {
 #define GTO_OK 0
 #define GTO_ERR_INVALID (-1)
 #define GTO_ERR_DEVICE (-2)
+/* host or device memory exhausted (hipErrorOutOfMemory): a caller may fall
+ * back to the CPU engine, as the reference's VRAM gate does (main.rs:654-662) */
 #define GTO_ERR_MEMORY (-3)
 #define GTO_ERR_IO (-4)
 
@@ -365,12 +367,33 @@ typedef struct gto_hip_profile {
 const char* gto_hip_profile_kind_name(int k);
 int gto_hip_profile_iterations(gto_hip* h, uint32_t n, gto_hip_profile* out);
 
-/* Average device time (ms) of the action-node update kernel over the last
- * timed `gto_hip_iterate` calls, and its algorithmic bytes per launch
- * group; used for the roofline line of the benchmark. */
-int gto_hip_kernel_stats(gto_hip* h, double* update_ms_per_iter, double* arena_bytes_per_iter,
-                         double* total_ms_per_iter);
-int gto_hip_enable_timing(gto_hip* h, int on);
+/* Run to a target: Solver::run (crates/solver/src/cfr.rs:194-226) with
+ * RunOptions / Progress (cfr.rs:91-115). Iterates while iteration <
+ * max_iterations and *stop (may be NULL) is 0; after every iteration that
+ * is a multiple of check_every (or the last one) it measures exploitability,
+ * fills a progress record and calls on_progress (may be NULL); a callback
+ * returning 0 stops the run; the run also stops once exploit_pct_pot <=
+ * target_exploit_pct. *out (may be NULL) receives the last check's record,
+ * as Solver::run returns it (before any check: the starting iteration, NaN
+ * exploitability, 0 s).
+ * elapsed_secs is wall time from the call; gpu_secs the part spent in
+ * iterations (launch to completion), as solve-cli's progress line reports
+ * (solve_cli.rs:416-450). */
+typedef struct gto_run_options {
+    uint32_t max_iterations;
+    double target_exploit_pct; /* percent of the starting pot, e.g. 0.3 */
+    uint32_t check_every;      /* 0 is taken as 1 */
+} gto_run_options;
+typedef struct gto_progress {
+    uint32_t iteration;
+    double exploit_chips;
+    double exploit_pct_pot;
+    double elapsed_secs;
+    double gpu_secs;
+} gto_progress;
+typedef int (*gto_progress_fn)(const gto_progress* progress, void* user);
+int gto_hip_run(gto_hip* h, const gto_run_options* opts, const volatile int* stop, gto_progress_fn on_progress,
+                void* user, gto_progress* out);
 
 /* Algorithmic arena entries D = sum over reached action nodes of
  * na * nh_actor, the unit of the roofline bytes (20 B per entry per f32

@@ -42,3 +42,35 @@ def test_engine_reports_device_errors_without_crashing(built):
     spot = Spot(workloads.river_spot())
     with pytest.raises(GtoError):
         GpuSolver(spot)
+
+
+def test_out_of_device_memory_is_its_own_error(built, monkeypatch):
+    """A device allocation failure reports GTO_ERR_MEMORY (not the generic
+    device error), so a caller can fall back to the CPU engine as the
+    reference's VRAM gate does (main.rs:654-662). GTO_INJECT_OOM=1 makes
+    engine creation report one before the device is touched."""
+    import ctypes as C
+
+    from gtopen_amd import Spot, _lib as L, workloads
+
+    spot = Spot(workloads.river_spot())
+    monkeypatch.setenv("GTO_INJECT_OOM", "1")
+    h = C.c_void_p()
+    opts = L.HipOptions(L.STORAGE_F32, L.ALGO_DCFR, 1, 0, 1)
+    rc = L.lib().gto_hip_create(spot.view_ptr, C.byref(opts), C.byref(h))
+    assert rc == -3  # GTO_ERR_MEMORY
+    assert b"out of device memory" in L.lib().gto_last_error()
+    assert not h.value
+
+
+def test_run_rejects_null_arguments(built):
+    """gto_hip_run (Solver::run, cfr.rs:194-226) validates its arguments
+    before touching the device."""
+    import ctypes as C
+
+    from gtopen_amd import _lib as L
+
+    opts = L.RunOptions(10, 0.3, 5)
+    rc = L.lib().gto_hip_run(None, C.byref(opts), None, L.PROGRESS_FN(0), None, None)
+    assert rc == -1
+    assert b"null" in L.lib().gto_last_error()
