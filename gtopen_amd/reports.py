@@ -170,7 +170,8 @@ def run_report(flops, target=0.35, max_iterations=600, check_every=20, path=None
                     if i in stats:
                         players, root, vs_check = stats[i]
                         rows.append({"board": cfg.board, "weight": weight, "iterations": it, "exploit_pct": pct,
-                                     "seconds": time.time() - tw, "nodes": spot.num_nodes, "players": players,
+                                     "seconds": time.time() - tw, "nodes": spot.num_nodes,
+                                     "arena_entries": gpus[i].reached_entries, "players": players,
                                      "root": root, "vs_check": vs_check})
                         done.append(i)
                         if progress:

@@ -67,14 +67,21 @@ def test_game_value_and_range_gradient_turn_spot(built):
         assert s_root == pytest.approx(u, abs=RTOL * scale_u)
 
 
+@pytest.mark.timeout(900)
 def test_range_perturbations_against_fp64(built):
+    """The differentiable-game-value workload: the turn spot solved for 500
+    DCFR iterations (the trajectory's converged end), strategies frozen, 64
+    range perturbations N(0, 0.02^2) clipped to [0,1]. At every one, the
+    value against the FP64 oracle and both range gradients against the
+    oracle's CFR counterfactual values (dU/dw_p = cfv_p, dU/dw_o = -cfv_o
+    without rake); FP64 central differences on 4 of them."""
     from gtopen_amd import workloads
 
-    _say("case: turn spot, 64 range perturbations N(0, 0.02^2) clipped to [0,1]; FD on 4 of them")
-    spot, sigma, gv, o64 = _solved(workloads.turn_spot(), 100)
+    _say("case: turn spot (500 iterations), 64 range perturbations N(0, 0.02^2) clipped to [0,1]; FD on 4 of them")
+    spot, sigma, gv, o64 = _solved(workloads.turn_spot(), 500)
     rng = np.random.default_rng(2024)
     w = [spot.weights[q].astype(np.float64) for q in range(2)]
-    worst = 0.0
+    worst, worst_g = 0.0, 0.0
     for k in range(64):
         wk = [np.clip(w[q] + rng.normal(0.0, 0.02, size=w[q].shape), 0.0, 1.0).astype(np.float32) for q in range(2)]
         gv.set_weights(*wk)
@@ -82,9 +89,14 @@ def test_range_perturbations_against_fp64(built):
         u, gw, _ = gv.value_and_grad(0, grad_strategy=False)
         u_ref = o64.game_value(0, sigma)
         c0 = o64.root_cfv(0, mode=1)
+        c1 = o64.root_cfv(1, mode=1)
         scale = float(np.dot(wk[0].astype(np.float64), np.abs(c0)))
         worst = max(worst, abs(u - u_ref) / scale)
         assert abs(u - u_ref) <= RTOL * scale, f"perturbation {k}"
+        for g, ref in ((gw[0], c0), (gw[1], -c1)):
+            err = float(np.abs(g.astype(np.float64) - ref).max()) / float(np.abs(ref).max())
+            worst_g = max(worst_g, err)
+            assert err <= RTOL, f"perturbation {k}: gradient vs CFR values {err:.3e}"
         if k % 16 == 0:
             # FP64 central differences along random directions in both ranges
             for q in range(2):
@@ -104,7 +116,7 @@ def test_range_perturbations_against_fp64(built):
                 mag = float(np.dot(np.abs(dq), np.abs(gw[q].astype(np.float64))))
                 _say(f"  perturbation {k} range {q}: FD {fd:.6f} grad.d {lin:.6f}")
                 assert abs(fd - lin) <= RTOL * mag
-    _say(f"  worst relative value error over 64 perturbations: {worst:.3e}")
+    _say(f"  worst relative value error over 64 perturbations: {worst:.3e}; gradient vs CFR values {worst_g:.3e}")
 
 
 def test_strategy_gradient_against_fp64_differences(built):
