@@ -39,30 +39,62 @@ def _close_arenas_chunked(a, b, scale, rtol=2e-4, chunk=1 << 24):
         _say(f"  player {p}: worst |gpu-cpu| / tolerance {worst:.3f}")
 
 
+def _rel_l2(a, b, chunk=1 << 24):
+    """||a - b|| / ||b|| over a whole arena, in chunks."""
+    num = den = 0.0
+    for k in range(0, a.size, chunk):
+        x = a[k:k + chunk].astype(np.float64)
+        y = b[k:k + chunk].astype(np.float64)
+        num += float(np.dot(x - y, x - y))
+        den += float(np.dot(y, y))
+    return (num / den) ** 0.5 if den > 0 else num ** 0.5
+
+
 @pytest.mark.timeout(900)
 def test_flop_srp_f32_matches_oracle(built):
-    """The headline workload: 20 f32 iterations on the GPU and on the CPU
-    oracle, exploitability within 1e-4 of the pot at 10 and 20, arenas
-    entry by entry."""
+    """The headline workload: f32 iterations on the GPU and on the CPU
+    oracle. After 2 iterations the arenas agree entry by entry (the only
+    difference is the association of the f64 terminal sums); exploitability
+    within 1e-4 of the pot at 10 and 20. Over 20 iterations of a 1.4M-node
+    tree f32 rounding differences are amplified where regret matching
+    switches, so two f32 engines drift apart (here ~3e-3 relative L2 in the
+    regrets, with exploitability still within 4e-6 of the pot): at 20 the
+    GPU's arenas must be as close to the FP64 oracle's as the CPU f32
+    engine's are (relative L2, within a factor 2)."""
     import oracle
     from gtopen_amd import GpuSolver, Spot, workloads
 
     spot = Spot(workloads.flop_srp_spot())
     _say(f"case: flop SRP spot ({spot.num_nodes} nodes, hands {spot.num_hands}), f32, 20 iterations")
-    gpu, cpu = GpuSolver(spot), oracle.Oracle(spot)
+    gpu, cpu, ref = GpuSolver(spot), oracle.Oracle(spot), oracle.Oracle(spot, precision=64)
     pot = spot.starting_pot
-    for n in (10, 10):
-        gpu.iterate(n)
-        cpu.iterate(n)
+    gpu.iterate(2)
+    cpu.iterate(2)
+    rg, sg = gpu.download_arenas()
+    rc, sc = cpu.arenas()
+    _say("  arenas after 2 iterations, entry by entry:")
+    _close_arenas_chunked(rg, rc, pot)
+    _close_arenas_chunked(sg, sc, 1.0)
+    del rg, sg, rc, sc
+    for it in (10, 20):
+        gpu.iterate(it - gpu.iteration)
+        cpu.iterate(it - cpu.iteration)
         eg, ec = gpu.exploitability(), cpu.exploitability()
         _say(f"  iter {gpu.iteration}: gpu {eg:.6f} cpu {ec:.6f} chips ({abs(eg - ec) / pot:.2e} of the pot apart)")
         assert abs(eg - ec) / pot < POT_TOL
+    ref.iterate(20)
     rg, sg = gpu.download_arenas()
     gpu.close()
     rc, sc = cpu.arenas()
     del cpu
-    _close_arenas_chunked(rg, rc, pot)
-    _close_arenas_chunked(sg, sc, 1.0)
+    r64, s64 = ref.arenas_f64()
+    del ref
+    for name, a, b, c in (("regrets", rg, rc, r64), ("strategy sums", sg, sc, s64)):
+        for p in range(2):
+            d_gc, d_g64, d_c64 = _rel_l2(a[p], b[p]), _rel_l2(a[p], c[p]), _rel_l2(b[p], c[p])
+            _say(f"  {name} of player {p} after 20 iterations, relative L2: gpu-cpu {d_gc:.2e}, "
+                 f"gpu-fp64 {d_g64:.2e}, cpu-fp64 {d_c64:.2e}")
+            assert d_g64 <= 2.0 * d_c64 + 1e-6
 
 
 @pytest.mark.timeout(1200)
