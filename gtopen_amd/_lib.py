@@ -166,6 +166,8 @@ SIGNATURES = [
     ("gto_save_read_arenas", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_f32p, c_f32p, c_f32p, c_f32p]),
     ("gto_validate_locks", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_u64p, c_u64p, c_f32p]),
     ("gto_hip_game_value", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f64p, c_f32p, c_f32p, c_f32p, c_f32p]),
+    ("gto_hip_game_value_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f64p, c_f32p, c_f32p, c_f32p,
+                                                  c_f32p]),
     ("gto_hip_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("gto_hip_profile_kind_name", ctypes.c_char_p, [ctypes.c_int]),
     ("gto_hip_profile_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(HipProfile)]),

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -42,6 +43,33 @@ struct DeviceOutOfMemory : std::runtime_error {
         if (e_ != hipSuccess)                                                                              \
             throw std::runtime_error(std::string("hip: ") + hipGetErrorString(e_) + " in " #expr);     \
     } while (0)
+
+// Static device scratch of the allocation-free game value
+// (gto_hip_game_value_noalloc): one player's strategy gradient is staged
+// here before the copy to the caller. Every GPU the library runs on holds
+// its own copy (64 MiB) until the process ends; host code reaches a
+// device's copy only through the address hipGetSymbolAddress returns with
+// that device current (scratch_on), never through the array's name.
+constexpr uint64_t kScratchFloats = uint64_t(16) << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch` (the device must be current); looked
+// up once per device, at the first engine created on it.
+float* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device] = (float*)p;
+    }
+    return g_scratch_ptr[device];
+}
 
 enum ProfKind { PK_ROOT = 0, PK_DOWN_ACTION, PK_DOWN_CHANCE, PK_UP_FOLD, PK_UP_SHOW, PK_UP_CHANCE, PK_UP_ACTION, PK_OTHER };
 const char* kProfNames[GTO_PROF_KINDS] = {"root", "down_action", "down_chance", "up_fold",
@@ -127,6 +155,8 @@ class Engine {
             c2_[q].assign(v.hand_c2[q], v.hand_c2[q] + v.num_hands[q]);
             same_[q].assign(v.same_combo[q], v.same_combo[q] + v.num_hands[q]);
         }
+        scratch_ = scratch_on(o.device);
+        root_row_.assign(std::max<uint32_t>(plan_.nh_max, 1), 0.0f);
         upload_tree(v);
         alloc_state();
         reach_valid_[0] = reach_valid_[1] = false;
@@ -385,37 +415,56 @@ class Engine {
     // strategy, so dU/dw_p = cfv_p(root), dU/dw_o = the root values of o's
     // hands under p's payoffs, and dU/dsigma_q(n, a, i) = reach_q(n, i) *
     // chance weight * value of child a (k_grad_sigma).
-    void game_value(int p, double* U, float* gw[2], float* gs[2]) {
+    //
+    // noalloc: the strategy gradient is staged in the library's static
+    // device scratch (at most kScratchFloats entries per player, cleared by a
+    // fill kernel) instead of a buffer the engine allocates on first use; no
+    // device or host allocation happens in the call.
+    void game_value(int p, double* U, float* gw[2], float* gs[2], bool noalloc = false) {
         if (compressed_) throw std::runtime_error("game values need an engine with f32 arenas");
         // the strategy is an input per node; under suit isomorphism copied
         // branches would silently take their representatives' strategies
         if (iso_) throw std::runtime_error("game values need an engine created with isomorphism off");
         const int o = 1 - p;
+        if (noalloc)
+            for (int q = 0; q < 2; ++q)
+                if (gs[q] && data_size_[q] > kScratchFloats)
+                    throw std::runtime_error("gto_hip_game_value_noalloc: the strategy arena of player " +
+                                             std::to_string(q) + " has " + std::to_string(data_size_[q]) +
+                                             " entries, more than the " + std::to_string(kScratchFloats) +
+                                             " the static device scratch holds");
         launch_root(3);
         launch_down(3, kRawF32);
         reach_valid_[0] = reach_valid_[1] = false;
-        std::vector<float> c;
+        std::vector<float>& c = root_row_;  // sized at construction: no allocation here
         for (int pass = 0; pass < 2; ++pass) {
             const int t = pass == 0 ? p : o;  // traverser of this sweep
             if (pass == 1 && !gw[o] && !gs[o]) break;
             launch_up(t, 4, pass);
-            c.resize(plan_.nh[t]);
             HIP_OK(hipMemcpyAsync(c.data(), S_.cfv + (size_t)plan_.cfv_slot[0] * plan_.nh_max,
                                   sizeof(float) * plan_.nh[t], hipMemcpyDeviceToHost, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
-            if (pass == 0 && U) *U = dot(p, c);
-            if (gw[t]) std::copy(c.begin(), c.end(), gw[t]);
+            if (pass == 0 && U) *U = dot(p, c.data(), plan_.nh[p]);
+            if (gw[t]) std::copy(c.begin(), c.begin() + plan_.nh[t], gw[t]);
             if (gs[t] && data_size_[t]) {
-                if (!grad_buf_) grad_buf_ = (float*)dalloc(sizeof(float) * std::max(data_size_[0], data_size_[1]));
-                HIP_OK(hipMemsetAsync(grad_buf_, 0, sizeof(float) * data_size_[t], stream_));
+                float* buf;
+                if (noalloc) {
+                    buf = scratch_;
+                    const uint32_t blocks = (uint32_t)std::min<uint64_t>((data_size_[t] + kBlock - 1) / kBlock, 8192);
+                    launch(PK_OTHER, k_fill, blocks, 0, buf, (uint64_t)data_size_[t], 0.0f);
+                } else {
+                    if (!grad_buf_)
+                        grad_buf_ = (float*)dalloc(sizeof(float) * std::max(data_size_[0], data_size_[1]));
+                    HIP_OK(hipMemsetAsync(grad_buf_, 0, sizeof(float) * data_size_[t], stream_));
+                    buf = grad_buf_;
+                }
                 for (size_t l = 0; l < lv_.size(); ++l) {
                     const DevLevel& D = lv_[l];
                     if (!D.n_act[t]) continue;
                     launch(PK_OTHER, k_grad_sigma, wave_grid(D.n_act[t]), 0, T_, S_, (const ActRec*)D.act[t],
-                           D.n_act[t], t, grad_buf_);
+                           D.n_act[t], t, buf);
                 }
-                HIP_OK(hipMemcpyAsync(gs[t], grad_buf_, sizeof(float) * data_size_[t], hipMemcpyDeviceToHost,
-                                      stream_));
+                HIP_OK(hipMemcpyAsync(gs[t], buf, sizeof(float) * data_size_[t], hipMemcpyDeviceToHost, stream_));
                 HIP_OK(hipStreamSynchronize(stream_));
             }
         }
@@ -986,6 +1035,8 @@ class Engine {
     float* disc_buf_ = nullptr;
     float* lock_sigma_buf_ = nullptr;
     float* grad_buf_ = nullptr;
+    float* scratch_ = nullptr;        // this device's copy of the static scratch array
+    std::vector<float> root_row_;     // host copy of a root value row (nh_max floats)
     // all-level terminal lists
     FoldRec* fold_all_ = nullptr;
     uint32_t n_fold_all_ = 0;
@@ -1432,9 +1483,10 @@ class Engine {
         return total;
     }
 
-    double dot(int p, const std::vector<float>& c) const {
+    double dot(int p, const std::vector<float>& c) const { return dot(p, c.data(), c.size()); }
+    double dot(int p, const float* c, size_t n) const {
         double acc = 0.0;
-        for (size_t i = 0; i < c.size(); ++i) acc += (double)weights_[p][i] * (double)c[i];
+        for (size_t i = 0; i < n; ++i) acc += (double)weights_[p][i] * (double)c[i];
         return acc;
     }
 };
@@ -1745,6 +1797,19 @@ int gto_hip_game_value(gto_hip* h, int player, double* value, float* grad_w0, fl
         float* gw[2] = {grad_w0, grad_w1};
         float* gs[2] = {grad_s0, grad_s1};
         h->e->game_value(player, value, gw, gs);
+    });
+}
+
+int gto_hip_game_value_noalloc(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
+                               float* grad_s0, float* grad_s1) {
+    if (!h || player < 0 || player > 1) {
+        set_last_error("gto_hip_game_value_noalloc: null handle or bad player");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        float* gw[2] = {grad_w0, grad_w1};
+        float* gs[2] = {grad_s0, grad_s1};
+        h->e->game_value(player, value, gw, gs, true);
     });
 }
 

@@ -595,6 +595,12 @@ __global__ __launch_bounds__(kBlock) void k_root(DevTree T, DevState S, int mask
     }
 }
 
+// Fill n floats with v (clears device buffers where a memset node must not
+// be used: code a caller may capture into a graph).
+__global__ __launch_bounds__(kBlock) void k_fill(float* __restrict__ p, uint64_t n, float v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) p[i] = v;
+}
+
 // Advance the device iteration counter (discount table index).
 __global__ void k_begin_iteration(DevState S) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *S.iter += 1;

@@ -89,8 +89,10 @@ class GameValue:
         ptr = [None if s is None else s.ctypes.data_as(L.c_f32p) for s in keep]
         L.check(L.lib().gto_hip_upload_arenas(self.engine._h, None, None, ptr[0], ptr[1]), "upload strategy")
 
-    def value_and_grad(self, player, grad_weights=True, grad_strategy=True):
-        """(U, [dU/dw0, dU/dw1], [dU/dsigma0, dU/dsigma1]); skipped parts are None."""
+    def value_and_grad(self, player, grad_weights=True, grad_strategy=True, noalloc=False):
+        """(U, [dU/dw0, dU/dw1], [dU/dsigma0, dU/dsigma1]); skipped parts are None.
+        noalloc: through gto_hip_game_value_noalloc (no allocation in the
+        call; strategy arenas of at most 16M entries per player)."""
         sp = self.spot
         gw = [np.zeros(sp.num_hands[q], dtype=np.float32) if grad_weights else None for q in range(2)]
         gs = [np.zeros(sp.data_size[q], dtype=np.float32) if grad_strategy else None for q in range(2)]
@@ -100,8 +102,8 @@ class GameValue:
             return None if a is None else a.ctypes.data_as(L.c_f32p)
 
         L.check(
-            L.lib().gto_hip_game_value(self.engine._h, player, ctypes.byref(val), p(gw[0]), p(gw[1]), p(gs[0]),
-                                       p(gs[1])),
+            (L.lib().gto_hip_game_value_noalloc if noalloc else L.lib().gto_hip_game_value)(
+                self.engine._h, player, ctypes.byref(val), p(gw[0]), p(gw[1]), p(gs[0]), p(gs[1])),
             "game value",
         )
         return val.value, gw, gs

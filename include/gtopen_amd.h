@@ -347,6 +347,19 @@ int gto_hip_set_weights(gto_hip* h, const float* w0, const float* w1);
  * is an input). */
 int gto_hip_game_value(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
                        float* grad_s0, float* grad_s1);
+/* The same value and gradients with no allocation in the call (no
+ * counterpart in the reference): for callers that cannot allocate device
+ * memory — code under graph capture, tight serving loops over range
+ * perturbations, harnesses that forbid hipMalloc. gto_hip_game_value
+ * allocates its strategy-gradient staging buffer on first use; this entry
+ * stages it in a static device array of the library instead, 16777216
+ * floats (64 MiB) per GPU, held until the process ends. A strategy gradient
+ * requested for a player whose arena is larger is rejected with
+ * GTO_ERR_DEVICE and a message. Every engine on one GPU shares that array:
+ * calls that overlap on one GPU (engines used from several threads) race;
+ * consecutive calls are independent (the array is cleared where needed). */
+int gto_hip_game_value_noalloc(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
+                               float* grad_s0, float* grad_s1);
 
 /* Node locks: lock_off[num_nodes] (-1 = unlocked, else offset into sigma),
  * sigma = concatenated na*nh_actor blocks. Replaces update_locks /
