@@ -54,16 +54,25 @@ def cpu_info():
     return info
 
 
-def measured_traffic(args, kind):
-    """HBM bytes per iteration of the kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload (profiles/), if any."""
+def traffic_profile(args):
+    """The committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
+    workload (profiles/<config>_<storage>_traffic.json), if any."""
     path = os.path.join(ROOT, "profiles", f"{args.config}_{args.storage}_traffic.json")
-    if not os.path.exists(path):
+    return json.load(open(path)) if os.path.exists(path) else None
+
+
+def measured_traffic(args, kind):
+    """HBM bytes per iteration of the kernel kind from the traffic summary."""
+    prof = traffic_profile(args)
+    if prof is None:
         return None
-    prof = json.load(open(path))
     store = {"f32": 0, "compressed": 1, "half": 2}[args.storage]
-    key = {"up_action": "k_up_action<0, %d>" % store,
-           "down_action": "k_down_action<%d>" % {0: 0, 1: 2, 2: 5}[store]}.get(kind)
+    if kind == "river":
+        keys = [k for k in prof["kernels"] if k.startswith("k_river<") and k.endswith(f", {store}>")]
+        key = keys[0] if keys else None
+    else:
+        key = {"up_action": "k_up_action<0, %d>" % store,
+               "down_action": "k_down_action<%d>" % {0: 0, 1: 2, 2: 5}[store]}.get(kind)
     k = prof["kernels"].get(key)
     if not k:
         return None
@@ -99,7 +108,7 @@ def time_to_target(spot, storage, target, cap, showdown="exact", sched=None, alg
 
     g = GpuSolver(spot, storage=storage, showdown=showdown, algo=algo)
     if sched is not None:
-        g.set_schedule(sched["chosen"]["show_nodes"], sched["chosen"]["per_level"])
+        g.set_schedule(sched["chosen"]["show_nodes"], sched["chosen"]["per_level"], sched["chosen"]["river"])
     t0 = time.perf_counter()
     e = None
     while g.iteration < cap:
@@ -131,16 +140,21 @@ def gpu_rate(spot, steps, warmup, storage="f32", showdown="exact"):
     return dt * 1e3 / steps, steps / dt, D
 
 
-def roofline_of(prof, steps, D, eb, args=None):
+def roofline_of(prof, steps, D, Dr, eb, args=None):
     """Roofline of the dominant kernel kind of a profile: algorithmic arena
-    bytes of that kind per iteration over its summed launch time."""
+    bytes of that kind per iteration over its summed launch time. Dr: the
+    arena entries inside river subtrees, which the fused river kernel
+    streams when it runs (the level kernels then cover D - Dr)."""
     kinds = {k: v for k, v in prof.items() if v[1] > 0}
     dom = max(kinds, key=lambda k: kinds[k][0])
     dom_ms = kinds[dom][0] / steps
-    # regret/strategy arena entries D: the update kernel reads+writes R and S
-    # (16 B per f32 entry per iteration, both sweeps), the down sweep reads R
-    # once more (4 B): 20*D per iteration in total
-    alg = {"up_action": 4.0 * eb * D, "down_action": eb * D}.get(dom)
+    if "river" not in kinds:
+        Dr = 0
+    # regret/strategy arena entries: the update reads+writes R and S (16 B
+    # per f32 entry per iteration, both sweeps), the down sweep reads R once
+    # more (4 B): 20 B per entry per iteration in total, all of it in the
+    # fused river kernel for river entries
+    alg = {"up_action": 4.0 * eb * (D - Dr), "down_action": eb * (D - Dr), "river": 5.0 * eb * Dr}.get(dom)
     split = {k: round(v[0] / steps, 4) for k, v in kinds.items()}
     if alg is None:
         return None, split
@@ -206,7 +220,7 @@ def main():
     pot = spot.starting_pot
 
     gpu = GpuSolver(spot, storage=args.storage, showdown=args.showdown)
-    D = gpu.reached_entries
+    D, Dr = gpu.reached_entries, gpu.river_entries
     gpu.iterate(args.warmup)
     gpu.synchronize()
     log(f"bench: warmup {args.warmup} done")
@@ -214,7 +228,8 @@ def main():
     if args.autotune > 0:
         chosen, cand = gpu.autotune(args.autotune)
         sched = {"chosen": chosen, "candidates_ms_per_iter": [round(x, 3) for x in cand],
-                 "candidates": "show_nodes 16/8/4 x terminals all-levels-first / per-level"}
+                 "candidates": "level sweeps: show_nodes 16/8/4 x terminals all-levels-first / per-level; "
+                               "fused river sweep"}
         log(f"bench: autotuner chose {chosen}; candidates ms/iter {sched['candidates_ms_per_iter']}")
     t0 = time.perf_counter()
     gpu.iterate(args.steps)
@@ -230,10 +245,19 @@ def main():
     # number of steps (eager launches, one event pair per launch)
     prof = gpu.profile(args.steps)
     eb = 4.0 if args.storage == "f32" else 2.0  # bytes per arena entry
-    roofline, split = roofline_of(prof, args.steps, D, eb, args)
+    roofline, split = roofline_of(prof, args.steps, D, Dr, eb, args)
     log("bench: kernel split (ms/iter): " + ", ".join(f"{k} {v:.3f}" for k, v in split.items()))
     whole = 5.0 * eb * D / (ms_per_step * 1e-3) / 1e9
     gpu.close()
+    # HBM bytes of a whole iteration from the committed counter profile
+    tp = traffic_profile(args)
+    iteration_traffic = None
+    if tp is not None:
+        tb = sum(k["read_bytes_per_iter"] + k["write_bytes_per_iter"] for k in tp["kernels"].values())
+        iteration_traffic = {"bytes": round(tb), "algorithmic_bytes": 5.0 * eb * D,
+                             "ratio": round(tb / (5.0 * eb * D), 3),
+                             "source": f"profiles/{args.config}_{args.storage}_traffic.json "
+                                       "(all kernels of the profiled run, per iteration)"}
 
     judge = None
     if not args.no_target and not args.no_fp64:
@@ -340,13 +364,15 @@ def main():
                  + ("" if args.showdown == "exact" else f", {args.showdown} showdown products"),
         "data": "synthetic",
         "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
-                   "hands": list(spot.num_hands), "arena_entries_D": D, "parallelism": "single GPU"},
+                   "hands": list(spot.num_hands), "arena_entries_D": D, "river_arena_entries": Dr,
+                   "parallelism": "single GPU"},
         "sec_to_target": None if sec_to_target is None else round(sec_to_target, 3),
         "iters_to_target": iters_to_target,
         "target_pct_pot": args.target,
         "exploitability_pct_at_target": None if e_final is None else round(e_final, 4),
         "fp64_exploitability_pct": None if e64_final is None else round(e64_final, 4),
         "whole_iteration_arena_GBps": round(whole, 1),
+        "iteration_traffic": iteration_traffic,
         "kernel_ms_per_iter": split,
         "roofline": roofline,
         "cpu_baseline": cpu,

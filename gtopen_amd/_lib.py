@@ -38,7 +38,7 @@ def algo_code(name):
 
 
 STRICT, LENIENT_LOAD = 0, 1
-PROF_KINDS = 8
+PROF_KINDS = 9
 
 
 class SpotView(ctypes.Structure):
@@ -96,10 +96,10 @@ class HipProfile(ctypes.Structure):
 
 
 class Schedule(ctypes.Structure):
-    _fields_ = [("show_nodes", ctypes.c_uint32), ("per_level", ctypes.c_uint32)]
+    _fields_ = [("show_nodes", ctypes.c_uint32), ("per_level", ctypes.c_uint32), ("river", ctypes.c_uint32)]
 
 
-SCHEDULE_CANDIDATES = 6
+SCHEDULE_CANDIDATES = 7
 
 
 class RunOptions(ctypes.Structure):
@@ -174,6 +174,7 @@ SIGNATURES = [
     ("gto_hip_run", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RunOptions), ctypes.POINTER(ctypes.c_int),
                                    PROGRESS_FN, ctypes.c_void_p, ctypes.POINTER(Progress)]),
     ("gto_hip_reached_entries", ctypes.c_uint64, [ctypes.c_void_p]),
+    ("gto_hip_river_entries", ctypes.c_uint64, [ctypes.c_void_p]),
     ("gto_hip_estimate_vram", ctypes.c_uint64, [ctypes.POINTER(SpotView), ctypes.c_int]),
     ("gto_hip_vram_info", ctypes.c_int, [ctypes.c_int, c_u64p, c_u64p]),
 ]

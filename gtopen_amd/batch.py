@@ -28,7 +28,9 @@ class Batch:
         best = L.Schedule()
         ms = (ctypes.c_float * L.SCHEDULE_CANDIDATES)()
         L.check(L.lib().gto_batch_autotune(self._h, iters, ctypes.byref(best), ms), "batch autotune")
-        return {"show_nodes": best.show_nodes, "per_level": bool(best.per_level)}, [float(x) for x in ms]
+        from .solver import schedule_dict
+
+        return schedule_dict(best), [float(x) for x in ms]
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

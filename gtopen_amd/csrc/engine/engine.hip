@@ -71,9 +71,10 @@ float* scratch_on(int device) {
     return g_scratch_ptr[device];
 }
 
-enum ProfKind { PK_ROOT = 0, PK_DOWN_ACTION, PK_DOWN_CHANCE, PK_UP_FOLD, PK_UP_SHOW, PK_UP_CHANCE, PK_UP_ACTION, PK_OTHER };
-const char* kProfNames[GTO_PROF_KINDS] = {"root", "down_action", "down_chance", "up_fold",
-                                          "up_show", "up_chance", "up_action", "other"};
+enum ProfKind { PK_ROOT = 0, PK_DOWN_ACTION, PK_DOWN_CHANCE, PK_UP_FOLD, PK_UP_SHOW, PK_UP_CHANCE, PK_UP_ACTION, PK_RIVER,
+                PK_OTHER };
+const char* kProfNames[GTO_PROF_KINDS] = {"root", "down_action", "down_chance", "up_fold", "up_show",
+                                          "up_chance", "up_action", "river", "other"};
 
 struct DevLevel {
     ActRec* act[2] = {nullptr, nullptr};
@@ -88,6 +89,11 @@ struct DevLevel {
     uint32_t n_chance = 0;
     EdgeRec* edge = nullptr;
     uint32_t n_edge = 0;
+    // the same lists without river-subtree nodes (fused river sweeps)
+    ActRec* act_up[2] = {nullptr, nullptr};
+    uint32_t n_act_up[2] = {0, 0};
+    EdgeRec* edge_up = nullptr;
+    uint32_t n_edge_up = 0;
 };
 
 // IEEE half <-> float on the host (round to nearest even)
@@ -178,7 +184,7 @@ class Engine {
     // --- batch support: several engines' iterations in one parent graph ---
     // Steady state: the regrets-changed reach of player 1 is the only stale
     // one, which is what one captured iteration assumes.
-    bool steady() const { return reach_valid_[0] && !reach_valid_[1] && !prof_; }
+    bool steady() const { return reach_valid_[0] && !reach_valid_[1] && !prof_ && !(rows_stale_ && !fused()); }
     hipStream_t stream() const { return stream_; }
     int device() const { return opts_.device; }
     // bumped whenever a captured iteration's kernel arguments go stale
@@ -217,8 +223,7 @@ class Engine {
     void iterate(uint32_t n) {
         ensure_disc(iteration_ + n + 1);
         for (uint32_t k = 0; k < n; ++k) {
-            bool steady = reach_valid_[0] && !reach_valid_[1];
-            if (opts_.use_graphs && steady && !prof_) {
+            if (opts_.use_graphs && steady()) {
                 if (!graph_) capture_graph();
                 HIP_OK(hipGraphLaunch(graph_, stream_));
                 reach_valid_[0] = true;
@@ -617,6 +622,11 @@ class Engine {
     void update_locks(const int64_t* lock_off, const float* sigma, uint64_t n_sigma) {
         HIP_OK(hipStreamSynchronize(stream_));
         drop_graph();
+        // locked nodes play their fixed strategy in the level sweeps; the
+        // fused river sweep runs only while no node is locked
+        locked_ = false;
+        if (lock_off)
+            for (uint32_t n = 0; n < num_nodes_; ++n) locked_ = locked_ || lock_off[n] >= 0;
         // host copies: symmetrize copies a locked source's strategy
         lock_off_h_.assign(num_nodes_, -1);
         if (lock_off) lock_off_h_.assign(lock_off, lock_off + num_nodes_);
@@ -738,18 +748,24 @@ class Engine {
         int c = -1;
         for (int k = 0; k < 3; ++k)
             if (kShowCaps[k] == sc.show_nodes) c = k;
-        if (c < 0 || sc.per_level > 1) throw std::runtime_error("unknown schedule");
-        if (c == cap_ && sc.per_level == sched_.per_level) return;
+        if (c < 0 || sc.per_level > 1 || sc.river > 1) throw std::runtime_error("unknown schedule");
+        if (c == cap_ && sc.per_level == sched_.per_level && sc.river == sched_.river) return;
         HIP_OK(hipStreamSynchronize(stream_));
         drop_graph();
         cap_ = c;
         sched_ = sc;
     }
     gto_schedule schedule() const { return sched_; }
+    // the fused river sweep runs (schedule, spot shape and engine options allow it)
+    bool fused() const { return sched_.river && river_avail_ && show_mode_ == GTO_SHOWDOWN_EXACT && !locked_; }
 
-    // Candidate i of the autotuner: per-level terminals for i >= 3, showdown
-    // runs of kShowCaps[i % 3] nodes.
-    static gto_schedule candidate(int i) { return gto_schedule{kShowCaps[i % 3], (uint32_t)(i / 3)}; }
+    // Candidate i of the autotuner: i < 6 level-synchronous sweeps with
+    // per-level terminals for i >= 3 and showdown runs of kShowCaps[i % 3]
+    // nodes; i = 6 the fused river sweep.
+    static gto_schedule candidate(int i) {
+        if (i >= 6) return gto_schedule{kShowCaps[0], 0u, 1u};
+        return gto_schedule{kShowCaps[i % 3], (uint32_t)(i / 3), 0u};
+    }
 
     // GPU milliseconds per iteration of `iters` iterations after a warm-up
     // iteration (which captures the graph of a new schedule)
@@ -789,6 +805,7 @@ class Engine {
     }
 
     uint64_t reached_entries() const { return plan_.reached_entries; }
+    uint64_t river_entries() const { return river_avail_ ? plan_.river_entries : 0; }
 
     // An upper bound of the device memory an engine allocates for `v` with
     // DCFR / CFR+ (PCFR+ adds 4 bytes per arena entry), the exact showdown,
@@ -1043,7 +1060,20 @@ class Engine {
     ShowRec* show_all_ = nullptr;
     uint4* show_chunks_[3] = {nullptr, nullptr, nullptr};  // all-level runs per entry of kShowCaps
     uint32_t n_show_chunks_[3] = {0, 0, 0};
-    gto_schedule sched_{kShowCaps[0], 0};
+    gto_schedule sched_{kShowCaps[0], 0, 1};
+    // fused river sweeps: subtree records and programs, the LDS they need,
+    // the upper tree's terminals; river-level reach rows are left stale by
+    // fused sweeps (level sweeps recompute both players' rows after them)
+    bool river_avail_ = false, rows_stale_ = false, locked_ = false;
+    RiverSub* river_subs_ = nullptr;
+    RiverOp* river_ops_ = nullptr;
+    uint32_t n_river_ = 0, river_mo_ = 0;
+    size_t river_lds_ = 0;
+    FoldRec* fold_up_ = nullptr;
+    uint32_t n_fold_up_ = 0;
+    ShowRec* show_up_ = nullptr;
+    uint4* show_up_chunks_[3] = {nullptr, nullptr, nullptr};
+    uint32_t n_show_up_chunks_[3] = {0, 0, 0};
     int cap_ = 0;  // index of sched_.show_nodes in kShowCaps
     uint4* show_chunks_mma_ = nullptr;
     uint32_t n_show_chunks_mma_ = 0;
@@ -1174,7 +1204,40 @@ class Engine {
             if (D.n_chance) D.chance = upload_vec(L.chance_rec);
             D.n_edge = (uint32_t)L.edge_rec.size();
             if (D.n_edge) D.edge = upload_vec(L.edge_rec);
+            for (int q = 0; q < 2; ++q) {
+                D.n_act_up[q] = (uint32_t)L.act_up[q].size();
+                if (D.n_act_up[q]) D.act_up[q] = upload_vec(L.act_up[q]);
+            }
+            D.n_edge_up = (uint32_t)L.edge_up.size();
+            if (D.n_edge_up) D.edge_up = upload_vec(L.edge_up);
         }
+        upload_river();
+    }
+
+    // The fused river sweep needs the programs, f32 or f16 arenas without
+    // PCFR+ (the other formats keep the level sweeps) and its LDS rows
+    // within the workgroup's 160 KB.
+    void upload_river() {
+        river_avail_ = false;
+        if (!plan_.river_ok || plan_.river_subs.empty() || (compressed_ && !half_) || pred_) return;
+        river_mo_ = std::max(plan_.max_board_cnt[0], plan_.max_board_cnt[1]);
+        river_lds_ = river_lds_layout(plan_.river_rows, plan_.nh_max, river_mo_).bytes;
+        if (river_lds_ > kLdsBudget) return;
+        for (const void* f : {(const void*)river_kernel<kStoreF32>(), (const void*)river_kernel<kStoreH>()})
+            HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)river_lds_));
+        river_subs_ = upload_vec(plan_.river_subs);
+        river_ops_ = upload_vec(plan_.river_ops);
+        n_river_ = (uint32_t)plan_.river_subs.size();
+        n_fold_up_ = (uint32_t)plan_.fold_up.size();
+        if (n_fold_up_) fold_up_ = upload_vec(plan_.fold_up);
+        if (!plan_.show_up.empty()) {
+            show_up_ = upload_vec(plan_.show_up);
+            for (int c = 0; c < 3; ++c) {
+                n_show_up_chunks_[c] = (uint32_t)(plan_.show_up_chunks[c].size() / 4);
+                show_up_chunks_[c] = (uint4*)upload_vec(plan_.show_up_chunks[c]);
+            }
+        }
+        river_avail_ = true;
     }
 
     void alloc_state() {
@@ -1295,13 +1358,15 @@ class Engine {
 
     void launch_root(int mask) { launch(PK_ROOT, k_root, 1, 0, T_, S_, mask); }
 
-    void launch_down(int mask, int kind) {
+    // upper: only the nodes outside river subtrees (fused river sweeps)
+    void launch_down(int mask, int kind, bool upper = false) {
         for (size_t l = 0; l < lv_.size(); ++l) {
             const DevLevel& D = lv_[l];
             for (int q = 0; q < 2; ++q) {
-                if (!((mask >> q) & 1) || !D.n_act[q]) continue;
-                const ActRec* recs = D.act[q];
-                const uint32_t n = D.n_act[q], g = wave_grid(n);
+                const uint32_t n = upper ? D.n_act_up[q] : D.n_act[q];
+                if (!((mask >> q) & 1) || !n) continue;
+                const ActRec* recs = upper ? D.act_up[q] : D.act[q];
+                const uint32_t g = wave_grid(n);
                 if (kind == kRawF32) {
                     launch(PK_DOWN_ACTION, k_down_action<kRawF32>, g, 0, T_, S_, recs, n, q);
                 } else if (kind == kAverageF32) {
@@ -1314,9 +1379,10 @@ class Engine {
                     else launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, g, 0, T_, S_, recs, n, q);
                 }
             }
-            if (D.n_edge)
-                launch(PK_DOWN_CHANCE, k_down_chance, wave_grid(D.n_edge), 0, T_, S_, (const EdgeRec*)D.edge,
-                       D.n_edge, mask);
+            const uint32_t ne = upper ? D.n_edge_up : D.n_edge;
+            if (ne)
+                launch(PK_DOWN_CHANCE, k_down_chance, wave_grid(ne), 0, T_, S_,
+                       (const EdgeRec*)(upper ? D.edge_up : D.edge), ne, mask);
         }
     }
 
@@ -1328,7 +1394,73 @@ class Engine {
     // showdown, so exploitability judges every form with the same arithmetic.
     // the exact showdown kernel sized to the larger hand count (3 rows per
     // thread cover up to 768 hands) and to the longest per-card list
-    auto show_kernel(int) const {
+    // the fused river kernel instance: hands per thread and card-list
+    // segments as the exact showdown kernel's
+    using RiverFn = void (*)(DevTree, DevState, const RiverSub*, uint32_t, const RiverOp*, int, uint32_t, uint32_t);
+    using ShowFn = void (*)(DevTree, DevState, const ShowRec*, const uint4*, uint32_t, int, int);
+    template <int STORE>
+    RiverFn river_kernel() const {
+        if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_river<kRowPer, kMaxCardList / 4, STORE>;
+        return max_card_list_ <= 4u * kShortSeg ? k_river<3, kShortSeg, STORE> : k_river<3, kMaxCardList / 4, STORE>;
+    }
+
+    // Up sweep of a fused DCFR iteration for traverser p: the upper tree's
+    // fold terminals on the second stream beside the river kernel (one
+    // workgroup per river subtree: its reach, terminals and updates), then
+    // the upper tree's levels bottom-up (the river chance nodes merge the
+    // subtree values).
+    void launch_up_fused(int p) {
+        const bool fold = n_fold_up_ != 0, fork = fold && !prof_;
+        if (fork) {
+            HIP_OK(hipEventRecord(fork_ev_, stream_));
+            HIP_OK(hipStreamWaitEvent(aux_, fork_ev_, 0));
+            lstream_ = aux_;
+        }
+        std::exception_ptr err;
+        try {
+            if (fold)
+                launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_up_)), fold_lds_[p], T_, S_,
+                       (const FoldRec*)fold_up_, n_fold_up_, p, 0);
+        } catch (...) {
+            err = std::current_exception();
+        }
+        lstream_ = stream_;
+        if (!err) try {
+                const uint32_t g = (n_river_ + 7u) & ~7u;
+                if (half_)
+                    launch(PK_RIVER, river_kernel<kStoreH>(), g, river_lds_, T_, S_, (const RiverSub*)river_subs_,
+                           n_river_, (const RiverOp*)river_ops_, p, plan_.river_rows, river_mo_);
+                else
+                    launch(PK_RIVER, river_kernel<kStoreF32>(), g, river_lds_, T_, S_, (const RiverSub*)river_subs_,
+                           n_river_, (const RiverOp*)river_ops_, p, plan_.river_rows, river_mo_);
+                if (n_show_up_chunks_[0])
+                    launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_up_chunks_[0], 1), show_lds_[p], T_, S_,
+                           (const ShowRec*)show_up_, (const uint4*)show_up_chunks_[0], n_show_up_chunks_[0], p, 0);
+            } catch (...) {
+                err = std::current_exception();
+            }
+        if (fork) {
+            const hipError_t r1 = hipEventRecord(join_ev_, aux_);
+            const hipError_t r2 = r1 == hipSuccess ? hipStreamWaitEvent(stream_, join_ev_, 0) : r1;
+            if (!err) HIP_OK(r2);
+        }
+        if (err) std::rethrow_exception(err);
+        for (size_t l = lv_.size(); l-- > 0;) {
+            const DevLevel& D = lv_[l];
+            if (D.n_chance)
+                launch_b(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), kChanceBlock, 0, T_, S_,
+                         (const ChanceRec*)D.chance, D.n_chance, p);
+            for (int q = 0; q < 2; ++q) {
+                if (!D.n_act_up[q]) continue;
+                const ActRec* recs = D.act_up[q];
+                const uint32_t n = D.n_act_up[q], g = wave_grid(n);
+                if (half_) launch(PK_UP_ACTION, k_up_action<0, kStoreH>, g, 0, T_, S_, recs, n, p, q);
+                else launch(PK_UP_ACTION, k_up_action<0, kStoreF32>, g, 0, T_, S_, recs, n, p, q);
+            }
+        }
+    }
+
+    ShowFn show_kernel(int) const {
         if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4>;
         return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg> : k_up_show<3, kMaxCardList / 4>;
     }
@@ -1431,18 +1563,27 @@ class Engine {
     // were last pushed down.
     void launch_iteration() {
         launch(PK_OTHER, k_begin_iteration, 1, 0, S_);
+        const bool fu = fused();
+        if (!fu && rows_stale_) {
+            // fused sweeps leave the river-level reach rows behind: a level
+            // sweep after them recomputes both players' rows
+            reach_valid_[0] = reach_valid_[1] = false;
+            rows_stale_ = false;
+        }
         for (int p = 0; p < 2; ++p) {
             int mask = 0;
             for (int q = 0; q < 2; ++q)
                 if (!reach_valid_[q]) mask |= 1 << q;
             if (mask) {
                 launch_root(mask);
-                launch_down(mask, kCurrentF32);
+                launch_down(mask, kCurrentF32, fu);
             }
             reach_valid_[0] = reach_valid_[1] = true;
-            launch_up(p, 0);
+            if (fu) launch_up_fused(p);
+            else launch_up(p, 0);
             reach_valid_[p] = false;
         }
+        if (fu) rows_stale_ = true;
     }
 
     void capture_graph() {
@@ -1866,6 +2007,8 @@ int gto_hip_run(gto_hip* h, const gto_run_options* opts, const volatile int* sto
 }
 
 uint64_t gto_hip_reached_entries(const gto_hip* h) { return h->e->reached_entries(); }
+
+uint64_t gto_hip_river_entries(const gto_hip* h) { return h->e->river_entries(); }
 
 uint64_t gto_hip_estimate_vram(const gto_spot_view* spot, int storage) { return spot ? Engine::estimate(*spot, storage) : 0; }
 

@@ -96,6 +96,26 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, ui
     return L;
 }
 
+// LDS layout of the fused river kernel: `nrows` rows of nh_max floats
+// (reach and value rows of the subtree program), the board's 53 per-card
+// offsets, reduction slots, and the terminal scratch: the showdown's f64
+// prefix arrays for up to mo_max opponent hands on a board, or the fold's
+// 52 x 52 card matrix and card sums.
+struct RiverLds {
+    uint32_t rstride, coff, red, scr, bytes;
+};
+__host__ __device__ inline RiverLds river_lds_layout(uint32_t nrows, uint32_t nh_max, uint32_t mo_max) {
+    RiverLds L;
+    L.rstride = (nh_max + 3) & ~3u;
+    L.coff = nrows * L.rstride * 4;
+    L.red = L.coff + 224;                 // 16 floats of reductions, then 4 doubles (scan totals)
+    L.scr = L.red + 96;
+    const uint32_t show = 8 * (mo_max + 2) + 8 * (2 * mo_max + 53) + 4 * (mo_max + 1);
+    const uint32_t fold = 52 * 52 * 4 + 53 * 8;
+    L.bytes = (L.scr + (show > fold ? show : fold) + 15) & ~15u;
+    return L;
+}
+
 // LDS layout of the MFMA showdown for one board: opponent hand masks in
 // strength order, then the reach of up to 32 nodes in strength order as
 // bf16 ([node][k], k padded to 16), the nodes' table, per-node max |value|.

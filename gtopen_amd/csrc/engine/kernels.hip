@@ -1159,6 +1159,609 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
 }
 
 // ---------------------------------------------------------------------------
+// Fused river sweep (mode 0, DCFR iteration): one workgroup per river
+// subtree runs its program (RiverOp, records.h) — the depth-first walk of
+// the subtree with both players' reach rows and the traverser's value rows
+// in LDS — so river-level reach and value rows never go to HBM: only the
+// arenas stream, the parent's reach rows are read once and the subtree
+// root's value row is written once (for the river chance node's merge).
+// Each thread owns hands tid + 256 u of both players; reach, strategy and
+// update arithmetic per hand is thread-local and needs no barrier. Only the
+// terminals (and the opponent's all-zero test after each of its actions)
+// read across hands, behind workgroup barriers. Every per-hand expression,
+// the fold and showdown association included, is the level-sweep kernels'
+// (k_down_action, k_up_fold, k_up_show<kR, kSeg>, k_up_action<0, STORE>),
+// so the fused and level-synchronous sweeps give bit-identical results.
+// ---------------------------------------------------------------------------
+template <int KIND, int NA>
+__device__ __forceinline__ float pick_sigma(const HandSigma<NA, KIND>& hs, int a) {
+    float s = 0.0f;
+#pragma unroll
+    for (int x = 0; x < NA; ++x) s = x == a ? hs.sig[x] : s;
+    return s;
+}
+
+// destination modes of a value row entry: store, 0 + v, add
+__device__ __forceinline__ void put_value(float* __restrict__ row, uint32_t i, float v, int mode) {
+    if (mode == 0) row[i] = v;
+    else if (mode == 1) row[i] = 0.0f + v;
+    else row[i] = row[i] + v;
+}
+
+// max over the workgroup (all threads call it); slot: 4 floats of LDS
+__device__ __forceinline__ float block_max_f32(float x, float* slot) {
+    x = wave_max_f32(x);
+    if ((threadIdx.x & 63) == 0) slot[threadIdx.x >> 6] = x;
+    __syncthreads();
+    return fmaxf(fmaxf(slot[0], slot[1]), fmaxf(slot[2], slot[3]));
+}
+
+// A river op's fields as wavefront-uniform values (scalar registers): the
+// op index is uniform but depends on loaded data (skips), so the loads are
+// made uniform explicitly rather than left to the compiler's analysis.
+struct RiverOpU {
+    uint32_t op, na, a, q, src, dst, mode, vrow, cdst, cmode, skip, rel, doff;
+    float w[3];
+};
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ RiverOpU load_op(const RiverOp* __restrict__ p) {
+    const uint32_t* w = (const uint32_t*)p;
+    const uint32_t w0 = uni(w[0]), w1 = uni(w[1]), w2 = uni(w[2]);
+    RiverOpU o;
+    o.op = w0 & 0xFFu;
+    o.na = (w0 >> 8) & 0xFFu;
+    o.a = (w0 >> 16) & 0xFFu;
+    o.q = w0 >> 24;
+    o.src = w1 & 0xFFu;
+    o.dst = (w1 >> 8) & 0xFFu;
+    o.mode = (w1 >> 16) & 0xFFu;
+    o.vrow = w1 >> 24;
+    o.cdst = w2 & 0xFFu;
+    o.cmode = (w2 >> 8) & 0xFFu;
+    o.skip = w2 >> 16;
+    o.rel = uni(w[3]);
+    o.doff = uni(w[4]);
+    o.w[0] = __builtin_bit_cast(float, uni(w[5]));
+    o.w[1] = __builtin_bit_cast(float, uni(w[6]));
+    o.w[2] = __builtin_bit_cast(float, uni(w[7]));
+    return o;
+}
+// element q (uniform) of a two-entry array without dynamic indexing
+template <class X>
+__device__ __forceinline__ X sel(int q, X a0, X a1) { return q ? a1 : a0; }
+
+template <int kR, int kSeg, int STORE>
+__global__ __launch_bounds__(kBlock) void k_river(DevTree T, DevState S, const RiverSub* __restrict__ subs,
+                                                  uint32_t nsub, const RiverOp* __restrict__ ops, int p,
+                                                  uint32_t nrows, uint32_t mo_max) {
+    static_assert(STORE == kStoreF32 || STORE == kStoreH, "fused river sweeps: f32 or f16 arenas");
+    constexpr int KCUR = STORE == kStoreH ? kCurrentH : kCurrentF32;
+    extern __shared__ double lds[];
+    const uint32_t si = xcd_slice(blockIdx.x, gridDim.x);
+    if (si >= nsub) return;
+    // the subtree's record as uniform values
+    struct {
+        uint64_t base0, base1, dealt;
+        uint32_t node, prog, nops, src0, src1, card, cfv, board;
+    } sub;
+    {
+        const uint32_t* w = (const uint32_t*)(subs + si);
+        sub.base0 = (uint64_t)uni(w[0]) | ((uint64_t)uni(w[1]) << 32);
+        sub.base1 = (uint64_t)uni(w[2]) | ((uint64_t)uni(w[3]) << 32);
+        sub.dealt = (uint64_t)uni(w[4]) | ((uint64_t)uni(w[5]) << 32);
+        sub.node = uni(w[6]);
+        sub.prog = uni(w[7 + p]);
+        sub.nops = uni(w[9 + p]);
+        sub.src0 = uni(w[11]);
+        sub.src1 = uni(w[12]);
+        sub.card = uni(w[13]);
+        sub.cfv = uni(w[14]);
+        sub.board = uni(w[15]);
+    }
+    const int o = 1 - p;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nhp = T.nh[p], nho = T.nh[o];
+    const RiverLds L = river_lds_layout(nrows, T.nh_max, mo_max);
+    char* base = (char*)lds;
+    const uint32_t rs = L.rstride;
+    float* const rows = (float*)base;
+    auto row = [&](uint32_t r) -> float* { return rows + (size_t)r * rs; };
+    uint32_t* coff = (uint32_t*)(base + L.coff);
+    float* red = (float*)(base + L.red);            // 4 x 4 floats of reductions
+    double* wsum = (double*)(base + L.red + 64);    // per wave: showdown scan total
+    char* scr = base + L.scr;
+
+    const uint32_t it = *S.iter;
+    const float pos = S.disc[3 * it + 0], neg = S.disc[3 * it + 1], ds = S.disc[3 * it + 2];
+
+    // root reach rows: 1 = traverser, 2 = opponent (the chance edge's
+    // masking, k_down_chance)
+    const uint64_t cm = sub.card < 52 ? 1ull << sub.card : 0ull;
+    bool anyo = false;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = k == 0 ? p : o;
+        const uint32_t nh = sel(q, T.nh[0], T.nh[1]);
+        const uint32_t src = sel(q, sub.src0, sub.src1);
+        float* dst = row(1 + k);
+        const bool alive = sel(q, S.alive[0], S.alive[1])[src] != 0;
+        const float* __restrict__ in = sel(q, S.reach[0], S.reach[1]) + (size_t)src * nh;
+        const uint64_t* __restrict__ hm = sel(q, T.hmask[0], T.hmask[1]);
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const uint32_t j = tid + u * kBlock;
+            if (j < nh) {
+                const float x = (!alive || (hm[j] & cm)) ? 0.0f : in[j];
+                dst[j] = x;
+                if (k == 1) anyo |= x > 0.0f;
+            }
+        }
+    }
+    // static for the subtree: the showdown board's tables (k_up_show) and the
+    // fold kernel's hand words
+    const uint32_t b = sub.board;
+    const bool has_show = b != 0xFFFFFFFFu;
+    const uint32_t mo = has_show ? T.sd_cnt[o][b] : 0u, mp = has_show ? T.sd_cnt[p][b] : 0u;
+    if (has_show && tid < 53) coff[tid] = T.sd_coff[o][(size_t)b * 53 + tid];
+    anyo = __syncthreads_or(anyo);
+    float* __restrict__ out_g = S.cfv + (size_t)sub.cfv * T.nh_max;
+    if (!anyo) {  // no opponent reach: the subtree is pruned
+#pragma unroll
+        for (int u = 0; u < kR; ++u)
+            if (tid + u * kBlock < nhp) out_g[tid + u * kBlock] = 0.0f;
+        return;
+    }
+    const uint32_t cs = (mo + kBlock - 1) / kBlock;
+    const uint32_t lo = min(tid * cs, mo), hi = min(lo + cs, mo);
+    uint4 ent[kR];
+    uint32_t pidx[kR], oidx[kR];
+    uint32_t onb = 0;  // bit u: own traverser hand tid + 256 u is on the river board
+    {
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const uint32_t i = tid + u * kBlock;
+            if (i < nhp && !(T.hmask[p][i] & sub.dealt)) onb |= 1u << u;
+        }
+        const uint32_t sp = has_show ? T.sd_off[p][b] : 0u;
+        const uint4* __restrict__ tab = T.sd_tab[p] + sp;
+        const uint16_t* __restrict__ gp = T.sd_idx[p] + sp;
+        const uint32_t mlast = mp > 0 ? mp - 1 : 0;
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const uint32_t t = min(tid + u * kBlock, mlast);
+            ent[u] = mp > 0 ? tab[t] : make_uint4(0u, 0u, 0u, 0u);
+            pidx[u] = mp > 0 ? gp[t] : 0u;
+        }
+        const uint32_t so = has_show ? T.sd_off[o][b] : 0u;
+        const uint16_t* __restrict__ gi = T.sd_idx[o] + so;
+        const uint32_t olast = mo > 0 ? mo - 1 : 0;
+#pragma unroll
+        for (int u = 0; u < kR; ++u) oidx[u] = mo > 0 ? gi[min(lo + u, olast)] : 0u;
+    }
+    const uint32_t card = tid >> 2, seg = tid & 3;
+    uint32_t c0 = 0, len = 0, s0 = 0, sn = 0;
+    if (has_show && card < 52) {
+        c0 = coff[card];
+        len = coff[card + 1] - c0;
+        s0 = min(seg * (uint32_t)kSeg, len);
+        sn = min((uint32_t)kSeg, len - s0);
+    }
+    const uint32_t creg = 2 * mo + 52;
+    uint32_t cw[kSeg];
+    {
+        const uint16_t* __restrict__ gpos = T.sd_cpos[o] + 2 * (size_t)(has_show ? T.sd_off[o][b] : 0u);
+        const uint32_t last = mo > 0 ? 2 * mo - 1 : 0;
+#pragma unroll
+        for (int u = 0; u < (int)kSeg; ++u)
+            cw[u] = (uint32_t)u < sn ? gpos[min(c0 + s0 + u, last)] | ((c0 + card + s0 + u) << 16)
+                                     : mo | (creg << 16);
+    }
+
+    const RiverOp* __restrict__ prog = ops + sub.prog;
+    const uint32_t nops = sub.nops;
+    for (uint32_t k = 0; k < nops;) {
+        const RiverOpU op = load_op(prog + k);
+        ++k;
+        if (op.op == kOpDescend) {
+            // child reach of the actor q: parent reach x current strategy
+            const int q = (int)op.q;
+            const uint32_t nh = sel(q, T.nh[0], T.nh[1]);
+            const float* __restrict__ src = row(op.src);
+            float* __restrict__ dst = row(op.dst);
+            const auto* arena = sel(q, ArenaOf<KCUR>::get(S, 0), ArenaOf<KCUR>::get(S, 1)) + sel(q, sub.base0, sub.base1) +
+                                op.doff;
+            const int a = op.a;
+            bool any = false;
+            dispatch_na4(op.na, [&](auto NAc) {
+                constexpr int NA = decltype(NAc)::value;
+                if constexpr (NA > 0) {
+                    HandSigma<NA, KCUR> hs[kR];
+#pragma unroll
+                    for (int u = 0; u < kR; ++u) hs[u].fetch(arena, nh, min(tid + u * kBlock, nh - 1));
+#pragma unroll
+                    for (int u = 0; u < kR; ++u) {
+                        const uint32_t j = tid + u * kBlock;
+                        if (j < nh) {
+                            hs[u].finish();
+                            const float v = src[j] * pick_sigma(hs[u], a);
+                            dst[j] = v;
+                            any |= v > 0.0f;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kR; ++u) {
+                        const uint32_t j = tid + u * kBlock;
+                        if (j < nh) {
+                            SigmaHand<KCUR> sh;
+                            sh.init(arena, nh, j, op.na);
+                            const float v = src[j] * sh.template sigma<false>(a);
+                            dst[j] = v;
+                            any |= v > 0.0f;
+                        }
+                    }
+                }
+            });
+            if (q == o) {
+                // the opponent's row is complete (and read across hands by the
+                // terminals) after this barrier; an all-zero row prunes the child
+                any = __syncthreads_or(any);
+                if (!any) {
+                    float* cd = row(op.cdst);
+#pragma unroll
+                    for (int u = 0; u < kR; ++u)
+                        if (tid + u * kBlock < nhp) put_value(cd, tid + u * kBlock, 0.0f, op.cmode);
+                    k += op.skip;
+                }
+            }
+        } else if (op.op == kOpFold) {
+            // k_up_fold on the opponent's row src
+            const float* __restrict__ ro = row(op.src);
+            float* __restrict__ dst = row(op.dst);
+            float* M = (float*)scr;
+            double* s_card = (double*)(scr + 52 * 52 * 4);
+            double* s_tot = s_card + 52;
+            // the opponent's hands as card-matrix positions, the traverser's
+            // packed hand words
+            uint32_t hw[kR], mpos[kR];
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                mpos[u] = T.hpack[o][min(tid + u * kBlock, nho - 1)];
+                hw[u] = T.hpack[p][min(tid + u * kBlock, nhp - 1)];
+            }
+            __syncthreads();  // earlier readers of the scratch are done
+            {
+                float4* m4 = (float4*)M;
+                for (uint32_t x = tid; x < 52 * 52 / 4; x += kBlock) m4[x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kR; ++u)
+                if (tid + u * kBlock < nho) {
+                    const float r = ro[tid + u * kBlock];
+                    const uint32_t a1 = mpos[u] & 0xFFu, a2 = (mpos[u] >> 8) & 0xFFu;
+                    M[a1 * 52 + a2] = r;
+                    M[a2 * 52 + a1] = r;
+                }
+            __syncthreads();
+            if (wave == 0) {
+                // the total as k_up_fold forms it: per lane over hands
+                // lane + 64 u, then the wavefront sum
+                double part = 0.0;
+                for (uint32_t j = lane; j < nho; j += kWave) part += (double)ro[j];
+                const double tot = wave_sum_f64(part);
+                if (lane == 0) *s_tot = tot;
+                if (lane < 52) {
+                    const float4* mrow = (const float4*)(M + lane * 52);
+                    double acc = 0.0;
+#pragma unroll
+                    for (int t0 = 0; t0 < 13; t0 += 4) {
+                        float4 v[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            if (t0 + t < 13) v[t] = mrow[t0 + t];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            if (t0 + t < 13) {
+                                acc += (double)v[t].x;
+                                acc += (double)v[t].y;
+                                acc += (double)v[t].z;
+                                acc += (double)v[t].w;
+                            }
+                    }
+                    s_card[lane] = acc;
+                }
+            }
+            __syncthreads();
+            const double tot = *s_tot;
+            const float amount = op.q == (uint32_t)p ? op.w[1] : op.w[0];
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const uint32_t i = tid + u * kBlock;
+                if (i < nhp) {
+                    const uint32_t a1 = hw[u] & 0xFFu, a2 = (hw[u] >> 8) & 0xFFu;
+                    const float same_r = M[a1 * 52 + a2];
+                    const float valid = (float)(tot - s_card[a1] - s_card[a2]) + same_r;
+                    put_value(dst, i, amount * valid, op.mode);
+                }
+            }
+        } else if (op.op == kOpShow) {
+            // k_up_show's phases 2-4 on the opponent's row src
+            const float* __restrict__ rhs = row(op.src);
+            float* __restrict__ dst = row(op.dst);
+            const uint32_t sd_P = 0, sd_cpre = 8 * (mo + 2), sd_rs = sd_cpre + 8 * (creg + 1);
+            double* P = (double*)(scr + sd_P);
+            double* cpre = (double*)(scr + sd_cpre);
+            float* rsw = (float*)(scr + sd_rs);
+            const float win = op.w[0], lose = op.w[1], tie = op.w[2];
+            // hands off the board take 0 (k_up_show leaves them at 0); with no
+            // opponent hand on the board every hand does
+#pragma unroll
+            for (int u = 0; u < kR; ++u)
+                if (tid + u * kBlock < nhp && (mo == 0 || !((onb >> u) & 1u)))
+                    put_value(dst, tid + u * kBlock, 0.0f, op.mode);
+            if (mo == 0) continue;
+            __syncthreads();  // earlier readers of the scratch and writers of dst are done
+            float v[kR];
+            double part = 0.0;
+#pragma unroll
+            for (int u = 0; u < kR; ++u) v[u] = rhs[oidx[u]];
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const uint32_t kk = lo + u;
+                v[u] = kk < hi ? v[u] : 0.0f;
+                rsw[kk < hi ? kk : mo] = v[u];
+                part += (double)v[u];
+            }
+            if (tid == 0) rsw[mo] = 0.0f;
+            const double incl = wave_incl_sum_f64(part);
+            const double excl = dpp_zero<0x138>(incl);
+            if (lane == kWave - 1) wsum[wave] = incl;
+            __syncthreads();
+            double before = 0.0, total = 0.0;
+#pragma unroll
+            for (int w2 = 0; w2 < kWaves; ++w2) {
+                const double t = wsum[w2];
+                before += (uint32_t)w2 < wave ? t : 0.0;
+                total += t;
+            }
+            double run = before + excl;
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const uint32_t kk = lo + u;
+                P[kk < hi ? kk : mo + 1] = run;
+                run += (double)v[u];
+            }
+            if (tid == 0) P[mo] = total;
+            if (card < 52) {
+                float x[kSeg];
+#pragma unroll
+                for (int u = 0; u < (int)kSeg; ++u) x[u] = rsw[cw[u] & 0xFFFFu];
+                double a = 0.0;
+#pragma unroll
+                for (int u = 0; u < (int)kSeg; ++u) a += (double)x[u];
+                const double a0 = dpp_zero<0x00>(a), a1 = dpp_zero<0x55>(a), a2 = dpp_zero<0xAA>(a),
+                             a3 = dpp_zero<0xFF>(a);
+                double y = 0.0;
+                if (seg > 0) y += a0;
+                if (seg > 1) y += a1;
+                if (seg > 2) y += a2;
+#pragma unroll
+                for (int u = 0; u < (int)kSeg; ++u) {
+                    cpre[cw[u] >> 16] = y;
+                    y += (double)x[u];
+                }
+                if (seg == 3) cpre[c0 + card + len] = ((a0 + a1) + a2) + a3;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const uint32_t t = tid + u * kBlock;
+                if (t >= mp) continue;
+                const uint4 e = ent[u];
+                const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
+                const double* cp1 = cpre + (e.y & 0xFFFFu);
+                const double* cp2 = cpre + (e.y >> 16);
+                const double S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
+                const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
+                const float higher =
+                    (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
+                const uint32_t sp = e.w >> 16;
+                const float same_r = sp != 0xFFFFu ? rsw[sp] : 0.0f;
+                const float valid = (float)(total - S1 - S2) + same_r;
+                const float tied = valid - lower - higher;
+                put_value(dst, pidx[u], win * lower + lose * higher + tie * tied, op.mode);
+            }
+            __syncthreads();  // the scattered row is complete
+        } else if (op.op == kOpPostP) {
+            // the traverser's node: k_up_action<0, STORE> on LDS rows
+            const int na = op.na;
+            const float* __restrict__ rp = row(op.src);
+            const float* __restrict__ V = row(op.vrow);
+            float* __restrict__ dst = row(op.dst);
+            if constexpr (STORE == kStoreF32) {
+                const uint64_t off = sel(p, sub.base0, sub.base1) + op.doff;
+                float* __restrict__ R = sel(p, S.R[0], S.R[1]) + off;
+                float* __restrict__ Sm = sel(p, S.S[0], S.S[1]) + off;
+                dispatch_na4(na, [&](auto NAc) {
+                    constexpr int NA = decltype(NAc)::value;
+                    if constexpr (NA > 0) {
+                        HandSigma<NA, kCurrentF32> hs[kR];
+                        float sm[kR][NA];
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const uint32_t ii = min(tid + u * kBlock, nhp - 1);
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) {
+                                hs[u].raw[x] = R[(size_t)x * nhp + ii];
+                                sm[u][x] = Sm[(size_t)x * nhp + ii];
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const uint32_t i = tid + u * kBlock;
+                            if (i >= nhp) continue;
+                            hs[u].finish();
+                            float v[NA];
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) v[x] = V[(size_t)x * rs + i];
+                            const float reach = rp[i];
+                            float acc = 0.0f;
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) acc += hs[u].sig[x] * v[x];
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) {
+                                const float rv = hs[u].raw[x];
+                                const float d = rv > 0.0f ? pos : neg;
+                                R[(size_t)x * nhp + i] = rv * d + (v[x] - acc);
+                                Sm[(size_t)x * nhp + i] = sm[u][x] * ds + reach * hs[u].sig[x];
+                            }
+                            put_value(dst, i, acc, op.mode);
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const uint32_t i = tid + u * kBlock;
+                            if (i >= nhp) continue;
+                            SigmaHand<kCurrentF32> sh;
+                            sh.init(R, nhp, i, na);
+                            float acc = 0.0f;
+                            for (int x = 0; x < na; ++x) acc += sh.template sigma<false>(x) * V[(size_t)x * rs + i];
+                            const float reach = rp[i];
+                            for (int x = 0; x < na; ++x) {
+                                const size_t idx = (size_t)x * nhp + i;
+                                const float rv = R[idx];
+                                const float sg = sh.template sigma<false>(x);
+                                const float d = rv > 0.0f ? pos : neg;
+                                R[idx] = rv * d + (V[(size_t)x * rs + i] - acc);
+                                Sm[idx] = Sm[idx] * ds + reach * sg;
+                            }
+                            put_value(dst, i, acc, op.mode);
+                        }
+                    }
+                });
+            } else {
+                // f16 arenas: the scales come from the node's previous maxima
+                // and the maxima of its children's value rows and of its
+                // reach row (k_up_action<0, kStoreH>)
+                const uint64_t off = sel(p, sub.base0, sub.base1) + op.doff;
+                uint16_t* __restrict__ Rh = (uint16_t*)sel(p, S.Rq[0], S.Rq[1]) + off;
+                uint16_t* __restrict__ Sh = sel(p, S.Sq[0], S.Sq[1]) + off;
+                const uint32_t node = sub.node + op.rel;
+                float* __restrict__ Rs_ = sel(p, S.Rs[0], S.Rs[1]);
+                float* __restrict__ Ss_ = sel(p, S.Ss[0], S.Ss[1]);
+                float* __restrict__ Rm_ = sel(p, S.Rm[0], S.Rm[1]);
+                float* __restrict__ Sm_ = sel(p, S.Sm[0], S.Sm[1]);
+                float mv = 0.0f, mr = 0.0f;
+#pragma unroll
+                for (int u = 0; u < kR; ++u) {
+                    const uint32_t i = tid + u * kBlock;
+                    if (i < nhp) {
+                        for (int x = 0; x < na; ++x) mv = fmaxf(mv, fabsf(V[(size_t)x * rs + i]));
+                        mr = fmaxf(mr, rp[i]);
+                    }
+                }
+                mv = block_max_f32(mv, red);
+                mr = block_max_f32(mr, red + 4);
+                const float sR = Rs_[node], sS = Ss_[node];
+                const float kRs = half_scale(Rm_[node] * fmaxf(pos, neg) + 2.0f * mv);
+                const float kSs = half_scale(Sm_[node] * ds + mr);
+                const float iR = 1.0f / kRs, iS = 1.0f / kSs;
+                float xr = 0.0f, xs = 0.0f;
+                dispatch_na4(na, [&](auto NAc) {
+                    constexpr int NA = decltype(NAc)::value;
+                    if constexpr (NA > 0) {
+                        HandSigma<NA, kCurrentH> hs[kR];
+                        float sm[kR][NA];
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const uint32_t ii = min(tid + u * kBlock, nhp - 1);
+                            hs[u].fetch(Rh, nhp, ii);
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) sm[u][x] = h2f(Sh[(size_t)x * nhp + ii]);
+                        }
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const uint32_t i = tid + u * kBlock;
+                            if (i >= nhp) continue;
+                            hs[u].finish();
+                            float v[NA];
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) v[x] = V[(size_t)x * rs + i];
+                            const float reach = rp[i];
+                            float acc = 0.0f;
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) acc += hs[u].sig[x] * v[x];
+#pragma unroll
+                            for (int x = 0; x < NA; ++x) {
+                                const float rv = hs[u].raw[x] * sR;
+                                const float d = rv > 0.0f ? pos : neg;
+                                const float nr = rv * d + (v[x] - acc);
+                                const float ns = (sm[u][x] * sS) * ds + reach * hs[u].sig[x];
+                                Rh[(size_t)x * nhp + i] = f2h(nr * iR);
+                                Sh[(size_t)x * nhp + i] = f2h(ns * iS);
+                                xr = fmaxf(xr, fabsf(nr));
+                                xs = fmaxf(xs, ns);
+                            }
+                            put_value(dst, i, acc, op.mode);
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const uint32_t i = tid + u * kBlock;
+                            if (i >= nhp) continue;
+                            SigmaHand<kCurrentH> sh;
+                            sh.init(Rh, nhp, i, na);
+                            float acc = 0.0f;
+                            for (int x = 0; x < na; ++x) acc += sh.template sigma<false>(x) * V[(size_t)x * rs + i];
+                            const float reach = rp[i];
+                            for (int x = 0; x < na; ++x) {
+                                const size_t idx = (size_t)x * nhp + i;
+                                const float rv = h2f(Rh[idx]) * sR;
+                                const float sg = sh.template sigma<false>(x);
+                                const float d = rv > 0.0f ? pos : neg;
+                                const float nr = rv * d + (V[(size_t)x * rs + i] - acc);
+                                const float ns = (h2f(Sh[idx]) * sS) * ds + reach * sg;
+                                Rh[idx] = f2h(nr * iR);
+                                Sh[idx] = f2h(ns * iS);
+                                xr = fmaxf(xr, fabsf(nr));
+                                xs = fmaxf(xs, ns);
+                            }
+                            put_value(dst, i, acc, op.mode);
+                        }
+                    }
+                });
+                xr = wave_max_f32(xr);
+                xs = wave_max_f32(xs);
+                float* r2 = red + 8;  // a slot set of its own: red / red + 4 may still be read
+                if (lane == 0) {
+                    r2[wave] = xr;
+                    r2[4 + wave] = xs;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    Rs_[node] = kRs;
+                    Ss_[node] = kSs;
+                    Rm_[node] = fmaxf(fmaxf(r2[0], r2[1]), fmaxf(r2[2], r2[3]));
+                    Sm_[node] = fmaxf(fmaxf(r2[4], r2[5]), fmaxf(r2[6], r2[7]));
+                }
+            }
+        } else {  // kOpPostO: an opponent node's sum added into its parent's row
+            const float* __restrict__ A = row(op.vrow);
+            float* __restrict__ dst = row(op.dst);
+#pragma unroll
+            for (int u = 0; u < kR; ++u)
+                if (tid + u * kBlock < nhp) put_value(dst, tid + u * kBlock, A[tid + u * kBlock], 2);
+        }
+    }
+    // the subtree root's row, for the river chance node's merge
+    const float* __restrict__ res = row(0);
+#pragma unroll
+    for (int u = 0; u < kR; ++u)
+        if (tid + u * kBlock < nhp) out_g[tid + u * kBlock] = res[tid + u * kBlock];
+}
+
+// ---------------------------------------------------------------------------
 // Up sweep, showdown terminals as a matrix product on the matrix cores
 // (the BF16 precision form; the exact form is k_up_show). For the nodes of
 // one river board, sorted by strength on both sides:

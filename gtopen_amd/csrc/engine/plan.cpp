@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <string>
 #include <stdexcept>
 
 namespace gto {
@@ -31,7 +32,241 @@ uint32_t river_key_of(const gto_spot_view& v, const Item& it) {
 }
 // permute_card, cards.rs:94-96 (card = 4 * rank + suit)
 inline uint8_t permute_card(uint8_t c, const uint8_t* pm) { return (uint8_t)((c & ~3u) | pm[c & 3u]); }
+
+// runs of at most `cap` board-sorted records sharing a board: (board, first, count, 0)
+void runs(const std::vector<ShowRec>& recs, uint32_t cap, std::vector<uint32_t>& out) {
+    for (uint32_t k = 0; k < recs.size();) {
+        uint32_t e = k;
+        while (e < recs.size() && e - k < cap && recs[e].board == recs[k].board) ++e;
+        out.insert(out.end(), {recs[k].board, k, e - k, 0u});
+        k = e;
+    }
+}
+
+// The program of one river subtree for traverser p: its depth-first walk as
+// RiverOps with every LDS row assigned on a stack (a row is live from its
+// op until the walk leaves the node that needed it).
+struct RiverProgram {
+    const gto_spot_view& v;
+    int p;
+    uint32_t root;
+    uint64_t base[2];
+    std::vector<RiverOp> ops;
+    uint32_t top = 0, rows = 0;
+    bool ok = true;
+
+    RiverProgram(const gto_spot_view& v_, int p_, uint32_t root_, const uint64_t b[2]) : v(v_), p(p_), root(root_) {
+        base[0] = b[0];
+        base[1] = b[1];
+    }
+    uint32_t alloc(uint32_t k) {
+        const uint32_t r = top;
+        top += k;
+        rows = std::max(rows, top);
+        return r;
+    }
+    RiverOp make(uint8_t kind, uint32_t n) {
+        RiverOp op{};
+        op.op = kind;
+        op.rel = n - root;
+        return op;
+    }
+    // node n with reach rows rowP (traverser) and rowO (opponent); its value
+    // goes into row dst with the given mode
+    void emit(uint32_t n, uint32_t rowP, uint32_t rowO, uint32_t dst, uint8_t mode) {
+        const uint8_t kind = v.node_kind[n];
+        if (kind == GTO_KIND_FOLD || kind == GTO_KIND_SHOWDOWN) {
+            RiverOp op = make(kind == GTO_KIND_FOLD ? kOpFold : kOpShow, n);
+            op.q = v.node_player[n];
+            op.src = (uint8_t)rowO;
+            op.dst = (uint8_t)dst;
+            op.mode = mode;
+            op.w[0] = (float)v.node_t_win[n];
+            op.w[1] = (float)v.node_t_lose[n];
+            op.w[2] = (float)v.node_t_tie[n];
+            ops.push_back(op);
+            return;
+        }
+        if (kind != GTO_KIND_ACTION) {  // a chance node below the river: not a river subtree
+            ok = false;
+            return;
+        }
+        const int q = v.node_player[n];
+        const uint32_t na = v.node_num_children[n];
+        const uint64_t doff = v.node_data_offset[n] - base[q];
+        if (doff > 0xFFFFFFFFull || na > 250) {
+            ok = false;
+            return;
+        }
+        const uint32_t cs = v.node_children_start[n];
+        if (q == p) {
+            const uint32_t vbase = alloc(na);
+            for (uint32_t a = 0; a < na; ++a) {
+                const uint32_t c = v.children[cs + a];
+                if (v.node_kind[c] == GTO_KIND_ACTION) {
+                    const uint32_t save = top, nr = alloc(1);
+                    RiverOp op = make(kOpDescend, n);
+                    op.na = (uint8_t)na;
+                    op.a = (uint8_t)a;
+                    op.q = (uint8_t)q;
+                    op.src = (uint8_t)rowP;
+                    op.dst = (uint8_t)nr;
+                    op.doff = (uint32_t)doff;
+                    ops.push_back(op);
+                    emit(c, nr, rowO, vbase + a, 0);
+                    top = save;
+                } else {
+                    emit(c, rowP, rowO, vbase + a, 0);  // a terminal reads only the opponent's reach
+                }
+            }
+            RiverOp op = make(kOpPostP, n);
+            op.na = (uint8_t)na;
+            op.src = (uint8_t)rowP;
+            op.vrow = (uint8_t)vbase;
+            op.dst = (uint8_t)dst;
+            op.mode = mode;
+            op.doff = (uint32_t)doff;
+            ops.push_back(op);
+            top = vbase;
+        } else {
+            // the children's values are summed in action order, 0 + v0 + v1 ..
+            // (directly into dst unless dst itself accumulates)
+            const bool sep = mode == 2;
+            const uint32_t arow = sep ? alloc(1) : dst;
+            for (uint32_t a = 0; a < na; ++a) {
+                const uint32_t c = v.children[cs + a];
+                const uint32_t save = top, nr = alloc(1);
+                const size_t at = ops.size();
+                RiverOp op = make(kOpDescend, n);
+                op.na = (uint8_t)na;
+                op.a = (uint8_t)a;
+                op.q = (uint8_t)q;
+                op.src = (uint8_t)rowO;
+                op.dst = (uint8_t)nr;
+                op.doff = (uint32_t)doff;
+                op.cdst = (uint8_t)arow;
+                op.cmode = a == 0 ? 1 : 2;
+                ops.push_back(op);
+                emit(c, rowP, nr, arow, a == 0 ? 1 : 2);
+                const size_t skip = ops.size() - at - 1;
+                if (skip > 0xFFFF) ok = false;
+                ops[at].skip = (uint16_t)skip;
+                top = save;
+            }
+            if (sep) {
+                RiverOp op = make(kOpPostO, n);
+                op.vrow = (uint8_t)arow;
+                op.dst = (uint8_t)dst;
+                op.mode = 2;
+                ops.push_back(op);
+                top = arow;
+            }
+        }
+    }
+};
 }  // namespace
+
+// River subtrees for the fused river kernel: the traversed children of the
+// chance nodes that deal the river (or the root of a river spot), each with
+// its per-traverser program (shared by subtrees with identical programs),
+// and the level lists of the rest of the tree.
+static void build_river(const gto_spot_view& v, Plan& pl, const std::vector<uint64_t>& dmask) {
+    const uint32_t N = v.num_nodes;
+    std::vector<uint8_t> in_river(N, 0);
+    struct Root {
+        uint32_t node, src[2], card;
+    };
+    std::vector<Root> roots;
+    if (v.root_street == 2) {
+        roots.push_back({0u, {0u, 0u}, 52u});
+    } else {
+        for (const LevelLists& ll : pl.levels)
+            for (size_t e = 0; e < ll.edge_parent.size(); ++e) {
+                const uint32_t c = ll.edge_parent[e];
+                if (v.node_street[c] != 2) continue;  // chance nodes dealing the turn
+                roots.push_back({ll.edge_child[e], {pl.owner[0][c], pl.owner[1][c]}, ll.edge_card[e]});
+            }
+    }
+    pl.river_ok = true;
+    std::map<std::string, uint32_t> prog_of;
+    std::vector<uint32_t> stack;
+    for (const Root& r : roots) {
+        // the subtree: pre-order, contiguous node ids
+        uint64_t base[2] = {~0ull, ~0ull};
+        uint32_t board = kSent;
+        stack.assign(1, r.node);
+        while (!stack.empty()) {
+            const uint32_t n = stack.back();
+            stack.pop_back();
+            in_river[n] = 1;
+            const uint8_t kind = v.node_kind[n];
+            if (kind == GTO_KIND_ACTION) {
+                const int q = v.node_player[n];
+                base[q] = std::min<uint64_t>(base[q], v.node_data_offset[n]);
+                pl.river_entries += (uint64_t)v.node_num_children[n] * v.num_hands[q];
+                for (uint32_t a = 0; a < v.node_num_children[n]; ++a)
+                    stack.push_back(v.children[v.node_children_start[n] + a]);
+            } else if (kind == GTO_KIND_SHOWDOWN) {
+                if (board != kSent && board != pl.show_slot[n]) pl.river_ok = false;
+                board = pl.show_slot[n];
+            } else if (kind == GTO_KIND_CHANCE) {
+                pl.river_ok = false;
+            }
+        }
+        for (int q = 0; q < 2; ++q)
+            if (base[q] == ~0ull) base[q] = 0;
+        RiverSub sub{};
+        sub.base[0] = base[0];
+        sub.base[1] = base[1];
+        sub.dealt = dmask[r.node];
+        sub.node = r.node;
+        sub.src[0] = r.src[0];
+        sub.src[1] = r.src[1];
+        sub.card = r.card;
+        sub.cfv = pl.cfv_slot[r.node];
+        sub.board = board;
+        for (int p = 0; p < 2; ++p) {
+            RiverProgram prog(v, p, r.node, base);
+            const uint32_t out = prog.alloc(1);  // row 0: the subtree's value
+            const uint32_t rp = prog.alloc(1), ro = prog.alloc(1);  // rows 1, 2: the root's reach
+            prog.emit(r.node, rp, ro, out, 0);
+            if (!prog.ok || prog.rows > 255) {
+                pl.river_ok = false;
+                continue;
+            }
+            pl.river_rows = std::max(pl.river_rows, prog.rows);
+            const std::string key((const char*)prog.ops.data(), prog.ops.size() * sizeof(RiverOp));
+            auto it = prog_of.find(key);
+            if (it == prog_of.end()) {
+                it = prog_of.emplace(key, (uint32_t)pl.river_ops.size()).first;
+                pl.river_ops.insert(pl.river_ops.end(), prog.ops.begin(), prog.ops.end());
+            }
+            sub.prog[p] = it->second;
+            sub.nops[p] = (uint32_t)prog.ops.size();
+        }
+        pl.river_subs.push_back(sub);
+    }
+    if (!pl.river_ok) {
+        pl.river_subs.clear();
+        pl.river_ops.clear();
+        return;
+    }
+    // the rest of the tree: level records of nodes outside river subtrees
+    for (LevelLists& ll : pl.levels) {
+        for (int q = 0; q < 2; ++q)
+            for (const ActRec& r : ll.act_rec[q])
+                if (!in_river[r.node]) ll.act_up[q].push_back(r);
+        for (size_t e = 0; e < ll.edge_rec.size(); ++e)
+            if (!in_river[ll.edge_child[e]]) ll.edge_up.push_back(ll.edge_rec[e]);
+        for (size_t k = 0; k < ll.fold.size(); ++k)
+            if (!in_river[ll.fold[k]]) pl.fold_up.push_back(ll.fold_rec[k]);
+        for (size_t k = 0; k < ll.show.size(); ++k)
+            if (!in_river[ll.show[k]]) pl.show_up.push_back(ll.show_rec[k]);
+    }
+    std::stable_sort(pl.show_up.begin(), pl.show_up.end(),
+                     [](const ShowRec& a, const ShowRec& b) { return a.board < b.board; });
+    for (int c = 0; c < 3; ++c) runs(pl.show_up, kShowCaps[c], pl.show_up_chunks[c]);
+}
 
 Plan build_plan(const gto_spot_view& v, bool iso) {
     Plan pl;
@@ -51,6 +286,7 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
     std::vector<uint32_t> level(N, kSent);
     std::vector<uint32_t> key(N, kSent);
     std::vector<float> cw(N, 0.0f);
+    std::vector<uint64_t> dmask(N, 0);  // cards dealt by chance nodes above the node
 
     // depth-first walk over the reachable tree (chance nodes skip cards
     // already dealt, as cfr.rs:556-565 does)
@@ -68,6 +304,7 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
         pl.owner[0][n] = it.src[0];
         pl.owner[1][n] = it.src[1];
         cw[n] = (float)it.cw;
+        for (int d = 0; d < it.ndealt; ++d) dmask[n] |= 1ull << it.dealt[d];
         max_level = std::max(max_level, it.level);
         const uint8_t kind = v.node_kind[n];
         const uint32_t cs = v.node_children_start[n];
@@ -368,21 +605,13 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
     }
     std::stable_sort(pl.show_all.begin(), pl.show_all.end(),
                      [](const ShowRec& a, const ShowRec& b) { return a.board < b.board; });
-    // runs of at most `cap` board-sorted records sharing a board
-    auto runs = [](const std::vector<ShowRec>& recs, uint32_t cap, std::vector<uint32_t>& out) {
-        for (uint32_t k = 0; k < recs.size();) {
-            uint32_t e = k;
-            while (e < recs.size() && e - k < cap && recs[e].board == recs[k].board) ++e;
-            out.insert(out.end(), {recs[k].board, k, e - k, 0u});
-            k = e;
-        }
-    };
     for (int c = 0; c < 3; ++c) {
         runs(pl.show_all, kShowCaps[c], pl.show_chunks[c]);
         for (LevelLists& ll : pl.levels) runs(ll.show_rec, kShowCaps[c], ll.show_chunks[c]);
     }
     runs(pl.show_all, kMmaNodes, pl.show_chunks_mma);
 
+    build_river(v, pl, dmask);
     return pl;
 }
 

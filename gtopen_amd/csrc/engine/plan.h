@@ -19,6 +19,9 @@ struct LevelLists {
     std::vector<uint32_t> edge_parent, edge_child, edge_card;
     // work records, in the same order as the lists above
     std::vector<ActRec> act_rec[2];
+    // the same without the nodes inside river subtrees (fused river sweeps)
+    std::vector<ActRec> act_up[2];
+    std::vector<EdgeRec> edge_up;
     std::vector<FoldRec> fold_rec;
     std::vector<ShowRec> show_rec;  // sorted by board slot
     std::vector<ChanceRec> chance_rec;
@@ -100,6 +103,18 @@ struct Plan {
     std::vector<uint32_t> show_chunks[3];
     // the same runs capped at kMmaNodes nodes (the MFMA showdown's N tile)
     std::vector<uint32_t> show_chunks_mma;
+
+    // fused river sweeps: the subtrees, their programs (deduplicated), the
+    // LDS rows the programs need, and the terminals outside river subtrees
+    // (all levels, as fold_all / show_all)
+    std::vector<RiverSub> river_subs;
+    std::vector<RiverOp> river_ops;
+    uint32_t river_rows = 0;
+    bool river_ok = false;                 // every reached river subtree has a program
+    uint64_t river_entries = 0;            // reached arena entries inside river subtrees
+    std::vector<FoldRec> fold_up;
+    std::vector<ShowRec> show_up;
+    std::vector<uint32_t> show_up_chunks[3];
 
     // algorithmic arena entries over reached action nodes (sum na*nh_actor)
     uint64_t reached_entries = 0;

@@ -59,4 +59,56 @@ struct EdgeRec {
 };
 static_assert(sizeof(EdgeRec) == 32, "EdgeRec layout");
 
+// River subtree, the unit of the fused river kernel: a child of a chance
+// node dealing the river (or the root of a spot that starts on the river).
+// Every node below it shares one board, and the tree builder lays its nodes
+// and arena blocks out contiguously in pre-order, so a subtree is its root
+// plus per-player arena bases and a program of ops (RiverOp) shared by all
+// subtrees of the same shape and payoffs.
+struct RiverSub {
+    uint64_t base[2];   // arena offset of each player's first action node in the subtree
+    uint64_t dealt;     // cards dealt on the turn and river (the traverser's hands holding one are off the board)
+    uint32_t node;      // tree node of the subtree root (op node ids are relative to it)
+    uint32_t prog[2];   // first op of the program, per traverser
+    uint32_t nops[2];   // op count, per traverser
+    uint32_t src[2];    // reach slots of the parent (the river chance node), per player
+    uint32_t card;      // river card dealt into the subtree (52: none)
+    uint32_t cfv;       // value row of the subtree root
+    uint32_t board;     // river-board slot of its showdowns (0xFFFFFFFF: none)
+    uint32_t pad;
+};
+static_assert(sizeof(RiverSub) == 72, "RiverSub layout");
+
+// One step of a river subtree program for one traverser p. Programs are
+// the subtree's depth-first walk with every LDS row assigned statically
+// (stack discipline): reach rows of both players and value rows of the
+// traverser, nh_max floats each.
+//   kOpDescend: reach row `dst` of the actor q's child a = row `src` times
+//               q's current strategy for action a at the node. When q is the
+//               opponent and the new row is all zero (zero-reach pruning,
+//               cfr.rs:424-430) the child's value is 0, written into row
+//               `cdst` with mode `cmode`, and the next `skip` ops (the
+//               child's subtree) are skipped.
+//   kOpFold / kOpShow: terminal values from the opponent's reach row `src`
+//               into row `dst` (w: fold t_win, t_lose; showdown win, lose,
+//               tie; `q`: the player who folded).
+//   kOpPostP:   the traverser's node: out = sum sigma v over the child rows
+//               vrow.., regret and strategy-sum update with reach row `src`,
+//               out into row `dst`.
+//   kOpPostO:   an opponent node whose children summed into row `vrow`: its
+//               value added into row `dst`.
+// Destination modes: 0 = store v, 1 = store 0 + v (the first term of a sum),
+// 2 = add v.
+enum RiverOpKind : uint8_t { kOpDescend = 0, kOpFold = 1, kOpShow = 2, kOpPostP = 3, kOpPostO = 4 };
+struct RiverOp {
+    uint8_t op, na, a, q;
+    uint8_t src, dst, mode, vrow;
+    uint8_t cdst, cmode;
+    uint16_t skip;
+    uint32_t rel;   // node id minus the subtree root's
+    uint32_t doff;  // arena offset minus the subtree's base of the actor
+    float w[3];
+};
+static_assert(sizeof(RiverOp) == 32, "RiverOp layout");
+
 }  // namespace gto

@@ -11,6 +11,10 @@ from . import _lib as L
 from .query import QueryMixin
 
 
+def schedule_dict(sc):
+    return {"show_nodes": int(sc.show_nodes), "per_level": bool(sc.per_level), "river": bool(sc.river)}
+
+
 class Progress(NamedTuple):
     """Progress (cfr.rs:91-97), plus the seconds spent in iterations."""
     iteration: int
@@ -49,17 +53,19 @@ class GpuSolver(QueryMixin):
         if showdown != "exact":
             self.set_showdown(showdown)
 
-    def set_schedule(self, show_nodes=16, per_level=False):
-        """Launch schedule of the up sweep (results are identical under every
-        schedule): showdown nodes per workgroup (16, 8 or 4) and whether each
-        level's terminals launch with that level."""
-        sc = L.Schedule(show_nodes, int(bool(per_level)))
+    def set_schedule(self, show_nodes=16, per_level=False, river=True):
+        """Launch schedule (results are identical under every schedule):
+        river = fused river sweeps (one workgroup per river subtree, its rows
+        in LDS) where the engine allows them; for the level sweeps, showdown
+        nodes per workgroup (16, 8 or 4) and whether each level's terminals
+        launch with that level."""
+        sc = L.Schedule(show_nodes, int(bool(per_level)), int(bool(river)))
         L.check(L.lib().gto_hip_set_schedule(self._h, ctypes.byref(sc)), "set schedule")
 
     def schedule(self):
         sc = L.Schedule()
         L.check(L.lib().gto_hip_get_schedule(self._h, ctypes.byref(sc)), "get schedule")
-        return {"show_nodes": sc.show_nodes, "per_level": bool(sc.per_level)}
+        return schedule_dict(sc)
 
     def autotune(self, iters=2):
         """Time every candidate schedule over `iters` iterations (plus one
@@ -69,7 +75,13 @@ class GpuSolver(QueryMixin):
         ms = np.zeros(L.SCHEDULE_CANDIDATES, dtype=np.float32)
         L.check(L.lib().gto_hip_autotune(self._h, iters, ctypes.byref(best), ms.ctypes.data_as(L.c_f32p)),
                 "autotune")
-        return {"show_nodes": best.show_nodes, "per_level": bool(best.per_level)}, [float(x) for x in ms]
+        return schedule_dict(best), [float(x) for x in ms]
+
+    @property
+    def river_entries(self):
+        """Arena entries inside river subtrees (the fused river kernel's
+        share of reached_entries; 0 when fused sweeps are unavailable)."""
+        return int(L.lib().gto_hip_river_entries(self._h))
 
     def set_showdown(self, mode):
         """'exact' (f64 prefix sums, the CPU engine's arithmetic), 'bf16' or

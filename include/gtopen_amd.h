@@ -304,12 +304,21 @@ int gto_hip_set_showdown(gto_hip* h, int mode);
 typedef struct gto_schedule {
     uint32_t show_nodes;
     uint32_t per_level;
+    /* 1 = fused river sweeps (the default): one workgroup per river subtree
+     * runs the subtree's reach, terminals and updates with its rows in LDS,
+     * so river-level reach and value rows stay off HBM; the level sweeps
+     * cover the tree above. Applies to DCFR / CFR+ iterations with f32 or
+     * f16 arenas, the exact showdown and no locked node; otherwise (and
+     * with 0) every level is swept level-synchronously. Results are bit for
+     * bit those of the level sweeps. */
+    uint32_t river;
 } gto_schedule;
 int gto_hip_set_schedule(gto_hip* h, const gto_schedule* s);
 int gto_hip_get_schedule(const gto_hip* h, gto_schedule* s);
 
 /* Autotuner: for each of the GTO_SCHEDULE_CANDIDATES schedules (candidate
- * i: show_nodes = {16, 8, 4}[i % 3], per_level = i / 3), one warm-up
+ * i < 6: level sweeps with show_nodes = {16, 8, 4}[i % 3], per_level =
+ * i / 3; candidate 6: the fused river sweep), one warm-up
  * iteration (which captures the schedule's graph) and `iters` timed DCFR
  * iterations; the fastest schedule is then applied. The iterations are
  * ordinary iterations of the solve: it advances by
@@ -318,7 +327,7 @@ int gto_hip_get_schedule(const gto_hip* h, gto_schedule* s);
  * batch); best (may be NULL): the chosen schedule. The batch form applies
  * each candidate to every engine of the batch and times the batch's
  * parent-graph iterations. */
-#define GTO_SCHEDULE_CANDIDATES 6
+#define GTO_SCHEDULE_CANDIDATES 7
 int gto_hip_autotune(gto_hip* h, uint32_t iters, gto_schedule* best, float* ms);
 int gto_batch_autotune(gto_batch* b, uint32_t iters, gto_schedule* best, float* ms);
 
@@ -372,7 +381,7 @@ int gto_hip_synchronize(gto_hip* h);
 /* Per-kernel-kind device time over the last `gto_hip_profile_iterations`
  * call, in ms (HIP events around each launch group). Replaces
  * iterate_profiled / Profile (gpu/mod.rs:22-54, 351-362). */
-#define GTO_PROF_KINDS 8
+#define GTO_PROF_KINDS 9
 typedef struct gto_hip_profile {
     double ms[GTO_PROF_KINDS];
     uint32_t launches[GTO_PROF_KINDS];
@@ -412,6 +421,9 @@ int gto_hip_run(gto_hip* h, const gto_run_options* opts, const volatile int* sto
  * na * nh_actor, the unit of the roofline bytes (20 B per entry per f32
  * iteration, 10 B compressed). */
 uint64_t gto_hip_reached_entries(const gto_hip* h);
+/* The part of D inside river subtrees, which the fused river kernel streams
+ * (0 when the engine cannot run fused river sweeps). */
+uint64_t gto_hip_river_entries(const gto_hip* h);
 
 /* VRAM the engine would allocate for `spot` (gpu/mod.rs:946; game.rs:365-373). */
 uint64_t gto_hip_estimate_vram(const gto_spot_view* spot, int storage);

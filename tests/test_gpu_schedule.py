@@ -1,14 +1,17 @@
-"""Launch schedules and the autotuner. A schedule only changes how the up
-sweep's terminal work is launched (runs of 16, 8 or 4 showdown nodes per
-workgroup; all levels' terminals first, or each level's with that level),
-so every schedule, and any sequence of schedules chosen while tuning, must
-leave the arenas bit-identical to the default schedule's."""
+"""Launch schedules and the autotuner. A schedule only changes how work is
+launched: the fused river sweep (one workgroup per river subtree with its
+rows in LDS) or the level sweeps, and for the level sweeps the terminal
+launches (runs of 16, 8 or 4 showdown nodes per workgroup; all levels'
+terminals first, or each level's with that level). Every schedule, and any
+sequence of schedules chosen while tuning, must leave the arenas
+bit-identical to the default schedule's."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-CANDIDATES = [(16, False), (8, False), (4, False), (16, True), (8, True), (4, True)]
+CANDIDATES = [(16, False, False), (8, False, False), (4, False, False), (16, True, False), (8, True, False),
+              (4, True, False), (16, False, True)]
 
 
 def _say(msg):
@@ -26,16 +29,24 @@ def _same(a, b):
 
 @pytest.mark.parametrize("storage", ["f32", "half"])
 def test_every_schedule_gives_the_same_arenas(built, storage):
+    """Level sweeps under the 6 terminal schedules and the fused river sweep,
+    20 iterations each, on the river spot (one subtree: the whole tree),
+    the turn spot (48 river subtrees per river chance node, raises to a cap
+    of 10) and a report flop (1176 hands: six hands per thread)."""
     from gtopen_amd import GpuSolver, Spot, workloads
 
-    for name, cfg in (("river", workloads.river_spot()), ("turn", workloads.turn_spot())):
-        _say(f"case: {name} spot, {storage} arenas, 20 iterations under each of 6 schedules")
+    flop = workloads.batched_flops(n=184)[1]
+    for name, cfg in (("river", workloads.river_spot()), ("turn", workloads.turn_spot()), ("report flop", flop)):
         spot = Spot(cfg)
+        _say(f"case: {name} spot ({spot.num_nodes} nodes, hands {spot.num_hands}), {storage} arenas, "
+             f"20 iterations under each of {len(CANDIDATES)} schedules")
         ref = None
-        for show_nodes, per_level in CANDIDATES:
+        for show_nodes, per_level, river in CANDIDATES:
             g = GpuSolver(spot, storage=storage)
-            g.set_schedule(show_nodes, per_level)
-            assert g.schedule() == {"show_nodes": show_nodes, "per_level": per_level}
+            g.set_schedule(show_nodes, per_level, river)
+            assert g.schedule() == {"show_nodes": show_nodes, "per_level": per_level, "river": river}
+            if river:
+                assert g.river_entries > 0
             g.iterate(20)
             a = _arenas(g)
             e = g.exploitability()
@@ -43,24 +54,50 @@ def test_every_schedule_gives_the_same_arenas(built, storage):
             if ref is None:
                 ref = (a, e)
             else:
-                assert _same(ref[0], a), f"schedule ({show_nodes}, {per_level}) changed the arenas"
+                assert _same(ref[0], a), f"schedule ({show_nodes}, {per_level}, {river}) changed the arenas"
                 assert e == ref[1]
+
+
+def test_switching_between_fused_and_level_sweeps(built):
+    """The fused river sweep leaves river-level reach rows behind; a level
+    sweep after it recomputes them, so switching schedules mid-solve (as the
+    autotuner does) keeps the arenas bit-identical to a solve that never
+    switches; locks turn the fused sweep off and unlocking turns it back on."""
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    spot = Spot(workloads.turn_spot())
+    _say("case: turn spot, fused / level / fused sweeps in one solve vs level sweeps only")
+    a = GpuSolver(spot)
+    b = GpuSolver(spot)
+    b.set_schedule(16, False, False)
+    for river in (True, False, True, False):
+        a.set_schedule(16, False, river)
+        a.iterate(7)
+        b.iterate(7)
+        assert a.exploitability() == b.exploitability()
+    assert _same(_arenas(a), _arenas(b))
+    a.update_locks({})
+    a.iterate(3)
+    b.iterate(3)
+    assert _same(_arenas(a), _arenas(b))
+    a.close()
+    b.close()
 
 
 def test_autotune_is_an_ordinary_part_of_the_solve(built):
     from gtopen_amd import GpuSolver, Spot, workloads
 
-    _say("case: turn spot, autotune (1 timed iteration per candidate) vs 12 plain iterations")
+    _say("case: turn spot, autotune (1 timed iteration per candidate) vs the same number of plain iterations")
     spot = Spot(workloads.turn_spot())
     g = GpuSolver(spot)
     best, ms = g.autotune(1)
     _say(f"  chose {best}; ms per iteration per candidate {['%.3f' % x for x in ms]}")
-    assert (best["show_nodes"], best["per_level"]) in CANDIDATES
+    assert (best["show_nodes"], best["per_level"], best["river"]) in CANDIDATES
     assert g.schedule() == best
-    assert len(ms) == 6 and all(x > 0 for x in ms)
-    assert g.iteration == 12
+    assert len(ms) == len(CANDIDATES) and all(x > 0 for x in ms)
+    assert g.iteration == 2 * len(CANDIDATES)
     h = GpuSolver(spot)
-    h.iterate(12)
+    h.iterate(2 * len(CANDIDATES))
     assert _same(_arenas(g), _arenas(h))
     g.iterate(5)
     h.iterate(5)
@@ -79,14 +116,15 @@ def test_batch_autotune(built):
     b = Batch(gs)
     best, ms = b.autotune(1)
     _say(f"  chose {best}; ms per batch iteration per candidate {['%.3f' % x for x in ms]}")
-    assert (best["show_nodes"], best["per_level"]) in CANDIDATES
+    assert (best["show_nodes"], best["per_level"], best["river"]) in CANDIDATES
     assert all(g.schedule() == best for g in gs)
     b.iterate(4)
     b.close()
+    n = 2 * len(CANDIDATES) + 4
     for s, g in zip(spots, gs):
-        assert g.iteration == 16
+        assert g.iteration == n
         h = GpuSolver(s)
-        h.iterate(16)
+        h.iterate(n)
         assert _same(_arenas(g), _arenas(h))
         h.close()
         g.close()
