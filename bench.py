@@ -188,6 +188,8 @@ def main():
     ap.add_argument("--showdown", default="exact", choices=["exact", "bf16", "fp8"],
                     help="showdown evaluation: exact f64 prefix sums or the BF16 / FP8 matrix-core products")
     ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half"])
+    ap.add_argument("--river", default="auto", choices=["auto", "fused", "level"],
+                    help="with --autotune 0: the fused river sweep or the level sweeps (auto: the engine default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -221,6 +223,8 @@ def main():
 
     gpu = GpuSolver(spot, storage=args.storage, showdown=args.showdown)
     D, Dr = gpu.reached_entries, gpu.river_entries
+    if args.river != "auto":
+        gpu.set_schedule(16, False, args.river == "fused")
     gpu.iterate(args.warmup)
     gpu.synchronize()
     log(f"bench: warmup {args.warmup} done")

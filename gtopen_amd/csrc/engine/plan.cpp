@@ -192,7 +192,7 @@ static void build_river(const gto_spot_view& v, Plan& pl, const std::vector<uint
     std::vector<uint32_t> stack;
     for (const Root& r : roots) {
         // the subtree: pre-order, contiguous node ids
-        uint64_t base[2] = {~0ull, ~0ull};
+        uint64_t base[2] = {~0ull, ~0ull}, end[2] = {0, 0};
         uint32_t board = kSent;
         stack.assign(1, r.node);
         while (!stack.empty()) {
@@ -203,6 +203,7 @@ static void build_river(const gto_spot_view& v, Plan& pl, const std::vector<uint
             if (kind == GTO_KIND_ACTION) {
                 const int q = v.node_player[n];
                 base[q] = std::min<uint64_t>(base[q], v.node_data_offset[n]);
+                end[q] = std::max<uint64_t>(end[q], v.node_data_offset[n] + (uint64_t)v.node_num_children[n] * v.num_hands[q]);
                 pl.river_entries += (uint64_t)v.node_num_children[n] * v.num_hands[q];
                 for (uint32_t a = 0; a < v.node_num_children[n]; ++a)
                     stack.push_back(v.children[v.node_children_start[n] + a]);
@@ -225,6 +226,10 @@ static void build_river(const gto_spot_view& v, Plan& pl, const std::vector<uint
         sub.card = r.card;
         sub.cfv = pl.cfv_slot[r.node];
         sub.board = board;
+        for (int q = 0; q < 2; ++q) {
+            if (end[q] - base[q] > 0xFFFFFFFFull) pl.river_ok = false;
+            sub.size[q] = end[q] > base[q] ? (uint32_t)(end[q] - base[q]) : 0u;
+        }
         for (int p = 0; p < 2; ++p) {
             RiverProgram prog(v, p, r.node, base);
             const uint32_t out = prog.alloc(1);  // row 0: the subtree's value

@@ -75,7 +75,7 @@ struct RiverSub {
     uint32_t card;      // river card dealt into the subtree (52: none)
     uint32_t cfv;       // value row of the subtree root
     uint32_t board;     // river-board slot of its showdowns (0xFFFFFFFF: none)
-    uint32_t pad;
+    uint32_t size[2];   // arena entries of each player's action nodes in the subtree (from base)
 };
 static_assert(sizeof(RiverSub) == 72, "RiverSub layout");
 
