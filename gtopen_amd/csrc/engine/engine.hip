@@ -1060,7 +1060,7 @@ class Engine {
     ShowRec* show_all_ = nullptr;
     uint4* show_chunks_[3] = {nullptr, nullptr, nullptr};  // all-level runs per entry of kShowCaps
     uint32_t n_show_chunks_[3] = {0, 0, 0};
-    gto_schedule sched_{kShowCaps[0], 0, 1};
+    gto_schedule sched_{kShowCaps[0], 0, 0};
     // fused river sweeps: subtree records and programs, the LDS they need,
     // the upper tree's terminals; river-level reach rows are left stale by
     // fused sweeps (level sweeps recompute both players' rows after them)

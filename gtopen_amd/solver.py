@@ -53,7 +53,7 @@ class GpuSolver(QueryMixin):
         if showdown != "exact":
             self.set_showdown(showdown)
 
-    def set_schedule(self, show_nodes=16, per_level=False, river=True):
+    def set_schedule(self, show_nodes=16, per_level=False, river=False):
         """Launch schedule (results are identical under every schedule):
         river = fused river sweeps (one workgroup per river subtree, its rows
         in LDS) where the engine allows them; for the level sweeps, showdown

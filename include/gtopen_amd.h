@@ -304,13 +304,13 @@ int gto_hip_set_showdown(gto_hip* h, int mode);
 typedef struct gto_schedule {
     uint32_t show_nodes;
     uint32_t per_level;
-    /* 1 = fused river sweeps (the default): one workgroup per river subtree
+    /* 1 = fused river sweeps: one workgroup per river subtree
      * runs the subtree's reach, terminals and updates with its rows in LDS,
      * so river-level reach and value rows stay off HBM; the level sweeps
      * cover the tree above. Applies to DCFR / CFR+ iterations with f32 or
      * f16 arenas, the exact showdown and no locked node; otherwise (and
-     * with 0) every level is swept level-synchronously. Results are bit for
-     * bit those of the level sweeps. */
+     * with 0, the default) every level is swept level-synchronously.
+     * Results are bit for bit those of the level sweeps. */
     uint32_t river;
 } gto_schedule;
 int gto_hip_set_schedule(gto_hip* h, const gto_schedule* s);
