@@ -54,6 +54,10 @@ def cpu_info():
     return info
 
 
+DTYPE = {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math",
+         "half_rows": "f16 arenas and rows, f32 math"}
+
+
 def traffic_profile(args):
     """The committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
     workload (profiles/<config>_<storage>_traffic.json), if any."""
@@ -66,13 +70,14 @@ def measured_traffic(args, kind):
     prof = traffic_profile(args)
     if prof is None:
         return None
-    store = {"f32": 0, "compressed": 1, "half": 2}[args.storage]
+    store = {"f32": 0, "compressed": 1, "half": 2, "half_rows": 2}[args.storage]
+    h16 = "true" if args.storage == "half_rows" else "false"  # the kernels' f16-row template flag
     if kind == "river":
         keys = [k for k in prof["kernels"] if k.startswith("k_river<") and k.endswith(f", {store}>")]
         key = keys[0] if keys else None
     else:
-        key = {"up_action": "k_up_action<0, %d>" % store,
-               "down_action": "k_down_action<%d>" % {0: 0, 1: 2, 2: 5}[store]}.get(kind)
+        key = {"up_action": f"k_up_action<0, {store}, {h16}>",
+               "down_action": "k_down_action<%d, %s>" % ({0: 0, 1: 2, 2: 5}[store], h16)}.get(kind)
     k = prof["kernels"].get(key)
     if not k:
         return None
@@ -187,7 +192,7 @@ def main():
                     help="iterations per candidate launch schedule for the autotuner (0: default schedule)")
     ap.add_argument("--showdown", default="exact", choices=["exact", "bf16", "fp8"],
                     help="showdown evaluation: exact f64 prefix sums or the BF16 / FP8 matrix-core products")
-    ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half", "half_rows"])
     ap.add_argument("--river", default="auto", choices=["auto", "fused", "level"],
                     help="with --autotune 0: the fused river sweep or the level sweeps (auto: the engine default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -277,17 +282,21 @@ def main():
             + ("" if e64_final is None else f"; FP64 judge {e64_final:.4f}% pot"))
 
     # the precision forms on the same spot: f16 arenas (this build's format)
-    # with the exact, BF16 and FP8 showdowns, and the reference's i16/u16
+    # with f32 or f16 rows and the exact showdown, with the BF16 and FP8
+    # showdowns, and the reference's i16/u16
     # arenas; each final strategy judged by the FP64 oracle
     mixed = None
     if not args.no_target and not args.no_mixed and args.storage == "f32":
         mixed = {}
-        for key, st, sd, cap in (("half", "half", "exact", 2000), ("half_bf16_showdown", "half", "bf16", 2000),
+        for key, st, sd, cap in (("half", "half", "exact", 2000), ("half_rows", "half_rows", "exact", 2000),
+                                 ("half_bf16_showdown", "half", "bf16", 2000),
                                  ("half_fp8_showdown", "half", "fp8", 1000), ("compressed", "compressed", "exact", 800)):
             ms, _, _ = gpu_rate(spot, args.steps, args.warmup, st, sd)
             sec, its_t, e_t, e64 = time_to_target(spot, st, args.target, cap, sd, sched, judge=judge)
             reached = e_t <= args.target
-            mixed[key] = {"arenas": {"half": "f16 x per-node power of two", "compressed": "i16/u16 (reference)"}[st],
+            mixed[key] = {"arenas": {"half": "f16 x per-node power of two", "half_rows": "f16 x per-node power of two",
+                                     "compressed": "i16/u16 (reference)"}[st],
+                          "rows": "f16 x per-row power of two" if st == "half_rows" else "f32",
                           "showdown": {"exact": "f64 prefix sums", "bf16": "BF16 matrix-core products, f32 accumulate",
                                        "fp8": "FP8 (e4m3, per-node 2^k scale) matrix-core products, f32 accumulate"}[sd],
                           "ms_per_step": round(ms, 4), "iters_per_s": round(1e3 / ms, 3),
@@ -364,7 +373,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math"}[args.storage]
+        "dtype": DTYPE[args.storage]
                  + ("" if args.showdown == "exact" else f", {args.showdown} showdown products"),
         "data": "synthetic",
         "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
@@ -531,7 +540,7 @@ def bench_batched(args):
         "value": round(per_hour, 2), "unit": "flops/hour", "n_gpus": 1, "steps": args.flop_iters, "warmup": 0,
         "ms_per_step": round(dt * 1e3 / max(1, args.flop_iters), 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 arenas, f32 math"}[args.storage],
+        "dtype": DTYPE[args.storage],
         "data": "synthetic",
         "config": {"workload": f"batched_flops: {len(cfgs)} seeded canonical flops, per-flop menus from "
                                "{25,33,50,75,125,a}, 1176-combo Beta(0.7,0.7) ranges",

@@ -21,8 +21,9 @@ c_f64p = ctypes.POINTER(ctypes.c_double)
 GTO_OK = 0
 KIND_ACTION, KIND_CHANCE, KIND_FOLD, KIND_SHOWDOWN = 0, 1, 2, 3
 SENTINEL = 0xFFFFFFFF
-STORAGE_F32, STORAGE_COMPRESSED, STORAGE_HALF = 0, 1, 2
-STORAGE = {"f32": STORAGE_F32, "compressed": STORAGE_COMPRESSED, "half": STORAGE_HALF}
+STORAGE_F32, STORAGE_COMPRESSED, STORAGE_HALF, STORAGE_HALF_ROWS = 0, 1, 2, 3
+STORAGE = {"f32": STORAGE_F32, "compressed": STORAGE_COMPRESSED, "half": STORAGE_HALF,
+           "half_rows": STORAGE_HALF_ROWS}
 ALGO_DCFR, ALGO_CFRPLUS, ALGO_PCFRPLUS = 0, 1, 2
 
 

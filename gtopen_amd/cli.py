@@ -5,16 +5,16 @@ output (bench/make_chart.py:34-36) read these runs unchanged.
 
     python -m gtopen_amd.cli <config.json> [max_iterations] [target_exploit_pct]
 
-Environment (solve_cli.rs:3-11): SOLVER_STORAGE=f32|i16|half (arena
-storage on the device; i16 is the reference's compressed form; default
-f32), SOLVER_ALGO=dcfr|cfr+|pcfr+, SOLVER_ISO=0 disables suit isomorphism.
+Environment (solve_cli.rs:3-11): SOLVER_STORAGE=f32|i16|half|half_rows
+(arena storage on the device; i16 is the reference's compressed form,
+half_rows f16 arenas with f16 rows in the iterations; default f32), SOLVER_ALGO=dcfr|cfr+|pcfr+, SOLVER_ISO=0 disables suit isomorphism.
 Exploitability is checked every 10 iterations, as solve-cli's GPU path does.
 """
 import os
 import sys
 import time
 
-STORAGE_NAMES = {"f32": "F32", "compressed": "Compressed", "half": "Half"}
+STORAGE_NAMES = {"f32": "F32", "compressed": "Compressed", "half": "Half", "half_rows": "HalfRows"}
 ALGO_NAMES = {0: "Dcfr", 1: "CfrPlus", 2: "PcfrPlus"}
 
 
@@ -56,7 +56,8 @@ def main(argv=None):
     from .solver import GpuSolver
     from .spot import Spot
 
-    storage = {"f32": "f32", "i16": "compressed", "compressed": "compressed", "half": "half", "f16": "half"}[
+    storage = {"f32": "f32", "i16": "compressed", "compressed": "compressed", "half": "half", "f16": "half",
+               "half_rows": "half_rows"}[
         os.environ.get("SOLVER_STORAGE", "f32")]
     algo = os.environ.get("SOLVER_ALGO", "dcfr")
     iso = os.environ.get("SOLVER_ISO", "1") != "0"

@@ -128,10 +128,12 @@ void discounts(int algo, uint32_t it, float* out3) {
 class Engine {
    public:
     Engine(const gto_spot_view& v, const gto_hip_options& o) : opts_(o) {
-        if (o.storage != GTO_STORAGE_F32 && o.storage != GTO_STORAGE_COMPRESSED && o.storage != GTO_STORAGE_HALF)
+        if (o.storage != GTO_STORAGE_F32 && o.storage != GTO_STORAGE_COMPRESSED && o.storage != GTO_STORAGE_HALF &&
+            o.storage != GTO_STORAGE_HALF_ROWS)
             throw std::runtime_error("unknown storage mode");
         compressed_ = o.storage != GTO_STORAGE_F32;
-        half_ = o.storage == GTO_STORAGE_HALF;
+        half_ = o.storage == GTO_STORAGE_HALF || o.storage == GTO_STORAGE_HALF_ROWS;
+        rows16_ = o.storage == GTO_STORAGE_HALF_ROWS;
         pred_ = o.algo == GTO_ALGO_PCFRPLUS;
         if (pred_ && compressed_) throw std::runtime_error("PCFR+ runs with f32 arenas on the GPU");
         // fault injection: GTO_INJECT_OOM=1 reports a device allocation
@@ -293,7 +295,7 @@ class Engine {
             // regret matching, values weighted by the traverser's sigma
             launch_root(1 << o);
             launch_down(1 << o, kCurrentF32);
-            reach_valid_[o] = true;
+            reach_valid_[o] = !rows16_;  // f32 rows: the DCFR sweeps of f16 rows recompute them
             launch_up(p, 3);
         } else {
             launch_root(1 << o);
@@ -385,7 +387,7 @@ class Engine {
         launch_root(1 << o);
         if (mode == 2) {
             launch_down(1 << o, kCurrentF32);
-            reach_valid_[o] = true;
+            reach_valid_[o] = !rows16_;  // f32 rows: the DCFR sweeps of f16 rows recompute them
             launch_up(p, 3);
         } else {
             launch_down(1 << o, kAverageF32);
@@ -641,6 +643,10 @@ class Engine {
             HIP_OK(hipMemcpy(lock_sigma_buf_, sigma, sizeof(float) * n_sigma, hipMemcpyHostToDevice));
         }
         T_.lock_sigma = lock_sigma_buf_;
+        float lb = 1.0f;
+        for (uint64_t k = 0; k < n_sigma; ++k)
+            if (std::isfinite(sigma[k])) lb = std::max(lb, std::fabs(sigma[k]));
+        T_.lock_bound = lb;
         // patch the action records' lock offsets
         for (size_t l = 0; l < plan_.levels.size(); ++l)
             for (int q = 0; q < 2; ++q) {
@@ -716,6 +722,8 @@ class Engine {
     void set_showdown(int mode) {
         if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16 && mode != GTO_SHOWDOWN_FP8)
             throw std::runtime_error("unknown showdown mode");
+        if (mode != GTO_SHOWDOWN_EXACT && rows16_)
+            throw std::invalid_argument("the matrix-core showdown forms read f32 rows: not with f16 rows");
         if (mode != GTO_SHOWDOWN_EXACT) {
             const bool fp8 = mode == GTO_SHOWDOWN_FP8;
             build_mma_tables();
@@ -829,7 +837,7 @@ class Engine {
         }
         const uint64_t D = v.data_size[0] + v.data_size[1];
         uint64_t arenas = storage == GTO_STORAGE_F32 ? D * 8 : D * 4 + n * 16;
-        if (storage == GTO_STORAGE_HALF) arenas += n * 16;                 // per-node maxima
+        if (storage == GTO_STORAGE_HALF || storage == GTO_STORAGE_HALF_ROWS) arenas += n * 16;  // per-node maxima
         uint64_t reach = 0;
         for (int q = 0; q < 2; ++q) reach += slots[q] * (nh[q] * 4 + 9);    // row, alive, max (and slack)
         const uint64_t cfv = n * (nhm * 4 + 4);                               // value rows and their maxima
@@ -1065,6 +1073,9 @@ class Engine {
     // the upper tree's terminals; river-level reach rows are left stale by
     // fused sweeps (level sweeps recompute both players' rows after them)
     bool river_avail_ = false, rows_stale_ = false, locked_ = false;
+    // GTO_STORAGE_HALF_ROWS: the DCFR sweeps keep reach and value rows in f16
+    // (per-row power-of-two scales); other sweeps use f32 rows
+    bool rows16_ = false;
     RiverSub* river_subs_ = nullptr;
     RiverOp* river_ops_ = nullptr;
     uint32_t n_river_ = 0, river_mo_ = 0;
@@ -1113,6 +1124,7 @@ class Engine {
         if (plan_.max_actions > (uint32_t)kMaxActions)
             throw std::runtime_error("a node has more than 64 actions; the GPU engine supports at most 64");
         T_.lock_sigma = nullptr;
+        T_.lock_bound = 1.0f;
         T_.kids = upload_vec(plan_.kids);
         T_.chr_card = upload_vec(plan_.chr_card);
         T_.chr_cfv = upload_vec(plan_.chr_cfv);
@@ -1166,11 +1178,14 @@ class Engine {
         if (lds_max > kLdsBudget)
             throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
         for (const void* f : {(const void*)k_up_show<3, kShortSeg>, (const void*)k_up_show<3, kMaxCardList / 4>,
-                              (const void*)k_up_show<kRowPer, kMaxCardList / 4>})
+                              (const void*)k_up_show<kRowPer, kMaxCardList / 4>,
+                              (const void*)k_up_show<3, kShortSeg, true>, (const void*)k_up_show<3, kMaxCardList / 4, true>,
+                              (const void*)k_up_show<kRowPer, kMaxCardList / 4, true>})
             HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)std::max(show_lds_[0], show_lds_[1])));
-        HIP_OK(hipFuncSetAttribute((const void*)k_up_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)std::max(fold_lds_[0], fold_lds_[1])));
+        for (const void* f : {(const void*)k_up_fold<false>, (const void*)k_up_fold<true>})
+            HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)std::max(fold_lds_[0], fold_lds_[1])));
         n_fold_all_ = (uint32_t)plan_.fold_all.size();
         if (n_fold_all_) fold_all_ = upload_vec(plan_.fold_all);
         n_show_chunks_mma_ = (uint32_t)(plan_.show_chunks_mma.size() / 4);
@@ -1219,7 +1234,7 @@ class Engine {
     // within the workgroup's 160 KB.
     void upload_river() {
         river_avail_ = false;
-        if (!plan_.river_ok || plan_.river_subs.empty() || (compressed_ && !half_) || pred_) return;
+        if (!plan_.river_ok || plan_.river_subs.empty() || (compressed_ && !half_) || pred_ || rows16_) return;
         river_mo_ = std::max(plan_.max_board_cnt[0], plan_.max_board_cnt[1]);
         river_lds_ = river_lds_layout(plan_.river_rows, plan_.nh_max, river_mo_).bytes;
         if (river_lds_ > kLdsBudget) return;
@@ -1279,6 +1294,15 @@ class Engine {
         }
         S_.cfv = (float*)dalloc(sizeof(float) * (size_t)plan_.num_reached * plan_.nh_max);
         S_.vmax = nullptr;
+        S_.rsc[0] = S_.rsc[1] = S_.vsc = nullptr;
+        if (rows16_) {
+            for (int q = 0; q < 2; ++q) {
+                S_.rsc[q] = (float*)dalloc(sizeof(float) * std::max<uint32_t>(plan_.num_slots[q], 1));
+                HIP_OK(hipMemset(S_.rsc[q], 0, sizeof(float) * std::max<uint32_t>(plan_.num_slots[q], 1)));
+            }
+            S_.vsc = (float*)dalloc(sizeof(float) * std::max<uint32_t>(plan_.num_reached, 1));
+            HIP_OK(hipMemset(S_.vsc, 0, sizeof(float) * std::max<uint32_t>(plan_.num_reached, 1)));
+        }
         if (half_) {
             S_.vmax = (float*)dalloc(sizeof(float) * std::max<uint32_t>(plan_.num_reached, 1));
             HIP_OK(hipMemset(S_.vmax, 0, sizeof(float) * std::max<uint32_t>(plan_.num_reached, 1)));
@@ -1356,10 +1380,14 @@ class Engine {
         evs_.clear();
     }
 
-    void launch_root(int mask) { launch(PK_ROOT, k_root, 1, 0, T_, S_, mask); }
+    void launch_root(int mask, bool h16 = false) {
+        if (h16) launch(PK_ROOT, k_root<true>, 1, 0, T_, S_, mask);
+        else launch(PK_ROOT, k_root<false>, 1, 0, T_, S_, mask);
+    }
 
     // upper: only the nodes outside river subtrees (fused river sweeps)
-    void launch_down(int mask, int kind, bool upper = false) {
+    // h16: f16 rows (the DCFR sweeps of GTO_STORAGE_HALF_ROWS)
+    void launch_down(int mask, int kind, bool upper = false, bool h16 = false) {
         for (size_t l = 0; l < lv_.size(); ++l) {
             const DevLevel& D = lv_[l];
             for (int q = 0; q < 2; ++q) {
@@ -1374,15 +1402,18 @@ class Engine {
                     else if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, g, 0, T_, S_, recs, n, q);
                     else launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, g, 0, T_, S_, recs, n, q);
                 } else {
-                    if (half_) launch(PK_DOWN_ACTION, k_down_action<kCurrentH>, g, 0, T_, S_, recs, n, q);
+                    if (h16) launch(PK_DOWN_ACTION, k_down_action<kCurrentH, true>, g, 0, T_, S_, recs, n, q);
+                    else if (half_) launch(PK_DOWN_ACTION, k_down_action<kCurrentH>, g, 0, T_, S_, recs, n, q);
                     else if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kCurrentQ>, g, 0, T_, S_, recs, n, q);
                     else launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, g, 0, T_, S_, recs, n, q);
                 }
             }
             const uint32_t ne = upper ? D.n_edge_up : D.n_edge;
-            if (ne)
-                launch(PK_DOWN_CHANCE, k_down_chance, wave_grid(ne), 0, T_, S_,
-                       (const EdgeRec*)(upper ? D.edge_up : D.edge), ne, mask);
+            if (ne) {
+                const EdgeRec* er = (const EdgeRec*)(upper ? D.edge_up : D.edge);
+                if (h16) launch(PK_DOWN_CHANCE, k_down_chance<true>, wave_grid(ne), 0, T_, S_, er, ne, mask);
+                else launch(PK_DOWN_CHANCE, k_down_chance<false>, wave_grid(ne), 0, T_, S_, er, ne, mask);
+            }
         }
     }
 
@@ -1419,7 +1450,7 @@ class Engine {
         std::exception_ptr err;
         try {
             if (fold)
-                launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_up_)), fold_lds_[p], T_, S_,
+                launch(PK_UP_FOLD, k_up_fold<false>, wave_grid(fold_waves(n_fold_up_)), fold_lds_[p], T_, S_,
                        (const FoldRec*)fold_up_, n_fold_up_, p, 0);
         } catch (...) {
             err = std::current_exception();
@@ -1434,7 +1465,7 @@ class Engine {
                     launch(PK_RIVER, river_kernel<kStoreF32>(), g, river_lds_, T_, S_, (const RiverSub*)river_subs_,
                            n_river_, (const RiverOp*)river_ops_, p, plan_.river_rows, river_mo_);
                 if (n_show_up_chunks_[0])
-                    launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_up_chunks_[0], 1), show_lds_[p], T_, S_,
+                    launch(PK_UP_SHOW, show_kernel(false), wave_grid(n_show_up_chunks_[0], 1), show_lds_[p], T_, S_,
                            (const ShowRec*)show_up_, (const uint4*)show_up_chunks_[0], n_show_up_chunks_[0], p, 0);
             } catch (...) {
                 err = std::current_exception();
@@ -1448,8 +1479,8 @@ class Engine {
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
             if (D.n_chance)
-                launch_b(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), kChanceBlock, 0, T_, S_,
-                         (const ChanceRec*)D.chance, D.n_chance, p);
+                launch_b(PK_UP_CHANCE, k_up_chance<false>, wave_grid(D.n_chance, 1),
+                         kChanceBlock, 0, T_, S_, (const ChanceRec*)D.chance, D.n_chance, p);
             for (int q = 0; q < 2; ++q) {
                 if (!D.n_act_up[q]) continue;
                 const ActRec* recs = D.act_up[q];
@@ -1460,12 +1491,21 @@ class Engine {
         }
     }
 
-    ShowFn show_kernel(int) const {
+    // h16: f16 rows
+    ShowFn show_kernel(bool h16) const {
+        if (h16) {
+            if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4, true>;
+            return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg, true>
+                                                    : k_up_show<3, kMaxCardList / 4, true>;
+        }
         if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4>;
         return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg> : k_up_show<3, kMaxCardList / 4>;
     }
 
     void launch_up(int p, int mode, int swap = 0) {
+        // the DCFR sweeps of f16-row engines read and write f16 rows
+        const bool h16 = rows16_ && mode == 0;
+        const auto fold_fn = h16 ? k_up_fold<true> : k_up_fold<false>;
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
         if (mma)
             launch_b(PK_UP_SHOW, show_mode_ == GTO_SHOWDOWN_FP8 ? k_up_show_mma<true> : k_up_show_mma<false>,
@@ -1489,7 +1529,7 @@ class Engine {
             std::exception_ptr err;
             try {
                 if (fold)
-                    launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_,
+                    launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_,
                            (const FoldRec*)fold_all_, n_fold_all_, p, swap);
             } catch (...) {
                 err = std::current_exception();
@@ -1497,7 +1537,7 @@ class Engine {
             lstream_ = stream_;
             if (!err) try {
                     if (show)
-                        launch(PK_UP_SHOW, show_kernel(p), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
+                        launch(PK_UP_SHOW, show_kernel(h16), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
                                (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_],
                                p, swap);
                 } catch (...) {
@@ -1515,22 +1555,25 @@ class Engine {
             if (sched_.per_level) {
                 // this level's terminals, just before the kernels that read them
                 if (D.n_fold)
-                    launch(PK_UP_FOLD, k_up_fold, wave_grid(fold_waves(D.n_fold)), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
+                    launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(D.n_fold)), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
                            D.n_fold, p, swap);
                 if (!mma && D.n_show_chunks[cap_])
-                    launch(PK_UP_SHOW, show_kernel(p), wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
+                    launch(PK_UP_SHOW, show_kernel(h16), wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
                            (const ShowRec*)D.show, (const uint4*)D.show_chunks[cap_], D.n_show_chunks[cap_], p, swap);
             }
             if (D.n_chance)
-                launch_b(PK_UP_CHANCE, k_up_chance, wave_grid(D.n_chance, 1), kChanceBlock, 0, T_, S_,
-                         (const ChanceRec*)D.chance, D.n_chance, p);
+                launch_b(PK_UP_CHANCE, h16 ? k_up_chance<true> : k_up_chance<false>, wave_grid(D.n_chance, 1),
+                         kChanceBlock, 0, T_, S_, (const ChanceRec*)D.chance, D.n_chance, p);
             for (int q = 0; q < 2; ++q) {
                 if (!D.n_act[q]) continue;
                 const ActRec* recs = D.act[q];
                 const uint32_t n = D.n_act[q], g = wave_grid(n);
                 if (half_) {
                     switch (mode) {
-                        case 0: launch(PK_UP_ACTION, k_up_action<0, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
+                        case 0:
+                            if (h16) launch(PK_UP_ACTION, k_up_action<0, kStoreH, true>, g, 0, T_, S_, recs, n, p, q);
+                            else launch(PK_UP_ACTION, k_up_action<0, kStoreH>, g, 0, T_, S_, recs, n, p, q);
+                            break;
                         case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
                         case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
                         default: launch(PK_UP_ACTION, k_up_action<3, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
@@ -1575,8 +1618,8 @@ class Engine {
             for (int q = 0; q < 2; ++q)
                 if (!reach_valid_[q]) mask |= 1 << q;
             if (mask) {
-                launch_root(mask);
-                launch_down(mask, kCurrentF32, fu);
+                launch_root(mask, rows16_);
+                launch_down(mask, kCurrentF32, fu, rows16_);
             }
             reach_valid_[0] = reach_valid_[1] = true;
             if (fu) launch_up_fused(p);
@@ -1643,6 +1686,9 @@ int guarded(F&& f) {
     } catch (const DeviceOutOfMemory& e) {
         set_last_error(e.what());
         return GTO_ERR_MEMORY;
+    } catch (const std::invalid_argument& e) {
+        set_last_error(e.what());
+        return GTO_ERR_INVALID;
     } catch (const std::exception& e) {
         set_last_error(e.what());
         return GTO_ERR_DEVICE;

@@ -35,6 +35,7 @@ __host__ __device__ inline float half_scale(float b) {
 // Static hand and table data (read-only for the whole solve).
 struct DevTree {
     const float* lock_sigma;
+    float lock_bound;          // max(1, largest lock sigma): a locked child's reach / value bound factor
     const uint32_t* kids;      // ActRec children: value rows, then reach slots
     const uint8_t* chr_card;   // ChanceRec children: dealt card
     const uint32_t* chr_cfv;   // ChanceRec children: value row (the orbit representative's)
@@ -122,7 +123,7 @@ __host__ __device__ inline RiverLds river_lds_layout(uint32_t nrows, uint32_t nh
 constexpr size_t kLdsBudget = 160 * 1024;  // LDS per workgroup on gfx950
 
 struct MmaLds {
-    uint32_t B, bsc, nodes, vmx, oidx, pidx, lbub, O, os, bytes, kpad, rs, words;
+    uint32_t B, bsc, nodes, vmx, onb, oidx, pidx, lbub, O, os, bytes, kpad, rs, words;
 };
 // fp8 == false: B in bf16, [node][k], k padded to 16, row stride rs
 // elements. fp8 == true: B in e4m3 bytes, k padded to 32 and permuted within
@@ -141,7 +142,8 @@ __host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp, uint3
     L.bsc = L.B + (fp8 ? 1 : 2) * 32 * L.rs;  // 32 f32: per-node inverse scale of B
     L.nodes = L.bsc + 32 * 4;             // 32 ShowNode
     L.vmx = L.nodes + 32 * 24;            // 32 u32
-    L.lbub = L.vmx + 32 * 4;              // mp u32: rank bounds of p's sorted position i
+    L.onb = L.vmx + 32 * 4;               // 42 u32: bitmap of p's hands on the board (nout == 0)
+    L.lbub = L.onb + 42 * 4;              // mp u32: rank bounds of p's sorted position i
     L.oidx = L.lbub + 4 * mp;             // kpad u16: opponent hand of sorted position k
     L.pidx = L.oidx + 2 * L.kpad;         // mp u16: p's hand of sorted position i
     // 32 x os f32 output rows; os odd so the 32 node rows of one scattered
@@ -171,6 +173,11 @@ struct DevState {
     float* rmax[2];       // [slot]: max of the reach row
     float* vmax;          // [value row]: max |value| of the row (f16 engines; null otherwise)
     float* cfv;           // [cfv_slot][nh_max], traverser's values
+    // f16 rows (DCFR sweeps of GTO_STORAGE_HALF_ROWS): reach and value rows
+    // hold f16 entries at the same slot positions (half the bytes), each
+    // row with a power-of-two decode scale: value = f16 entry x scale
+    float* rsc[2];        // [slot]: reach row decode scale
+    float* vsc;           // [cfv_slot]: value row decode scale
     const float* disc;    // [iteration][3]: pos, neg, strat
     uint32_t* iter;       // device iteration counter (1-based after the first begin)
 };

@@ -76,6 +76,71 @@ __device__ __forceinline__ float h2f(uint16_t u) { return (float)__builtin_bit_c
 // round to nearest even (IEEE conversion)
 __device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
+// Reach and value rows of the DCFR sweeps. H16 = false: f32 entries. H16 =
+// true (GTO_STORAGE_HALF_ROWS): f16 entries at the same slot positions, the
+// row's value = entry x its power-of-two decode scale; a writer picks the
+// scale from a bound of the row's magnitude (half_scale: bound / scale <
+// 2^15), so encoding is one rounding to f16 (2^-11 relative) and decoding
+// is exact.
+// raw(i) / dec(raw, scale): an entry as stored and its value, for loads
+// issued ahead of their use (a prefetched row stays raw in registers until
+// it is used, so converting it does not wait for the load early).
+template <bool H16>
+struct RowIn {
+    using Raw = float;
+    const float* __restrict__ p;
+    float sc;
+    __device__ __forceinline__ RowIn() : p(nullptr), sc(1.0f) {}
+    __device__ __forceinline__ RowIn(const float* base, size_t off, float) : p(base + off), sc(1.0f) {}
+    __device__ __forceinline__ float operator[](uint32_t i) const { return p[i]; }
+    __device__ __forceinline__ Raw raw(uint32_t i) const { return p[i]; }
+    __device__ static __forceinline__ float dec(Raw x, float) { return x; }
+};
+template <>
+struct RowIn<true> {
+    // a full register per entry: entries packed two to a register would
+    // make every conditional load wait for the previous one
+    using Raw = uint32_t;
+    const uint16_t* __restrict__ p;
+    float sc;
+    __device__ __forceinline__ RowIn() : p(nullptr), sc(1.0f) {}
+    __device__ __forceinline__ RowIn(const float* base, size_t off, float s)
+        : p((const uint16_t*)base + off), sc(s) {}
+    __device__ __forceinline__ float operator[](uint32_t i) const { return h2f(p[i]) * sc; }
+    __device__ __forceinline__ Raw raw(uint32_t i) const { return p[i]; }
+    __device__ static __forceinline__ float dec(Raw x, float s) { return h2f((uint16_t)x) * s; }
+};
+// decode scale and encode factor of rows whose |entries| are <= bound
+struct RowEnc {
+    float sc, inv;
+    __device__ __forceinline__ explicit RowEnc(float bound) : sc(half_scale(bound)), inv(1.0f / sc) {}
+};
+template <bool H16>
+struct RowOut {
+    float* __restrict__ p;
+    __device__ __forceinline__ RowOut() : p(nullptr) {}
+    __device__ __forceinline__ RowOut(float* base, size_t off, const RowEnc&) : p(base + off) {}
+    __device__ __forceinline__ void set(uint32_t i, float x) const { p[i] = x; }
+};
+template <>
+struct RowOut<true> {
+    uint16_t* __restrict__ p;
+    float inv;
+    __device__ __forceinline__ RowOut() : p(nullptr), inv(1.0f) {}
+    __device__ __forceinline__ RowOut(float* base, size_t off, const RowEnc& e)
+        : p((uint16_t*)base + off), inv(e.inv) {}
+    __device__ __forceinline__ void set(uint32_t i, float x) const { p[i] = f2h(x * inv); }
+};
+// row of reach slot `slot` of player q / of value row `slot`
+template <bool H16>
+__device__ __forceinline__ RowIn<H16> reach_in(const DevTree& T, const DevState& S, int q, uint32_t slot) {
+    return RowIn<H16>(S.reach[q], (size_t)slot * T.nh[q], H16 ? S.rsc[q][slot] : 1.0f);
+}
+template <bool H16>
+__device__ __forceinline__ RowIn<H16> cfv_in(const DevTree& T, const DevState& S, uint32_t slot) {
+    return RowIn<H16>(S.cfv, (size_t)slot * T.nh_max, H16 ? S.vsc[slot] : 1.0f);
+}
+
 template <int KIND>
 struct SigmaHand;
 
@@ -572,6 +637,7 @@ __device__ __forceinline__ float wave_max_f32(float x) {
 // ---------------------------------------------------------------------------
 // Root: reach = range weights.
 // ---------------------------------------------------------------------------
+template <bool H16>
 __global__ __launch_bounds__(kBlock) void k_root(DevTree T, DevState S, int mask) {
     __shared__ float pmax[kWaves];
     for (int q = 0; q < 2; ++q) {
@@ -580,16 +646,23 @@ __global__ __launch_bounds__(kBlock) void k_root(DevTree T, DevState S, int mask
         float mx = 0.0f;
         for (uint32_t i = threadIdx.x; i < T.nh[q]; i += kBlock) {
             float w = T.weights[q][i];
-            S.reach[q][i] = w;
+            if constexpr (!H16) S.reach[q][i] = w;
             any |= w > 0.0f;
             mx = fmaxf(mx, w);
         }
         mx = wave_max_f32(mx);
         if ((threadIdx.x & 63) == 0) pmax[threadIdx.x >> 6] = mx;
         any = __syncthreads_or(any);
+        mx = fmaxf(fmaxf(pmax[0], pmax[1]), fmaxf(pmax[2], pmax[3]));
+        if constexpr (H16) {  // the row, encoded with the scale of its maximum
+            const RowEnc enc(mx);
+            const RowOut<true> out(S.reach[q], 0, enc);
+            for (uint32_t i = threadIdx.x; i < T.nh[q]; i += kBlock) out.set(i, T.weights[q][i]);
+            if (threadIdx.x == 0) S.rsc[q][0] = enc.sc;
+        }
         if (threadIdx.x == 0) {
             S.alive[q][0] = any ? 1 : 0;
-            S.rmax[q][0] = fmaxf(fmaxf(pmax[0], pmax[1]), fmaxf(pmax[2], pmax[3]));
+            S.rmax[q][0] = mx;
         }
         __syncthreads();
     }
@@ -611,7 +684,7 @@ __global__ void k_begin_iteration(DevState S) {
 // (cfr.rs:478-492, 509-515) and the child's "any reach > 0" flag, which is
 // the zero-reach pruning test of cfr.rs:424-430 for the subtree below.
 // ---------------------------------------------------------------------------
-template <int KIND>
+template <int KIND, bool H16 = false>
 __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
                                                         uint32_t count, int q) {
     const uint32_t w = wave_item();
@@ -621,20 +694,24 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
     const uint32_t nh = T.nh[q];
     const int na = r.na;
     const uint32_t* kid_own = T.kids + r.kid + na;
-    float* __restrict__ reach = S.reach[q];
     const uint32_t src = pick(r.own, q);
+    // f16 rows: the children share one scale, bounded by the parent's
+    // maximum (sigma <= 1; a lock's entries by the lock bound)
+    const RowEnc enc(H16 ? S.rmax[q][src] * (r.lock_off >= 0 ? T.lock_bound : 1.0f) : 0.0f);
+    auto child = [&](int a) { return RowOut<H16>(S.reach[q], (size_t)kid_own[a] * nh, enc); };
     if (!S.alive[q][src]) {
         for (int a = 0; a < na; ++a) {
-            float* dst = reach + (size_t)kid_own[a] * nh;
-            for (uint32_t i = lane; i < nh; i += kWave) dst[i] = 0.0f;
+            const RowOut<H16> dst = child(a);
+            for (uint32_t i = lane; i < nh; i += kWave) dst.set(i, 0.0f);
         }
         for (int a = lane; a < na; a += kWave) {
             S.alive[q][kid_own[a]] = 0;
             S.rmax[q][kid_own[a]] = 0.0f;
+            if constexpr (H16) S.rsc[q][kid_own[a]] = enc.sc;
         }
         return;
     }
-    const float* __restrict__ pr = reach + (size_t)src * nh;
+    const RowIn<H16> pr = reach_in<H16>(T, S, q, src);
     uint64_t anym = 0;
     // per-child reach maxima (the f16 scale bounds); the first kMaxUnroll
     // children in registers, the rest from memory below
@@ -647,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
             const float pv = pr[i];
             for (int a = 0; a < na; ++a) {
                 float v = pv * sig[(size_t)a * nh + i];
-                reach[(size_t)kid_own[a] * nh + i] = v;
+                child(a).set(i, v);
                 anym |= (uint64_t)(v > 0.0f) << a;
             }
         }
@@ -656,9 +733,9 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
         dispatch_na(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
             if constexpr (NA > 0) {
-                float* dst[NA];
+                RowOut<H16> dst[NA];
 #pragma unroll
-                for (int a = 0; a < NA; ++a) dst[a] = reach + (size_t)kid_own[a] * nh;
+                for (int a = 0; a < NA; ++a) dst[a] = child(a);
                 for (uint32_t i0 = lane; i0 < nh; i0 += kDownHands * kWave) {
                     HandSigma<NA, KIND> hs[kDownHands];
                     float pv[kDownHands];
@@ -676,7 +753,7 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
 #pragma unroll
                         for (int a = 0; a < NA; ++a) {
                             float v = pv[h] * hs[h].sig[a];
-                            dst[a][i] = v;
+                            dst[a].set(i, v);
                             anym |= (uint64_t)(v > 0.0f) << a;
                             mmax[a] = fmaxf(mmax[a], v);
                         }
@@ -689,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
                     const float pv = pr[i];
                     for (int a = 0; a < na; ++a) {
                         float v = pv * sh.template sigma<false>(a);
-                        reach[(size_t)kid_own[a] * nh + i] = v;
+                        child(a).set(i, v);
                         anym |= (uint64_t)(v > 0.0f) << a;
                     }
                 }
@@ -706,13 +783,14 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
         } else {
             // this wavefront's own writes above: read back in order
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            const float* row = reach + (size_t)kid_own[a] * nh;
+            const RowIn<H16> row(S.reach[q], (size_t)kid_own[a] * nh, enc.sc);
             for (uint32_t i = lane; i < nh; i += kWave) mx = fmaxf(mx, row[i]);
         }
         mx = wave_max_f32(mx);
         if (lane == 0) {
             S.alive[q][kid_own[a]] = any ? 1 : 0;
             S.rmax[q][kid_own[a]] = mx;
+            if constexpr (H16) S.rsc[q][kid_own[a]] = enc.sc;
         }
     }
 }
@@ -721,6 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
 // Down sweep, chance edges: child reach = parent reach with the hands that
 // hold the dealt card zeroed (cfr.rs:615-629), for the players in `mask`.
 // ---------------------------------------------------------------------------
+template <bool H16 = false>
 __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, const EdgeRec* __restrict__ recs,
                                                         uint32_t count, int mask) {
     const uint32_t w = wave_item();
@@ -731,9 +810,13 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
     for (int q = 0; q < 2; ++q) {
         if (!((mask >> q) & 1)) continue;
         const uint32_t nh = T.nh[q];
-        float* out = S.reach[q] + (size_t)pick(r.dst, q) * nh;
         const bool alive = S.alive[q][pick(r.src, q)];
-        const float* in = S.reach[q] + (size_t)pick(r.src, q) * nh;
+        // f16 rows: the child keeps the parent's scale (its entries are a
+        // subset), so entries are copied as stored
+        const RowIn<H16> in = reach_in<H16>(T, S, q, pick(r.src, q));
+        RowEnc enc(0.0f);
+        if constexpr (H16) enc.sc = in.sc, enc.inv = 1.0f / in.sc;
+        const RowOut<H16> out(S.reach[q], (size_t)pick(r.dst, q) * nh, enc);
         bool any = false;
         float mmx = 0.0f;
         for (uint32_t i0 = lane; i0 < nh; i0 += kBatch * kWave) {
@@ -750,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
                 const uint32_t i = i0 + u * kWave;
                 if (i >= nh) break;
                 const float x = (!alive || (m[u] & cm)) ? 0.0f : v[u];
-                out[i] = x;
+                out.set(i, x);  // f16: x * inv is the stored entry, exactly
                 any |= x > 0.0f;
                 mmx = fmaxf(mmx, x);
             }
@@ -760,6 +843,7 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
         if (lane == 0) {
             S.alive[q][pick(r.dst, q)] = any ? 1 : 0;
             S.rmax[q][pick(r.dst, q)] = mmx;
+            if constexpr (H16) S.rsc[q][pick(r.dst, q)] = enc.sc;
         }
     }
 }
@@ -786,6 +870,7 @@ constexpr size_t kFoldWaveBytes = 52 * 52 * 4 + 52 * 8;  // card matrix + per-ca
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
 __device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return hi * (hi - 1) / 2 + lo; }
 
+template <bool H16 = false>
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
                                                     uint32_t count, int p, int swap) {
     extern __shared__ double lds[];
@@ -819,17 +904,18 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         float4* m4 = (float4*)M;
         for (uint32_t k = lane; k < 52 * 52 / 4; k += kWave) m4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    float rh[kMaxH];
+    typename RowIn<H16>::Raw rh[kMaxH];
+    float rsc;  // rh's decode scale
     {
-        const float* __restrict__ ro = S.reach[o] + (size_t)pick(recs[first].own, o) * nho;
+        const RowIn<H16> ro = reach_in<H16>(T, S, o, pick(recs[first].own, o));
+        rsc = ro.sc;
 #pragma unroll
         for (int u = 0; u < kMaxH; ++u)
-            if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
+            if (u * kWave < (int)nho) rh[u] = ro.raw(min(lane + u * kWave, nho - 1));
     }
 #pragma unroll 1
     for (uint32_t n = first; n < last; ++n) {
         const FoldRec r = recs[n];
-        float* out = S.cfv + (size_t)r.cfv * T.nh_max;
         const bool live = S.alive[o][pick(r.own, o)] != 0;
         double part = 0.0;
         if (live) {
@@ -837,21 +923,25 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 #pragma unroll
             for (int u = 0; u < kMaxH; ++u)
                 if (lane + u * kWave < nho) {
-                    M[mo_pos[u] & 0xFFFFu] = rh[u];
-                    M[mo_pos[u] >> 16] = rh[u];
-                    part += (double)rh[u];
+                    const float x = RowIn<H16>::dec(rh[u], rsc);
+                    M[mo_pos[u] & 0xFFFFu] = x;
+                    M[mo_pos[u] >> 16] = x;
+                    part += (double)x;
                 }
         }
         // the next node's row, in flight during the rest of this node
         if (n + 1 < last) {
-            const float* __restrict__ ro = S.reach[o] + (size_t)pick(recs[n + 1].own, o) * nho;
+            const RowIn<H16> ro = reach_in<H16>(T, S, o, pick(recs[n + 1].own, o));
+            rsc = ro.sc;
 #pragma unroll
             for (int u = 0; u < kMaxH; ++u)
-                if (u * kWave < (int)nho) rh[u] = ro[min(lane + u * kWave, nho - 1)];
+                if (u * kWave < (int)nho) rh[u] = ro.raw(min(lane + u * kWave, nho - 1));
         }
         if (!live) {
-            for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
+            const RowOut<H16> out(S.cfv, (size_t)r.cfv * T.nh_max, RowEnc(0.0f));
+            for (uint32_t i = lane; i < nhp; i += kWave) out.set(i, 0.0f);
             if (S.vmax && lane == 0) S.vmax[r.cfv] = 0.0f;
+            if (H16 && lane == 0) S.vsc[r.cfv] = 1.0f;
             continue;
         }
         const double tot = wave_sum_f64(part);
@@ -880,6 +970,10 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         // swap: the opponent's payoff (win and lose exchanged), for game-value
         // gradients with respect to the opponent's range and strategy
         const float amount = (r.folder == (uint32_t)p) != (swap != 0) ? r.t_lose : r.t_win;
+        // f16 rows: |value| <= |amount| x the opponent's total reach
+        const RowEnc enc(H16 ? fabsf(amount) * (float)tot : 0.0f);
+        const RowOut<H16> out(S.cfv, (size_t)r.cfv * T.nh_max, enc);
+        if (H16 && lane == 0) S.vsc[r.cfv] = enc.sc;
         float vm = 0.0f;
         // a rolled loop (the unrolled form, with its early exit removed,
         // holds every hand's loads at once and runs slower)
@@ -891,7 +985,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
                 const uint32_t a1 = h & 0xFFu, a2 = (h >> 8) & 0xFFu;
                 const float same_r = M[a1 * 52 + a2];
                 const float valid = (float)(tot - s_card[a1] - s_card[a2]) + same_r;
-                out[i] = amount * valid;
+                out.set(i, amount * valid);
                 vm = fmaxf(vm, fabsf(amount * valid));
             }
         }
@@ -929,7 +1023,7 @@ constexpr int kShortSeg = 9;      // card-list entries per lane when no list exc
 // kR: hand rows per thread, kR * kBlock >= either player's hand count (also
 // the strength-order chunk per thread in phase 2); kSeg: entries per lane
 // of a card's list (four lanes per card), 4 kSeg >= the longest list
-template <int kR, int kSeg>
+template <int kR, int kSeg, bool H16 = false>
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
                                                     int swap) {
@@ -995,12 +1089,14 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         if (tid < 53) coff[tid] = T.sd_coff[o][(size_t)b * 53 + tid];
     }
     __syncthreads();
-    // reach row of the first node
-    float rh[kR];
+    // reach row of the first node (as stored, and its decode scale)
+    typename RowIn<H16>::Raw rh[kR];
+    float rsc;
     {
-        const float* __restrict__ ro = S.reach[o] + (size_t)nodes[0].oslot * nho;
+        const RowIn<H16> ro = reach_in<H16>(T, S, o, nodes[0].oslot);
+        rsc = ro.sc;
 #pragma unroll
-        for (int u = 0; u < kR; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
+        for (int u = 0; u < kR; ++u) rh[u] = ro.raw(min(tid + u * kBlock, nho - 1));
     }
     // per-card segment of this thread (cards 0..51, four lanes each)
     const uint32_t card = tid >> 2, seg = tid & 3;
@@ -1029,34 +1125,39 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     bool pending = false;
     uint32_t pend_cfv = 0;
     auto flush = [&]() {
-        float* __restrict__ po = S.cfv + (size_t)pend_cfv * T.nh_max;
-        for (uint32_t i = tid; i < nhp; i += kBlock) po[i] = O[i];
-        if (S.vmax && tid == 0) S.vmax[pend_cfv] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        const float m = S.vmax ? fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])) : 0.0f;
+        const RowEnc enc(H16 ? m : 0.0f);  // f16 rows: the scale of the row's maximum
+        const RowOut<H16> po(S.cfv, (size_t)pend_cfv * T.nh_max, enc);
+        for (uint32_t i = tid; i < nhp; i += kBlock) po.set(i, O[i]);
+        if (S.vmax && tid == 0) S.vmax[pend_cfv] = m;
+        if (H16 && tid == 0) S.vsc[pend_cfv] = enc.sc;
         pending = false;
     };
 #pragma unroll 1
     for (uint32_t n = 0; n < cnt; ++n) {
         const ShowNode nd = nodes[n];
-        float* __restrict__ out = S.cfv + (size_t)nd.cfv * T.nh_max;
         const bool live = nd.live != 0;
         if (live) {
 #pragma unroll
             for (int u = 0; u < kR; ++u)
-                if (tid + u * kBlock < nho) rhs[tid + u * kBlock] = rh[u];
+                if (tid + u * kBlock < nho) rhs[tid + u * kBlock] = RowIn<H16>::dec(rh[u], rsc);
         }
         // 1. the next node's reach row, in flight during this node
         if (n + 1 < cnt) {
-            const float* __restrict__ ro = S.reach[o] + (size_t)nodes[n + 1].oslot * nho;
+            const RowIn<H16> ro = reach_in<H16>(T, S, o, nodes[n + 1].oslot);
+            rsc = ro.sc;
 #pragma unroll
-            for (int u = 0; u < kR; ++u) rh[u] = ro[min(tid + u * kBlock, nho - 1)];
+            for (int u = 0; u < kR; ++u) rh[u] = ro.raw(min(tid + u * kBlock, nho - 1));
         }
         if (!live) {  // no reach, or no opponent hand on this board: all 0
             if (pending) {
                 __syncthreads();
                 flush();
             }
-            for (uint32_t i = tid; i < nhp; i += kBlock) out[i] = 0.0f;
+            const RowOut<H16> out(S.cfv, (size_t)nd.cfv * T.nh_max, RowEnc(0.0f));
+            for (uint32_t i = tid; i < nhp; i += kBlock) out.set(i, 0.0f);
             if (S.vmax && tid == 0) S.vmax[nd.cfv] = 0.0f;
+            if (H16 && tid == 0) S.vsc[nd.cfv] = 1.0f;
             continue;
         }
         __syncthreads();
@@ -1936,16 +2037,22 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
         for (uint32_t i = tid; i < (32 * L.os + 3) / 4; i += kMmaBlock) o4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __syncthreads();
     } else {
-        // board-conflicting hands of this thread (table entries loaded now,
-        // zeroed once the node table is visible)
+        // hands that conflict with the board (not in p's sorted list) get 0
+        // in every node row; the products write the others
+        uint32_t* onb = (uint32_t*)(base + L.onb);
+        if (tid < 42) onb[tid] = 0u;
+        __syncthreads();
+        for (uint32_t i = tid; i < mp; i += kMmaBlock) {
+            const uint32_t h = T.sd_idx[p][sp + i];
+            atomicOr(&onb[h >> 5], 1u << (h & 31u));
+        }
+        __syncthreads();
         uint32_t bad = 0;
-        const uint4* __restrict__ tab = T.sd_tab[p] + (size_t)b * nhp;
 #pragma unroll
         for (int u = 0; u < kMmaRowPer; ++u) {
             const uint32_t h = tid + u * kMmaBlock;
-            if (h < nhp && tab[h].x == 0xFFFFFFFFu) bad |= 1u << u;
+            if (h < nhp && !((onb[h >> 5] >> (h & 31u)) & 1u)) bad |= 1u << u;
         }
-        __syncthreads();
         if (bad)
             for (uint32_t n = 0; n < cnt; ++n) {
                 float* row = S.cfv + (size_t)nodes[n].cfv * T.nh_max;
@@ -2156,6 +2263,8 @@ __global__ __launch_bounds__(kMmaBlock) void k_up_show_mma(DevTree T, DevState S
 // batches of 8 children ahead of the in-order adds.
 // ---------------------------------------------------------------------------
 constexpr int kChanceBlock = 1024;
+static_assert(2 * kChanceBlock >= 1326, "k_up_chance holds at most two hands per thread");
+template <bool H16 = false>
 __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState S, const ChanceRec* __restrict__ recs,
                                                             uint32_t count, int p) {
     const uint32_t w = xcd_slice(blockIdx.x, gridDim.x);
@@ -2163,10 +2272,11 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
     const int o = 1 - p;
     const ChanceRec r = recs[w];
     const uint32_t nhp = T.nh[p];
-    float* out = S.cfv + (size_t)r.cfv * T.nh_max;
     if (!S.alive[o][pick(r.own, o)]) {
-        for (uint32_t i = threadIdx.x; i < nhp; i += kChanceBlock) out[i] = 0.0f;
+        const RowOut<H16> out(S.cfv, (size_t)r.cfv * T.nh_max, RowEnc(0.0f));
+        for (uint32_t i = threadIdx.x; i < nhp; i += kChanceBlock) out.set(i, 0.0f);
         if (S.vmax && threadIdx.x == 0) S.vmax[r.cfv] = 0.0f;
+        if (H16 && threadIdx.x == 0) S.vsc[r.cfv] = 1.0f;
         return;
     }
     float vm = 0.0f;
@@ -2174,7 +2284,12 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
     const uint32_t* __restrict__ rows = T.chr_cfv + r.off;
     const uint8_t* __restrict__ perms = T.chr_perm + r.off;
     const uint16_t* __restrict__ hperm = T.hand_perm[p];
-    const float* __restrict__ cfv = S.cfv;
+    // entry ix of child t's row
+    auto child = [&](uint32_t t, uint32_t ix) -> float {
+        const size_t e = (size_t)rows[t] * T.nh_max + ix;
+        if constexpr (H16) return h2f(((const uint16_t*)S.cfv)[e]) * S.vsc[rows[t]];
+        else return S.cfv[e];
+    };
     // under suit isomorphism entry t reads its representative's row at the
     // image of hand i under the entry's symmetry (identity: i itself;
     // cfr.rs:641-656); without it every entry reads its own child's row at i
@@ -2182,6 +2297,10 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
         if (!hperm) return i;
         return perms[t] ? hperm[(size_t)perms[t] * nhp + i] : i;
     };
+    // f16 rows: the (at most two) values of this thread wait for the row's
+    // maximum, which sets the row's scale; f32 rows are written at once
+    float x0 = 0.0f, x1 = 0.0f;
+    const RowOut<false> out32(S.cfv, (size_t)r.cfv * T.nh_max, RowEnc(0.0f));
     for (uint32_t i = threadIdx.x; i < nhp; i += kChanceBlock) {
         const uint64_t hm = T.hmask[p][i];
         float acc = 0.0f;
@@ -2190,7 +2309,7 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
             for (; t + 8 <= r.cnt; t += 8) {
                 float v[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = cfv[(size_t)rows[t + u] * T.nh_max + i];
+                for (int u = 0; u < 8; ++u) v[u] = child(t + u, i);
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (!((hm >> cards[t + u]) & 1ull)) acc += v[u];
@@ -2202,16 +2321,22 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
                 for (int u = 0; u < 8; ++u) ix[u] = at(t + u, i);
                 float v[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = cfv[(size_t)rows[t + u] * T.nh_max + ix[u]];
+                for (int u = 0; u < 8; ++u) v[u] = child(t + u, ix[u]);
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (!((hm >> cards[t + u]) & 1ull)) acc += v[u];
             }
         }
         for (; t < r.cnt; ++t)
-            if (!((hm >> cards[t]) & 1ull)) acc += cfv[(size_t)rows[t] * T.nh_max + at(t, i)];
-        out[i] = acc * r.inv;
-        vm = fmaxf(vm, fabsf(acc * r.inv));
+            if (!((hm >> cards[t]) & 1ull)) acc += child(t, at(t, i));
+        const float x = acc * r.inv;
+        if constexpr (H16) {
+            if (i < kChanceBlock) x0 = x;
+            else x1 = x;
+        } else {
+            out32.set(i, x);
+        }
+        vm = fmaxf(vm, fabsf(x));
     }
     if (S.vmax) {
         constexpr int kW = kChanceBlock / kWave;
@@ -2219,10 +2344,15 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
         vm = wave_max_f32(vm);
         if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = vm;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            float m = wm[0];
-            for (int k = 1; k < kW; ++k) m = fmaxf(m, wm[k]);
-            S.vmax[r.cfv] = m;
+        float m = wm[0];
+        for (int k = 1; k < kW; ++k) m = fmaxf(m, wm[k]);
+        if (threadIdx.x == 0) S.vmax[r.cfv] = m;
+        if constexpr (H16) {
+            const RowEnc enc(m);
+            const RowOut<true> out(S.cfv, (size_t)r.cfv * T.nh_max, enc);
+            if (threadIdx.x < nhp) out.set(threadIdx.x, x0);
+            if (threadIdx.x + kChanceBlock < nhp) out.set(threadIdx.x + kChanceBlock, x1);
+            if (threadIdx.x == 0) S.vsc[r.cfv] = enc.sc;
         }
     }
 }
@@ -2244,23 +2374,39 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
 //     strategy itself (kRawF32), no update.
 // Subtrees whose opponent reach is all zero produce zeros and no update.
 // ---------------------------------------------------------------------------
-template <int MODE, int STORE>
+template <int MODE, int STORE, bool H16 = false>
 // Returns this lane's max |value| over the row entries it wrote.
 __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState& S, const ActRec& r, int p, int q,
                                                 uint32_t lane) {
+    static_assert(!H16 || (MODE == 0 && STORE == kStoreH), "f16 rows serve the DCFR sweeps of f16 arenas");
     float vm = 0.0f;
     const int o = 1 - p;
     const uint32_t nhp = T.nh[p];
     const uint32_t nh_max = T.nh_max;
     const int na = r.na;
-    float* __restrict__ out = S.cfv + (size_t)r.cfv * nh_max;
+    const uint32_t* kid_cfv = T.kids + r.kid;
     if (!S.alive[o][pick(r.own, o)]) {
-        for (uint32_t i = lane; i < nhp; i += kWave) out[i] = 0.0f;
+        const RowOut<H16> z(S.cfv, (size_t)r.cfv * nh_max, RowEnc(0.0f));
+        for (uint32_t i = lane; i < nhp; i += kWave) z.set(i, 0.0f);
+        if (H16 && lane == 0) S.vsc[r.cfv] = 1.0f;
         return 0.0f;
     }
-    const uint32_t* kid_cfv = T.kids + r.kid;
-    const float* __restrict__ cfv = S.cfv;
-    auto row = [&](int a) -> const float* { return cfv + (size_t)kid_cfv[a] * nh_max; };
+    // f16 rows: |value| <= the sum of the children's row maxima (opponent
+    // and locked nodes; a lock's weights are <= the lock bound), or their
+    // max (the traverser's regret-matched sigma sums to 1)
+    float vb = 0.0f;
+    if constexpr (H16) {
+        const bool sum = q != p || r.lock_off >= 0;
+        for (int a = 0; a < na; ++a) {
+            const float m = S.vmax[kid_cfv[a]];
+            vb = sum ? vb + m : fmaxf(vb, m);
+        }
+        if (r.lock_off >= 0) vb *= T.lock_bound;
+    }
+    const RowEnc enc(vb);
+    const RowOut<H16> out(S.cfv, (size_t)r.cfv * nh_max, enc);
+    if (H16 && lane == 0) S.vsc[r.cfv] = enc.sc;
+    auto row = [&](int a) { return cfv_in<H16>(T, S, kid_cfv[a]); };
     if (q != p || r.lock_off >= 0 || MODE == 1) {
         // opponent node: sum; locked node: lock-weighted; best response: max
         const float* sig = r.lock_off >= 0 ? T.lock_sigma + r.lock_off : nullptr;
@@ -2268,7 +2414,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
         dispatch_na4(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
             if constexpr (NA > 0) {
-                const float* rw[NA];
+                RowIn<H16> rw[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) rw[a] = row(a);
                 for (uint32_t i0 = lane; i0 < nhp; i0 += kSumHands * kWave) {
@@ -2295,7 +2441,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
 #pragma unroll
                             for (int a = 1; a < NA; ++a) acc = v[h][a] > acc ? v[h][a] : acc;
                         }
-                        out[i] = acc;
+                        out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                     }
                 }
@@ -2308,7 +2454,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                         else if (kind == 1) acc += sig[(size_t)a * nhp + i] * v;
                         else if (a > 0 && v > acc) acc = v;
                     }
-                    out[i] = acc;
+                    out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                 }
             }
@@ -2333,7 +2479,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     float acc = 0.0f;
 #pragma unroll
                     for (int a = 0; a < NA; ++a) acc += hs.sig[a] * row(a)[i];
-                    out[i] = acc;
+                    out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                 }
             } else {
@@ -2342,7 +2488,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     sh.init(arena, nhp, i, na);
                     float acc = 0.0f;
                     for (int a = 0; a < na; ++a) acc += sh.template sigma<false>(a) * row(a)[i];
-                    out[i] = acc;
+                    out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                 }
             }
@@ -2351,7 +2497,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
     } else {
         const uint32_t t = *S.iter;
         const float pos = S.disc[3 * t + 0], neg = S.disc[3 * t + 1], ds = S.disc[3 * t + 2];
-        const float* __restrict__ rp = S.reach[p] + (size_t)pick(r.own, p) * nhp;
+        const RowIn<H16> rp = reach_in<H16>(T, S, p, pick(r.own, p));
         if constexpr (STORE == kStoreF32 || STORE == kStoreF32P) {
             // PCFR+ (kStoreF32P): sigma from Q = max(R,0) + prediction, and
             // the update also stores Q' = max(R',0) + (v - out), the next
@@ -2363,7 +2509,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
-                    const float* rw[NA];
+                    RowIn<H16> rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
                     for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
@@ -2405,7 +2551,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                                 Sm[(size_t)a * nhp + i] = sm[h][a] * ds + reach[h] * hs[h].sig[a];
                                 if constexpr (PRED) Qp[(size_t)a * nhp + i] = fmaxf(nr, 0.0f) + (v[h][a] - acc);
                             }
-                            out[i] = acc;
+                            out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                         }
                     }
@@ -2426,7 +2572,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             Sm[idx] = Sm[idx] * ds + reach * sg;
                             if constexpr (PRED) Qp[idx] = fmaxf(nr, 0.0f) + (row(a)[i] - acc);
                         }
-                        out[i] = acc;
+                        out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                     }
                 }
@@ -2457,7 +2603,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
-                    const float* rw[NA];
+                    RowIn<H16> rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
                     for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
@@ -2496,7 +2642,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                                 xr = fmaxf(xr, fabsf(nr));
                                 xs = fmaxf(xs, ns);
                             }
-                            out[i] = acc;
+                            out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                         }
                     }
@@ -2519,7 +2665,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             xr = fmaxf(xr, fabsf(nr));
                             xs = fmaxf(xs, ns);
                         }
-                        out[i] = acc;
+                        out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                     }
                 }
@@ -2590,7 +2736,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                         rmax = fmaxf(rmax, fabsf(nr));
                         smax = fmaxf(smax, ns);
                     });
-                    out[i] = acc;
+                    out.set(i, acc);
                     vm = fmaxf(vm, fabsf(acc));
                 }
                 RM = wave_max_f32(rmax);
@@ -2615,14 +2761,14 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
     return vm;
 }
 
-template <int MODE, int STORE>
+template <int MODE, int STORE, bool H16 = false>
 __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
                                                       uint32_t count, int p, int q) {
     const uint32_t w = wave_item();
     if (w >= count) return;
     const uint32_t lane = threadIdx.x & 63;
     const ActRec r = recs[w];
-    float vm = up_action_node<MODE, STORE>(T, S, r, p, q, lane);
+    float vm = up_action_node<MODE, STORE, H16>(T, S, r, p, q, lane);
     if (S.vmax) {  // the row's max |value|, for the parent's f16 scale
         vm = wave_max_f32(vm);
         if (lane == 0) S.vmax[r.cfv] = vm;

@@ -279,6 +279,9 @@ uint64_t gto_spot_arena_bytes(const gto_spot* spot, int storage) {
     const gto::Tree& t = spot->spot->tree;
     uint64_t entries = t.data_size[0] + t.data_size[1];
     if (storage == GTO_STORAGE_COMPRESSED) return entries * 2 * 2 + (uint64_t)t.num_nodes() * 16;
+    // f16 arenas: 2 bytes per entry, per-node scales and maxima
+    if (storage == GTO_STORAGE_HALF || storage == GTO_STORAGE_HALF_ROWS)
+        return entries * 2 * 2 + (uint64_t)t.num_nodes() * 32;
     return entries * 2 * 4;
 }
 

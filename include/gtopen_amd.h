@@ -54,6 +54,13 @@ extern "C" {
  * magnitude. No counterpart in the reference (an engine-side format; saves
  * and downloads are f32 as always). */
 #define GTO_STORAGE_HALF 2
+/* GTO_STORAGE_HALF arenas, and the DCFR iterations' reach and value rows in
+ * f16 too (each row with a power-of-two scale bounding its magnitude): half
+ * the row traffic of the level sweeps. Best response, exploitability, node
+ * values and the game value keep f32 rows; the matrix-core showdown forms
+ * (gto_hip_set_showdown) are not available with it (GTO_ERR_INVALID). No
+ * counterpart in the reference. */
+#define GTO_STORAGE_HALF_ROWS 3
 
 /* Algorithms (crates/solver/src/cfr.rs:13-35). PCFR+ (predictive CFR+:
  * CFR+ discounts, the current strategy matched on max(R,0) plus the last
@@ -138,7 +145,8 @@ void gto_spot_free(gto_spot* spot);
 const gto_spot_view* gto_spot_view_of(const gto_spot* spot);
 /* Number of action nodes (game.rs:375-381). */
 uint64_t gto_spot_num_action_nodes(const gto_spot* spot);
-/* Arena bytes under a storage mode (game.rs:341-358). */
+/* Arena bytes under a storage mode (game.rs:341-358; the f16 modes, which
+ * the reference lacks, count 2 bytes per entry plus per-node scales). */
 uint64_t gto_spot_arena_bytes(const gto_spot* spot, int storage);
 /* Suit-symmetry group size (game.rs:262-317); 1 = only the identity. */
 uint32_t gto_spot_num_suit_perms(const gto_spot* spot);
@@ -202,7 +210,7 @@ int gto_validate_locks(const gto_spot* spot, uint32_t n_locks, const uint32_t* l
 typedef struct gto_hip gto_hip;
 
 typedef struct gto_hip_options {
-    int storage;        /* GTO_STORAGE_F32 or GTO_STORAGE_COMPRESSED (device arenas) */
+    int storage;        /* GTO_STORAGE_F32, _COMPRESSED, _HALF or _HALF_ROWS (device arenas and rows) */
     int algo;           /* GTO_ALGO_DCFR, GTO_ALGO_CFRPLUS or GTO_ALGO_PCFRPLUS (f32 arenas) */
     int use_graphs;     /* capture each sweep into a hipGraph after the first iteration */
     int device;         /* HIP device ordinal */

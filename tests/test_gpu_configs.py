@@ -99,7 +99,8 @@ def test_flop_srp_f32_matches_oracle(built):
 
 @pytest.mark.timeout(1200)
 def test_flop_srp_precision_forms_judged_by_fp64(built):
-    """f32, f16 arenas, f16 + BF16 and f16 + FP8 showdown products, each run
+    """f32, f16 arenas, f16 arenas with f16 rows, f16 + BF16 and f16 + FP8
+    showdown products, each run
     to 0.3% pot (exploitability checked every 10 iterations on the GPU), and
     the reference's i16/u16 arenas at a fixed stop (they floor above the
     target on this spot): the FP64 oracle's best response judges each final
@@ -111,8 +112,8 @@ def test_flop_srp_precision_forms_judged_by_fp64(built):
     spot = Spot(workloads.flop_srp_spot())
     pot = spot.starting_pot
     o64 = oracle.Oracle(spot, precision=64)
-    for storage, showdown, cap in (("f32", "exact", 600), ("half", "exact", 600), ("half", "bf16", 600),
-                                   ("half", "fp8", 700), ("compressed", "exact", 400)):
+    for storage, showdown, cap in (("f32", "exact", 600), ("half", "exact", 600), ("half_rows", "exact", 600),
+                                   ("half", "bf16", 600), ("half", "fp8", 700), ("compressed", "exact", 400)):
         _say(f"case: flop SRP, {storage} arenas, {showdown} showdown, to 0.3% pot (at most {cap} iterations)")
         g = GpuSolver(spot, storage=storage, showdown=showdown)
         e = None

@@ -27,7 +27,7 @@ def _same(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("storage", ["f32", "half"])
+@pytest.mark.parametrize("storage", ["f32", "half", "half_rows"])
 def test_every_schedule_gives_the_same_arenas(built, storage):
     """Level sweeps under the 6 terminal schedules and the fused river sweep,
     20 iterations each, on the river spot (one subtree: the whole tree),
@@ -45,7 +45,7 @@ def test_every_schedule_gives_the_same_arenas(built, storage):
             g = GpuSolver(spot, storage=storage)
             g.set_schedule(show_nodes, per_level, river)
             assert g.schedule() == {"show_nodes": show_nodes, "per_level": per_level, "river": river}
-            if river:
+            if river and storage != "half_rows":  # f16 rows: level sweeps only
                 assert g.river_entries > 0
             g.iterate(20)
             a = _arenas(g)
