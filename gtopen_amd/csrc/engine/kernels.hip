@@ -568,8 +568,12 @@ __device__ __forceinline__ uint32_t wave_item() {
 // load's first use otherwise waits for it before the next one is issued).
 constexpr int kBatch = 4;
 // Hands per lane in one step of the up-sweep action kernels: every load
-// of all of them is issued before the first store.
+// of all of them is issued before the first store. The f16-row update
+// takes three (93 VGPRs, five waves per SIMD: 3.40 ms per flop SRP
+// iteration against 3.59 with four); the f32-row updates are bound by
+// HBM bandwidth, where three hands (7 waves) change nothing.
 constexpr int kStepHands = 4;
+constexpr int kStepHands16 = 3;
 // Opponent-node sums (no arena traffic, fewer registers) take six.
 constexpr int kSumHands = 6;
 // The same in the down-sweep action kernel (two: 2.09 ms per f32 iteration,
@@ -2578,6 +2582,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                 }
             });
         } else if constexpr (STORE == kStoreH) {
+            constexpr int kStepH = H16 ? kStepHands16 : kStepHands;
             // f16 arenas with a power-of-two scale per node (no counterpart in
             // the reference). The new scales come from bounds of the new
             // maxima, known before any value is read, so every value is
@@ -2606,12 +2611,12 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     RowIn<H16> rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                    for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
-                        HandSigma<NA, kCurrentH> hs[kStepHands];
-                        float v[kStepHands][NA], sm[kStepHands][NA], reach[kStepHands];
-                        bool ok[kStepHands];
+                    for (uint32_t i0 = lane; i0 < nhp; i0 += kStepH * kWave) {
+                        HandSigma<NA, kCurrentH> hs[kStepH];
+                        float v[kStepH][NA], sm[kStepH][NA], reach[kStepH];
+                        bool ok[kStepH];
 #pragma unroll
-                        for (int h = 0; h < kStepHands; ++h) {
+                        for (int h = 0; h < kStepH; ++h) {
                             const uint32_t i = i0 + h * kWave;
                             ok[h] = i < nhp;
                             const uint32_t ii = ok[h] ? i : i0;
@@ -2624,7 +2629,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             reach[h] = rp[ii];
                         }
 #pragma unroll
-                        for (int h = 0; h < kStepHands; ++h) {
+                        for (int h = 0; h < kStepH; ++h) {
                             if (!ok[h]) continue;
                             const uint32_t i = i0 + h * kWave;
                             hs[h].finish();
