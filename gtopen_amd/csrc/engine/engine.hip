@@ -133,7 +133,7 @@ class Engine {
             throw std::runtime_error("unknown storage mode");
         compressed_ = o.storage != GTO_STORAGE_F32;
         half_ = o.storage == GTO_STORAGE_HALF || o.storage == GTO_STORAGE_HALF_ROWS;
-        rows16_ = o.storage == GTO_STORAGE_HALF_ROWS;
+        rows_ = o.storage == GTO_STORAGE_HALF_ROWS ? (kRows16Reach | kRows16Value) : 0;
         pred_ = o.algo == GTO_ALGO_PCFRPLUS;
         if (pred_ && compressed_) throw std::runtime_error("PCFR+ runs with f32 arenas on the GPU");
         // fault injection: GTO_INJECT_OOM=1 reports a device allocation
@@ -295,7 +295,7 @@ class Engine {
             // regret matching, values weighted by the traverser's sigma
             launch_root(1 << o);
             launch_down(1 << o, kCurrentF32);
-            reach_valid_[o] = !rows16_;  // f32 rows: the DCFR sweeps of f16 rows recompute them
+            reach_valid_[o] = !(rows_ & kRows16Reach);  // f32 rows: DCFR sweeps with f16 reach rows recompute them
             launch_up(p, 3);
         } else {
             launch_root(1 << o);
@@ -387,7 +387,7 @@ class Engine {
         launch_root(1 << o);
         if (mode == 2) {
             launch_down(1 << o, kCurrentF32);
-            reach_valid_[o] = !rows16_;  // f32 rows: the DCFR sweeps of f16 rows recompute them
+            reach_valid_[o] = !(rows_ & kRows16Reach);  // f32 rows: DCFR sweeps with f16 reach rows recompute them
             launch_up(p, 3);
         } else {
             launch_down(1 << o, kAverageF32);
@@ -722,7 +722,7 @@ class Engine {
     void set_showdown(int mode) {
         if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16 && mode != GTO_SHOWDOWN_FP8)
             throw std::runtime_error("unknown showdown mode");
-        if (mode != GTO_SHOWDOWN_EXACT && rows16_)
+        if (mode != GTO_SHOWDOWN_EXACT && rows_)
             throw std::invalid_argument("the matrix-core showdown forms read f32 rows: not with f16 rows");
         if (mode != GTO_SHOWDOWN_EXACT) {
             const bool fp8 = mode == GTO_SHOWDOWN_FP8;
@@ -1073,9 +1073,10 @@ class Engine {
     // the upper tree's terminals; river-level reach rows are left stale by
     // fused sweeps (level sweeps recompute both players' rows after them)
     bool river_avail_ = false, rows_stale_ = false, locked_ = false;
-    // GTO_STORAGE_HALF_ROWS: the DCFR sweeps keep reach and value rows in f16
-    // (per-row power-of-two scales); other sweeps use f32 rows
-    bool rows16_ = false;
+    // rows the DCFR sweeps keep in f16 (per-row power-of-two scales):
+    // kRows16Reach | kRows16Value under GTO_STORAGE_HALF_ROWS; other sweeps
+    // use f32 rows
+    int rows_ = 0;
     RiverSub* river_subs_ = nullptr;
     RiverOp* river_ops_ = nullptr;
     uint32_t n_river_ = 0, river_mo_ = 0;
@@ -1177,14 +1178,11 @@ class Engine {
         size_t lds_max = std::max(std::max(show_lds_[0], show_lds_[1]), std::max(fold_lds_[0], fold_lds_[1]));
         if (lds_max > kLdsBudget)
             throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
-        for (const void* f : {(const void*)k_up_show<3, kShortSeg>, (const void*)k_up_show<3, kMaxCardList / 4>,
-                              (const void*)k_up_show<kRowPer, kMaxCardList / 4>,
-                              (const void*)k_up_show<3, kShortSeg, true>, (const void*)k_up_show<3, kMaxCardList / 4, true>,
-                              (const void*)k_up_show<kRowPer, kMaxCardList / 4, true>})
-            HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+        for (const int rows : {0, kRows16Reach, kRows16Reach | kRows16Value})
+            HIP_OK(hipFuncSetAttribute((const void*)show_kernel(rows), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)std::max(show_lds_[0], show_lds_[1])));
-        for (const void* f : {(const void*)k_up_fold<false>, (const void*)k_up_fold<true>})
-            HIP_OK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+        for (const int rows : {0, kRows16Reach, kRows16Reach | kRows16Value})
+            HIP_OK(hipFuncSetAttribute((const void*)fold_kernel(rows), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)std::max(fold_lds_[0], fold_lds_[1])));
         n_fold_all_ = (uint32_t)plan_.fold_all.size();
         if (n_fold_all_) fold_all_ = upload_vec(plan_.fold_all);
@@ -1234,7 +1232,7 @@ class Engine {
     // within the workgroup's 160 KB.
     void upload_river() {
         river_avail_ = false;
-        if (!plan_.river_ok || plan_.river_subs.empty() || (compressed_ && !half_) || pred_ || rows16_) return;
+        if (!plan_.river_ok || plan_.river_subs.empty() || (compressed_ && !half_) || pred_ || rows_) return;
         river_mo_ = std::max(plan_.max_board_cnt[0], plan_.max_board_cnt[1]);
         river_lds_ = river_lds_layout(plan_.river_rows, plan_.nh_max, river_mo_).bytes;
         if (river_lds_ > kLdsBudget) return;
@@ -1295,7 +1293,7 @@ class Engine {
         S_.cfv = (float*)dalloc(sizeof(float) * (size_t)plan_.num_reached * plan_.nh_max);
         S_.vmax = nullptr;
         S_.rsc[0] = S_.rsc[1] = S_.vsc = nullptr;
-        if (rows16_) {
+        if (rows_) {
             for (int q = 0; q < 2; ++q) {
                 S_.rsc[q] = (float*)dalloc(sizeof(float) * std::max<uint32_t>(plan_.num_slots[q], 1));
                 HIP_OK(hipMemset(S_.rsc[q], 0, sizeof(float) * std::max<uint32_t>(plan_.num_slots[q], 1)));
@@ -1386,7 +1384,7 @@ class Engine {
     }
 
     // upper: only the nodes outside river subtrees (fused river sweeps)
-    // h16: f16 rows (the DCFR sweeps of GTO_STORAGE_HALF_ROWS)
+    // h16: f16 reach rows (the DCFR sweeps of GTO_STORAGE_HALF_ROWS)
     void launch_down(int mask, int kind, bool upper = false, bool h16 = false) {
         for (size_t l = 0; l < lv_.size(); ++l) {
             const DevLevel& D = lv_[l];
@@ -1450,7 +1448,7 @@ class Engine {
         std::exception_ptr err;
         try {
             if (fold)
-                launch(PK_UP_FOLD, k_up_fold<false>, wave_grid(fold_waves(n_fold_up_)), fold_lds_[p], T_, S_,
+                launch(PK_UP_FOLD, k_up_fold<0>, wave_grid(fold_waves(n_fold_up_)), fold_lds_[p], T_, S_,
                        (const FoldRec*)fold_up_, n_fold_up_, p, 0);
         } catch (...) {
             err = std::current_exception();
@@ -1465,7 +1463,7 @@ class Engine {
                     launch(PK_RIVER, river_kernel<kStoreF32>(), g, river_lds_, T_, S_, (const RiverSub*)river_subs_,
                            n_river_, (const RiverOp*)river_ops_, p, plan_.river_rows, river_mo_);
                 if (n_show_up_chunks_[0])
-                    launch(PK_UP_SHOW, show_kernel(false), wave_grid(n_show_up_chunks_[0], 1), show_lds_[p], T_, S_,
+                    launch(PK_UP_SHOW, show_kernel(0), wave_grid(n_show_up_chunks_[0], 1), show_lds_[p], T_, S_,
                            (const ShowRec*)show_up_, (const uint4*)show_up_chunks_[0], n_show_up_chunks_[0], p, 0);
             } catch (...) {
                 err = std::current_exception();
@@ -1491,21 +1489,28 @@ class Engine {
         }
     }
 
-    // h16: f16 rows
-    ShowFn show_kernel(bool h16) const {
-        if (h16) {
-            if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4, true>;
-            return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg, true>
-                                                    : k_up_show<3, kMaxCardList / 4, true>;
-        }
-        if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4>;
-        return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg> : k_up_show<3, kMaxCardList / 4>;
+    // rows: the f16 rows of the sweep (0, kRows16Reach, or both kinds)
+    template <int ROWS>
+    ShowFn show_kernel_of() const {
+        if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4, ROWS>;
+        return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg, ROWS> : k_up_show<3, kMaxCardList / 4, ROWS>;
+    }
+    ShowFn show_kernel(int rows) const {
+        if (rows == kRows16Reach) return show_kernel_of<kRows16Reach>();
+        if (rows) return show_kernel_of<kRows16Reach | kRows16Value>();
+        return show_kernel_of<0>();
+    }
+    using FoldFn = void (*)(DevTree, DevState, const FoldRec*, uint32_t, int, int);
+    static FoldFn fold_kernel(int rows) {
+        if (rows == kRows16Reach) return k_up_fold<kRows16Reach>;
+        if (rows) return k_up_fold<kRows16Reach | kRows16Value>;
+        return k_up_fold<0>;
     }
 
     void launch_up(int p, int mode, int swap = 0) {
         // the DCFR sweeps of f16-row engines read and write f16 rows
-        const bool h16 = rows16_ && mode == 0;
-        const auto fold_fn = h16 ? k_up_fold<true> : k_up_fold<false>;
+        const int rows = mode == 0 ? rows_ : 0;
+        const FoldFn fold_fn = fold_kernel(rows);
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
         if (mma)
             launch_b(PK_UP_SHOW, show_mode_ == GTO_SHOWDOWN_FP8 ? k_up_show_mma<true> : k_up_show_mma<false>,
@@ -1537,7 +1542,7 @@ class Engine {
             lstream_ = stream_;
             if (!err) try {
                     if (show)
-                        launch(PK_UP_SHOW, show_kernel(h16), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
+                        launch(PK_UP_SHOW, show_kernel(rows), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
                                (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_],
                                p, swap);
                 } catch (...) {
@@ -1558,11 +1563,11 @@ class Engine {
                     launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(D.n_fold)), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
                            D.n_fold, p, swap);
                 if (!mma && D.n_show_chunks[cap_])
-                    launch(PK_UP_SHOW, show_kernel(h16), wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
+                    launch(PK_UP_SHOW, show_kernel(rows), wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
                            (const ShowRec*)D.show, (const uint4*)D.show_chunks[cap_], D.n_show_chunks[cap_], p, swap);
             }
             if (D.n_chance)
-                launch_b(PK_UP_CHANCE, h16 ? k_up_chance<true> : k_up_chance<false>, wave_grid(D.n_chance, 1),
+                launch_b(PK_UP_CHANCE, (rows & kRows16Value) ? k_up_chance<true> : k_up_chance<false>, wave_grid(D.n_chance, 1),
                          kChanceBlock, 0, T_, S_, (const ChanceRec*)D.chance, D.n_chance, p);
             for (int q = 0; q < 2; ++q) {
                 if (!D.n_act[q]) continue;
@@ -1571,7 +1576,11 @@ class Engine {
                 if (half_) {
                     switch (mode) {
                         case 0:
-                            if (h16) launch(PK_UP_ACTION, k_up_action<0, kStoreH, true>, g, 0, T_, S_, recs, n, p, q);
+                            if (rows == kRows16Reach)
+                                launch(PK_UP_ACTION, k_up_action<0, kStoreH, kRows16Reach>, g, 0, T_, S_, recs, n, p, q);
+                            else if (rows)
+                                launch(PK_UP_ACTION, k_up_action<0, kStoreH, kRows16Reach | kRows16Value>, g, 0, T_, S_,
+                                       recs, n, p, q);
                             else launch(PK_UP_ACTION, k_up_action<0, kStoreH>, g, 0, T_, S_, recs, n, p, q);
                             break;
                         case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
@@ -1618,8 +1627,8 @@ class Engine {
             for (int q = 0; q < 2; ++q)
                 if (!reach_valid_[q]) mask |= 1 << q;
             if (mask) {
-                launch_root(mask, rows16_);
-                launch_down(mask, kCurrentF32, fu, rows16_);
+                launch_root(mask, (rows_ & kRows16Reach) != 0);
+                launch_down(mask, kCurrentF32, fu, (rows_ & kRows16Reach) != 0);
             }
             reach_valid_[0] = reach_valid_[1] = true;
             if (fu) launch_up_fused(p);

@@ -85,6 +85,9 @@ __device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uin
 // raw(i) / dec(raw, scale): an entry as stored and its value, for loads
 // issued ahead of their use (a prefetched row stays raw in registers until
 // it is used, so converting it does not wait for the load early).
+// f16-row selections of the kernels that read or write both kinds of row
+constexpr int kRows16Reach = 1, kRows16Value = 2;
+
 template <bool H16>
 struct RowIn {
     using Raw = float;
@@ -874,9 +877,10 @@ constexpr size_t kFoldWaveBytes = 52 * 52 * 4 + 52 * 8;  // card matrix + per-ca
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
 __device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return hi * (hi - 1) / 2 + lo; }
 
-template <bool H16 = false>
+template <int ROWS = 0>
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
                                                     uint32_t count, int p, int swap) {
+    constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
     extern __shared__ double lds[];
     const int o = 1 - p;
     const uint32_t nho = T.nh[o], nhp = T.nh[p];
@@ -908,10 +912,10 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         float4* m4 = (float4*)M;
         for (uint32_t k = lane; k < 52 * 52 / 4; k += kWave) m4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    typename RowIn<H16>::Raw rh[kMaxH];
+    typename RowIn<R16>::Raw rh[kMaxH];
     float rsc;  // rh's decode scale
     {
-        const RowIn<H16> ro = reach_in<H16>(T, S, o, pick(recs[first].own, o));
+        const RowIn<R16> ro = reach_in<R16>(T, S, o, pick(recs[first].own, o));
         rsc = ro.sc;
 #pragma unroll
         for (int u = 0; u < kMaxH; ++u)
@@ -927,7 +931,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 #pragma unroll
             for (int u = 0; u < kMaxH; ++u)
                 if (lane + u * kWave < nho) {
-                    const float x = RowIn<H16>::dec(rh[u], rsc);
+                    const float x = RowIn<R16>::dec(rh[u], rsc);
                     M[mo_pos[u] & 0xFFFFu] = x;
                     M[mo_pos[u] >> 16] = x;
                     part += (double)x;
@@ -935,17 +939,17 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         }
         // the next node's row, in flight during the rest of this node
         if (n + 1 < last) {
-            const RowIn<H16> ro = reach_in<H16>(T, S, o, pick(recs[n + 1].own, o));
+            const RowIn<R16> ro = reach_in<R16>(T, S, o, pick(recs[n + 1].own, o));
             rsc = ro.sc;
 #pragma unroll
             for (int u = 0; u < kMaxH; ++u)
                 if (u * kWave < (int)nho) rh[u] = ro.raw(min(lane + u * kWave, nho - 1));
         }
         if (!live) {
-            const RowOut<H16> out(S.cfv, (size_t)r.cfv * T.nh_max, RowEnc(0.0f));
+            const RowOut<V16> out(S.cfv, (size_t)r.cfv * T.nh_max, RowEnc(0.0f));
             for (uint32_t i = lane; i < nhp; i += kWave) out.set(i, 0.0f);
             if (S.vmax && lane == 0) S.vmax[r.cfv] = 0.0f;
-            if (H16 && lane == 0) S.vsc[r.cfv] = 1.0f;
+            if (V16 && lane == 0) S.vsc[r.cfv] = 1.0f;
             continue;
         }
         const double tot = wave_sum_f64(part);
@@ -975,9 +979,9 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         // gradients with respect to the opponent's range and strategy
         const float amount = (r.folder == (uint32_t)p) != (swap != 0) ? r.t_lose : r.t_win;
         // f16 rows: |value| <= |amount| x the opponent's total reach
-        const RowEnc enc(H16 ? fabsf(amount) * (float)tot : 0.0f);
-        const RowOut<H16> out(S.cfv, (size_t)r.cfv * T.nh_max, enc);
-        if (H16 && lane == 0) S.vsc[r.cfv] = enc.sc;
+        const RowEnc enc(V16 ? fabsf(amount) * (float)tot : 0.0f);
+        const RowOut<V16> out(S.cfv, (size_t)r.cfv * T.nh_max, enc);
+        if (V16 && lane == 0) S.vsc[r.cfv] = enc.sc;
         float vm = 0.0f;
         // a rolled loop (the unrolled form, with its early exit removed,
         // holds every hand's loads at once and runs slower)
@@ -1027,10 +1031,11 @@ constexpr int kShortSeg = 9;      // card-list entries per lane when no list exc
 // kR: hand rows per thread, kR * kBlock >= either player's hand count (also
 // the strength-order chunk per thread in phase 2); kSeg: entries per lane
 // of a card's list (four lanes per card), 4 kSeg >= the longest list
-template <int kR, int kSeg, bool H16 = false>
+template <int kR, int kSeg, int ROWS = 0>
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
                                                     int swap) {
+    constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
     extern __shared__ double lds[];
     const uint32_t c = xcd_slice(blockIdx.x, gridDim.x);
     if (c >= n_chunks) return;
@@ -1094,10 +1099,10 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     }
     __syncthreads();
     // reach row of the first node (as stored, and its decode scale)
-    typename RowIn<H16>::Raw rh[kR];
+    typename RowIn<R16>::Raw rh[kR];
     float rsc;
     {
-        const RowIn<H16> ro = reach_in<H16>(T, S, o, nodes[0].oslot);
+        const RowIn<R16> ro = reach_in<R16>(T, S, o, nodes[0].oslot);
         rsc = ro.sc;
 #pragma unroll
         for (int u = 0; u < kR; ++u) rh[u] = ro.raw(min(tid + u * kBlock, nho - 1));
@@ -1130,11 +1135,11 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     uint32_t pend_cfv = 0;
     auto flush = [&]() {
         const float m = S.vmax ? fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])) : 0.0f;
-        const RowEnc enc(H16 ? m : 0.0f);  // f16 rows: the scale of the row's maximum
-        const RowOut<H16> po(S.cfv, (size_t)pend_cfv * T.nh_max, enc);
+        const RowEnc enc(V16 ? m : 0.0f);  // f16 value rows: the scale of the row's maximum
+        const RowOut<V16> po(S.cfv, (size_t)pend_cfv * T.nh_max, enc);
         for (uint32_t i = tid; i < nhp; i += kBlock) po.set(i, O[i]);
         if (S.vmax && tid == 0) S.vmax[pend_cfv] = m;
-        if (H16 && tid == 0) S.vsc[pend_cfv] = enc.sc;
+        if (V16 && tid == 0) S.vsc[pend_cfv] = enc.sc;
         pending = false;
     };
 #pragma unroll 1
@@ -1144,11 +1149,11 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         if (live) {
 #pragma unroll
             for (int u = 0; u < kR; ++u)
-                if (tid + u * kBlock < nho) rhs[tid + u * kBlock] = RowIn<H16>::dec(rh[u], rsc);
+                if (tid + u * kBlock < nho) rhs[tid + u * kBlock] = RowIn<R16>::dec(rh[u], rsc);
         }
         // 1. the next node's reach row, in flight during this node
         if (n + 1 < cnt) {
-            const RowIn<H16> ro = reach_in<H16>(T, S, o, nodes[n + 1].oslot);
+            const RowIn<R16> ro = reach_in<R16>(T, S, o, nodes[n + 1].oslot);
             rsc = ro.sc;
 #pragma unroll
             for (int u = 0; u < kR; ++u) rh[u] = ro.raw(min(tid + u * kBlock, nho - 1));
@@ -1158,10 +1163,10 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
                 __syncthreads();
                 flush();
             }
-            const RowOut<H16> out(S.cfv, (size_t)nd.cfv * T.nh_max, RowEnc(0.0f));
+            const RowOut<V16> out(S.cfv, (size_t)nd.cfv * T.nh_max, RowEnc(0.0f));
             for (uint32_t i = tid; i < nhp; i += kBlock) out.set(i, 0.0f);
             if (S.vmax && tid == 0) S.vmax[nd.cfv] = 0.0f;
-            if (H16 && tid == 0) S.vsc[nd.cfv] = 1.0f;
+            if (V16 && tid == 0) S.vsc[nd.cfv] = 1.0f;
             continue;
         }
         __syncthreads();
@@ -2378,11 +2383,12 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
 //     strategy itself (kRawF32), no update.
 // Subtrees whose opponent reach is all zero produce zeros and no update.
 // ---------------------------------------------------------------------------
-template <int MODE, int STORE, bool H16 = false>
+template <int MODE, int STORE, int ROWS = 0>
 // Returns this lane's max |value| over the row entries it wrote.
 __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState& S, const ActRec& r, int p, int q,
                                                 uint32_t lane) {
-    static_assert(!H16 || (MODE == 0 && STORE == kStoreH), "f16 rows serve the DCFR sweeps of f16 arenas");
+    constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
+    static_assert(ROWS == 0 || (MODE == 0 && STORE == kStoreH), "f16 rows serve the DCFR sweeps of f16 arenas");
     float vm = 0.0f;
     const int o = 1 - p;
     const uint32_t nhp = T.nh[p];
@@ -2390,16 +2396,16 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
     const int na = r.na;
     const uint32_t* kid_cfv = T.kids + r.kid;
     if (!S.alive[o][pick(r.own, o)]) {
-        const RowOut<H16> z(S.cfv, (size_t)r.cfv * nh_max, RowEnc(0.0f));
+        const RowOut<V16> z(S.cfv, (size_t)r.cfv * nh_max, RowEnc(0.0f));
         for (uint32_t i = lane; i < nhp; i += kWave) z.set(i, 0.0f);
-        if (H16 && lane == 0) S.vsc[r.cfv] = 1.0f;
+        if (V16 && lane == 0) S.vsc[r.cfv] = 1.0f;
         return 0.0f;
     }
     // f16 rows: |value| <= the sum of the children's row maxima (opponent
     // and locked nodes; a lock's weights are <= the lock bound), or their
     // max (the traverser's regret-matched sigma sums to 1)
     float vb = 0.0f;
-    if constexpr (H16) {
+    if constexpr (V16) {
         const bool sum = q != p || r.lock_off >= 0;
         for (int a = 0; a < na; ++a) {
             const float m = S.vmax[kid_cfv[a]];
@@ -2408,9 +2414,9 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
         if (r.lock_off >= 0) vb *= T.lock_bound;
     }
     const RowEnc enc(vb);
-    const RowOut<H16> out(S.cfv, (size_t)r.cfv * nh_max, enc);
-    if (H16 && lane == 0) S.vsc[r.cfv] = enc.sc;
-    auto row = [&](int a) { return cfv_in<H16>(T, S, kid_cfv[a]); };
+    const RowOut<V16> out(S.cfv, (size_t)r.cfv * nh_max, enc);
+    if (V16 && lane == 0) S.vsc[r.cfv] = enc.sc;
+    auto row = [&](int a) { return cfv_in<V16>(T, S, kid_cfv[a]); };
     if (q != p || r.lock_off >= 0 || MODE == 1) {
         // opponent node: sum; locked node: lock-weighted; best response: max
         const float* sig = r.lock_off >= 0 ? T.lock_sigma + r.lock_off : nullptr;
@@ -2418,7 +2424,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
         dispatch_na4(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
             if constexpr (NA > 0) {
-                RowIn<H16> rw[NA];
+                RowIn<V16> rw[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) rw[a] = row(a);
                 for (uint32_t i0 = lane; i0 < nhp; i0 += kSumHands * kWave) {
@@ -2501,7 +2507,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
     } else {
         const uint32_t t = *S.iter;
         const float pos = S.disc[3 * t + 0], neg = S.disc[3 * t + 1], ds = S.disc[3 * t + 2];
-        const RowIn<H16> rp = reach_in<H16>(T, S, p, pick(r.own, p));
+        const RowIn<R16> rp = reach_in<R16>(T, S, p, pick(r.own, p));
         if constexpr (STORE == kStoreF32 || STORE == kStoreF32P) {
             // PCFR+ (kStoreF32P): sigma from Q = max(R,0) + prediction, and
             // the update also stores Q' = max(R',0) + (v - out), the next
@@ -2513,7 +2519,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
-                    RowIn<H16> rw[NA];
+                    RowIn<V16> rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
                     for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
@@ -2582,7 +2588,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                 }
             });
         } else if constexpr (STORE == kStoreH) {
-            constexpr int kStepH = H16 ? kStepHands16 : kStepHands;
+            constexpr int kStepH = ROWS ? kStepHands16 : kStepHands;
             // f16 arenas with a power-of-two scale per node (no counterpart in
             // the reference). The new scales come from bounds of the new
             // maxima, known before any value is read, so every value is
@@ -2608,7 +2614,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
-                    RowIn<H16> rw[NA];
+                    RowIn<V16> rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
                     for (uint32_t i0 = lane; i0 < nhp; i0 += kStepH * kWave) {
@@ -2766,14 +2772,14 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
     return vm;
 }
 
-template <int MODE, int STORE, bool H16 = false>
+template <int MODE, int STORE, int ROWS = 0>
 __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
                                                       uint32_t count, int p, int q) {
     const uint32_t w = wave_item();
     if (w >= count) return;
     const uint32_t lane = threadIdx.x & 63;
     const ActRec r = recs[w];
-    float vm = up_action_node<MODE, STORE, H16>(T, S, r, p, q, lane);
+    float vm = up_action_node<MODE, STORE, ROWS>(T, S, r, p, q, lane);
     if (S.vmax) {  // the row's max |value|, for the parent's f16 scale
         vm = wave_max_f32(vm);
         if (lane == 0) S.vmax[r.cfv] = vm;

@@ -1,7 +1,13 @@
 """Summaries of rocprofv3 runs of the benchmark, as committed under profiles/.
 
     python -m gtopen_amd.profiling kernels TRACE_DIR ITERATIONS
-    python -m gtopen_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR ITERATIONS COPY_MIB
+    python -m gtopen_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR ITERATIONS COPY_MIB [WORKLOAD]
+    python -m gtopen_amd.profiling report PREFIX ITERATIONS COPY_MIB ALGO_BYTES TITLE BENCH_COMMAND
+
+`report` reads PREFIX_trace, PREFIX_fetch, PREFIX_write and the copy runs
+copy_fetch / copy_write beside them, and prints the markdown summary: the
+commands, the kernel table, bytes per kernel and the iteration's total
+against ALGO_BYTES (the algorithmic bytes of one iteration).
 
 `kernels` reads the kernel-stats CSV of a `rocprofv3 --kernel-trace --stats
 --output-format csv` run and prints a markdown table (calls, total ms, ms
@@ -27,9 +33,11 @@ def _csvs(root, suffix):
 
 
 def short_name(name):
-    """Kernel name without the namespace, argument list and template
-    keywords: 'gto::k_up_action<0, 0>(...)' -> 'k_up_action<0, 0>'."""
+    """Kernel name without the return type, namespace and argument list:
+    'void gto::k_up_action<0, 0, false>(...)' -> 'k_up_action<0, 0, false>'."""
     n = name.strip().strip('"')
+    if n.startswith("void "):
+        n = n[5:]
     depth, end = 0, len(n)
     for k, ch in enumerate(n):
         if ch == "<":
@@ -97,12 +105,52 @@ def traffic(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, iterations, co
     return {"iterations_in_run": iterations, "fetch_scale": fscale, "write_scale": wscale, "kernels": ks}
 
 
+def report(prefix, iterations, copy_mib, algo_bytes, title, bench):
+    root = os.path.dirname(prefix)
+    cf, cw = os.path.join(root, "copy_fetch"), os.path.join(root, "copy_write")
+    t = traffic(prefix + "_fetch", prefix + "_write", cf, cw, iterations, copy_mib)
+    fk = max(counter_sums(cf, "FETCH_SIZE").values())
+    wk = max(counter_sums(cw, "WRITE_SIZE").values())
+    out = [f"# {title}", "", "Raw data: `rocprofv3` on one MI355X (gfx950). Commands (repo root, `TMPDIR=/tmp`;",
+           "OUT is the output folder):", "", "```"]
+    name = os.path.basename(prefix)
+    out.append(f"rocprofv3 --kernel-trace --stats --output-format csv -d OUT/{name}_trace -o run -- {bench}")
+    out.append(f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/{name}_fetch -o run -- {bench}")
+    out.append(f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/{name}_write -o run -- {bench}")
+    out.append(f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/copy_fetch -o copy -- python3 bench.py --copy-probe {copy_mib:g}")
+    out.append(f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/copy_write -o copy -- python3 bench.py --copy-probe {copy_mib:g}")
+    out.append(f"python3 -m gtopen_amd.profiling report OUT/{name} {iterations} {copy_mib:g} {algo_bytes:g} '<title>' '<bench command>'")
+    out += ["```", "", f"The process runs {iterations} DCFR iterations (warm-up, timed hipGraph replays, and "
+            "event-timed eager launches for the per-kernel split); every figure below is per iteration.", "",
+            "## Kernel trace (`--kernel-trace --stats`)", "", kernels_markdown(prefix + "_trace", iterations), "",
+            "## HBM bytes (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`)", "",
+            f"Calibration: a {copy_mib:g} MiB device-to-device copy reads FETCH_SIZE = {fk:.1f} KiB and "
+            f"WRITE_SIZE = {wk:.1f} KiB, so the counters are scaled by {t['fetch_scale']:.3f} (fetch) and "
+            f"{t['write_scale']:.3f} (write).", "",
+            "| kernel | read GB / iteration | written GB / iteration |", "|---|---:|---:|"]
+    tr = tw = 0.0
+    for k, v in sorted(t["kernels"].items(), key=lambda kv: -(kv[1]["read_bytes_per_iter"] + kv[1]["write_bytes_per_iter"])):
+        r, w = v["read_bytes_per_iter"], v["write_bytes_per_iter"]
+        tr, tw = tr + r, tw + w
+        out.append(f"| `{k}` | {r / 1e9:.3f} | {w / 1e9:.3f} |")
+    out += [f"| **iteration** | **{tr / 1e9:.3f}** | **{tw / 1e9:.3f}** |", "",
+            f"Whole iteration: {(tr + tw) / 1e9:.2f} GB against {algo_bytes / 1e9:.2f} GB of algorithmic arena "
+            f"bytes ({(tr + tw) / algo_bytes:.2f}x)."]
+    return "\n".join(out)
+
+
 def main(argv):
     if argv and argv[0] == "kernels" and len(argv) == 3:
         print(kernels_markdown(argv[1], int(argv[2])))
         return 0
-    if argv and argv[0] == "traffic" and len(argv) == 7:
-        print(json.dumps(traffic(argv[1], argv[2], argv[3], argv[4], int(argv[5]), float(argv[6])), indent=1))
+    if argv and argv[0] == "traffic" and len(argv) in (7, 8):
+        t = traffic(argv[1], argv[2], argv[3], argv[4], int(argv[5]), float(argv[6]))
+        if len(argv) == 8:
+            t = {"workload": argv[7], **t}
+        print(json.dumps(t, indent=1))
+        return 0
+    if argv and argv[0] == "report" and len(argv) == 7:
+        print(report(argv[1], int(argv[2]), float(argv[3]), float(argv[4]), argv[5], argv[6]))
         return 0
     print(__doc__, file=sys.stderr)
     return 2
