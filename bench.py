@@ -71,13 +71,16 @@ def measured_traffic(args, kind):
     if prof is None:
         return None
     store = {"f32": 0, "compressed": 1, "half": 2, "half_rows": 2}[args.storage]
-    h16 = "true" if args.storage == "half_rows" else "false"  # the kernels' f16-row template flag
+    f16 = args.storage == "half_rows"  # the kernels' f16-row template argument (a flag, or a reach/value mask)
     if kind == "river":
-        keys = [k for k in prof["kernels"] if k.startswith("k_river<") and k.endswith(f", {store}>")]
-        key = keys[0] if keys else None
+        names = [k for k in prof["kernels"] if k.startswith("k_river<") and k.endswith(f", {store}>")]
+    elif kind == "up_action":
+        names = [f"k_up_action<0, {store}, {x}>" for x in (("3", "true") if f16 else ("0", "false"))]
+    elif kind == "down_action":
+        names = ["k_down_action<%d, %s>" % ({0: 0, 1: 2, 2: 5}[store], "true" if f16 else "false")]
     else:
-        key = {"up_action": f"k_up_action<0, {store}, {h16}>",
-               "down_action": "k_down_action<%d, %s>" % ({0: 0, 1: 2, 2: 5}[store], h16)}.get(kind)
+        names = []
+    key = next((k for k in names if k in prof["kernels"]), None)
     k = prof["kernels"].get(key)
     if not k:
         return None

@@ -3,8 +3,12 @@
     python -m gtopen_amd.profiling kernels TRACE_DIR ITERATIONS
     python -m gtopen_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR ITERATIONS COPY_MIB [WORKLOAD]
     python -m gtopen_amd.profiling report PREFIX ITERATIONS COPY_MIB ALGO_BYTES TITLE BENCH_COMMAND
+    python -m gtopen_amd.profiling counters PMC_DIR ITERATIONS
 
-`report` reads PREFIX_trace, PREFIX_fetch, PREFIX_write and the copy runs
+`counters` prints every counter of a `--pmc` run per kernel and iteration
+(the kernels with the most SQ_WAVE_CYCLES first), with the LDS
+bank-conflict share SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE when both are
+present. `report` reads PREFIX_trace, PREFIX_fetch, PREFIX_write and the copy runs
 copy_fetch / copy_write beside them, and prints the markdown summary: the
 commands, the kernel table, bytes per kernel and the iteration's total
 against ALGO_BYTES (the algorithmic bytes of one iteration).
@@ -139,6 +143,27 @@ def report(prefix, iterations, copy_mib, algo_bytes, title, bench):
     return "\n".join(out)
 
 
+def counters_markdown(pmc_dir, iterations):
+    names = set()
+    for path in _csvs(pmc_dir, "counter_collection.csv"):
+        with open(path, newline="") as f:
+            names.update(row["Counter_Name"] for row in csv.DictReader(f))
+    names = sorted(names)
+    vals = {c: counter_sums(pmc_dir, c) for c in names}
+    kernels = set().union(*(v.keys() for v in vals.values()))
+    order = sorted(kernels, key=lambda k: -vals.get("SQ_WAVE_CYCLES", {}).get(k, 0.0))
+    share = "SQ_LDS_BANK_CONFLICT" in vals and "SQ_LDS_IDX_ACTIVE" in vals
+    head = ["kernel"] + names + (["bank-conflict share"] if share else [])
+    lines = ["| " + " | ".join(head) + " |", "|---" + "|---:" * (len(head) - 1) + "|"]
+    for k in order:
+        row = [f"`{k}`"] + [f"{vals[c].get(k, 0.0) / iterations:.4g}" for c in names]
+        if share:
+            act = vals["SQ_LDS_IDX_ACTIVE"].get(k, 0.0)
+            row.append(f"{vals['SQ_LDS_BANK_CONFLICT'].get(k, 0.0) / act:.3f}" if act else "-")
+        lines.append("| " + " | ".join(row) + " |")
+    return "\n".join(lines)
+
+
 def main(argv):
     if argv and argv[0] == "kernels" and len(argv) == 3:
         print(kernels_markdown(argv[1], int(argv[2])))
@@ -148,6 +173,9 @@ def main(argv):
         if len(argv) == 8:
             t = {"workload": argv[7], **t}
         print(json.dumps(t, indent=1))
+        return 0
+    if argv and argv[0] == "counters" and len(argv) == 3:
+        print(counters_markdown(argv[1], int(argv[2])))
         return 0
     if argv and argv[0] == "report" and len(argv) == 7:
         print(report(argv[1], int(argv[2]), float(argv[3]), float(argv[4]), argv[5], argv[6]))
