@@ -548,7 +548,7 @@ def bench_batched(args):
         "config": {"workload": f"batched_flops: {len(cfgs)} seeded canonical flops, per-flop menus from "
                                "{25,33,50,75,125,a}, 1176-combo Beta(0.7,0.7) ranges",
                    "nodes_total": int(sum(r["nodes"] for r in rep["flops"])), "waves": rep["waves"],
-                   "parallelism": "single GPU, engines of a wave in one parent hipGraph",
+                   "parallelism": "single GPU, the engines of a wave iterating together (each engine's iteration graph on its own stream)",
                    "schedules": [{k: sc[k] for k in ("wave", "show_nodes", "per_level")} for sc in rep.get("schedules", [])]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,

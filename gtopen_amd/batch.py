@@ -1,7 +1,7 @@
-"""Batches of GPU engines advanced by one graph launch per iteration
-(gto_batch_*): every engine's steady-state iteration is a child graph of one
-parent graph, so independent spots (the flops of a report) iterate
-concurrently on the device."""
+"""Batches of GPU engines iterated together (gto_batch_*): each batch
+iteration launches every engine's captured iteration graph on the engine's
+own stream, engines interleaved, so independent spots (the flops of a
+report) share the device."""
 import ctypes
 
 from . import _lib as L

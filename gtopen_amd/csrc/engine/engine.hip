@@ -1710,75 +1710,23 @@ struct gto_hip {
     std::unique_ptr<Engine> e;
 };
 
-// A batch of engines (independent spots, e.g. the flops of a report) whose
-// steady-state iterations are captured as child graphs of one parent graph:
-// one hipGraphLaunch advances every spot by an iteration, and the children,
-// having no edges between them, run concurrently on the device.
+// A batch of engines (independent spots, e.g. the flops of a report) that
+// iterate together: each batch iteration launches every engine's captured
+// iteration graph on that engine's own stream, engines interleaved, and the
+// batch synchronizes once at the end. Spots' kernels then share the device
+// through its hardware queues. Measured on 16 report flops: 4.7 ms per
+// flop-iteration, against 5.4 ms for the engines one after another and 5.6
+// ms for child graphs of parent graphs (a parent graph's children ran one
+// after another, 1.04 kernels in flight on average, even with one parent
+// graph per stream).
 struct gto_batch {
     std::vector<Engine*> eng;
-    std::vector<hipEvent_t> ready;
-    std::vector<uint64_t> gens;
-    hipStream_t stream = nullptr;
-    hipGraphExec_t exec = nullptr;
-
-    ~gto_batch() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        for (hipEvent_t e : ready) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    void build() {
-        if (exec) {
-            HIP_OK(hipGraphExecDestroy(exec));
-            exec = nullptr;
-        }
-        hipGraph_t parent = nullptr;
-        HIP_OK(hipGraphCreate(&parent, 0));
-        try {
-            for (Engine* e : eng) {
-                hipGraph_t child = e->capture_iteration();
-                hipGraphNode_t node;
-                hipError_t rc = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, child);
-                (void)hipGraphDestroy(child);
-                HIP_OK(rc);
-            }
-            HIP_OK(hipGraphInstantiate(&exec, parent, nullptr, nullptr, 0));
-        } catch (...) {
-            (void)hipGraphDestroy(parent);
-            throw;
-        }
-        HIP_OK(hipGraphDestroy(parent));
-        gens.clear();
-        for (Engine* e : eng) gens.push_back(e->generation());
-    }
 
     void iterate(uint32_t n) {
         for (Engine* e : eng) e->reserve_iterations(n);
-        uint32_t k = 0;
-        // engines not in the steady state (fresh, or arenas just uploaded)
-        // take one iteration on their own first, so the counts stay aligned
-        bool all = true;
-        for (Engine* e : eng) all = all && e->steady();
-        if (!all && n > 0) {
+        for (uint32_t k = 0; k < n; ++k)
             for (Engine* e : eng) e->iterate(1);
-            k = 1;
-        }
-        if (k == n) {
-            for (Engine* e : eng) e->synchronize();
-            return;
-        }
-        bool stale = !exec;
-        for (size_t i = 0; i < eng.size() && !stale; ++i) stale = gens[i] != eng[i]->generation();
-        if (stale) build();
-        for (size_t i = 0; i < eng.size(); ++i) {
-            HIP_OK(hipEventRecord(ready[i], eng[i]->stream()));
-            HIP_OK(hipStreamWaitEvent(stream, ready[i], 0));
-        }
-        for (; k < n; ++k) {
-            HIP_OK(hipGraphLaunch(exec, stream));
-            for (Engine* e : eng) e->advanced(1);
-        }
-        HIP_OK(hipStreamSynchronize(stream));
+        for (Engine* e : eng) e->synchronize();
     }
 
     // wall milliseconds per batch iteration of `iters` iterations after a
@@ -1879,13 +1827,6 @@ int gto_batch_create(gto_hip* const* engines, uint32_t n, gto_batch** out) {
             b->eng.push_back(engines[i]->e.get());
             if (b->eng.back()->device() != b->eng[0]->device())
                 throw std::runtime_error("gto_batch_create: engines of one batch must share a device");
-        }
-        HIP_OK(hipSetDevice(b->eng[0]->device()));
-        HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-        for (uint32_t i = 0; i < n; ++i) {
-            hipEvent_t ev;
-            HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            b->ready.push_back(ev);
         }
         *out = b.release();
     });

@@ -4,7 +4,7 @@ server's report worker (crates/server/src/main.rs:1583-1772, report_solve
 
 Flops are grouped into waves that fit the device memory (the engines' own
 VRAM estimates against free memory); a wave's engines iterate together,
-one parent-graph launch per iteration (gtopen_amd.batch). Every
+each engine's iteration graph on its own stream (gtopen_amd.batch). Every
 ``check_every`` iterations each unfinished flop's exploitability is taken;
 a flop that reaches the target (or the iteration cap) is recorded and leaves
 the batch, so uneven trees finish at their own pace. Rows carry the

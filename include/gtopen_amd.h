@@ -257,13 +257,14 @@ int gto_hip_download_compressed(gto_hip* h, int p, int16_t* regrets_q, uint16_t*
                                 float* regret_scale, float* strat_scale);
 
 /* Batches: several engines (independent spots on one device, e.g. the
- * flops of a report, main.rs:1583-1772) advanced together. Each engine's
- * steady-state iteration is captured as a child graph of one parent graph,
- * so one hipGraphLaunch iterates every spot and the children run
- * concurrently. Engines stay owned by the caller and must outlive the
- * batch; between gto_batch_iterate calls each engine may be used on its own
- * (exploitability, downloads, locks: the batch re-captures what changed).
- * No reference counterpart: the reference solves report flops one by one. */
+ * flops of a report, main.rs:1583-1772) advanced together. A batch
+ * iteration launches every engine's captured iteration graph on the
+ * engine's own stream, engines interleaved, so the spots' kernels share the
+ * device through its hardware queues. Engines stay owned by the caller and
+ * must outlive the batch; between gto_batch_iterate calls each engine may
+ * be used on its own (exploitability, downloads, locks: an engine
+ * re-captures its graph when they change it). No reference counterpart:
+ * the reference solves report flops one by one. */
 typedef struct gto_batch gto_batch;
 int gto_batch_create(gto_hip* const* engines, uint32_t n, gto_batch** out);
 /* n iterations of every engine (an engine not in the steady state first
@@ -334,7 +335,7 @@ int gto_hip_get_schedule(const gto_hip* h, gto_schedule* s);
  * milliseconds per iteration (GPU time for an engine, wall time for a
  * batch); best (may be NULL): the chosen schedule. The batch form applies
  * each candidate to every engine of the batch and times the batch's
- * parent-graph iterations. */
+ * iterations. */
 #define GTO_SCHEDULE_CANDIDATES 7
 int gto_hip_autotune(gto_hip* h, uint32_t iters, gto_schedule* best, float* ms);
 int gto_batch_autotune(gto_batch* b, uint32_t iters, gto_schedule* best, float* ms);

@@ -1,4 +1,4 @@
-"""Batched engines (one parent-graph launch per iteration) and the report
+"""Batched engines (every engine's iteration graph on its own stream) and the report
 runner on the GPU: batched spots must evolve exactly as when each is solved
 alone, and match the CPU oracle."""
 import json

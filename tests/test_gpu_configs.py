@@ -206,8 +206,8 @@ def test_raked_trajectories_match_oracle(built):
 @pytest.mark.timeout(900)
 def test_batched_flops_through_the_batch_path_match_oracle(built):
     """Report flops of the batched-flops workload (1176-combo Beta(0.7,0.7)
-    ranges, random bet menus, no raises) advanced together by the batch's
-    parent-graph launches: 20 iterations against the oracle, exploitability
+    ranges, random bet menus, no raises) advanced together by the batch:
+    20 iterations against the oracle, exploitability
     within 1e-4 of the pot at 10 and 20."""
     import oracle
     from gtopen_amd import GpuSolver, Spot, workloads
