@@ -22,6 +22,32 @@ def _arenas(g):
     return [np.asarray(a, np.float64).copy() for a in (*r, *s)]
 
 
+def _odd_turn_spot():
+    """A turn spot with odd hand counts (301 / 457 combos): the update then
+    takes the hands one at a time (with an even count it reads and writes
+    them in pairs)."""
+    from gtopen_amd import Spot, SpotConfig, StreetSizing, TreeConfig, card_str, workloads as W
+
+    rng = np.random.default_rng(5)
+    board = W._board(rng, 4)
+    free = W._free_combos(board)
+    rngs = []
+    for n in (301, 457):
+        rngs.append(W._range_text(sorted(int(x) for x in rng.choice(free, size=n, replace=False)), W._beta(rng, n)))
+    s = [StreetSizing.of("", ""), StreetSizing.of("33 75 a", "2.5x a"), StreetSizing.of("33 75 a", "2.5x a")]
+    tree = TreeConfig(starting_pot=12.0, effective_stack=94.0, oop=s, ip=[StreetSizing(**vars(x)) for x in s])
+    spot = Spot(SpotConfig("".join(card_str(c) for c in board), rngs[0], rngs[1], tree))
+    assert spot.num_hands[0] % 2 == 1 and spot.num_hands[1] % 2 == 1
+    return spot
+
+
+def _spots():
+    from gtopen_amd import Spot, workloads
+
+    return (("river", Spot(workloads.river_spot())), ("turn", Spot(workloads.turn_spot())),
+            ("odd-hand turn", _odd_turn_spot()))
+
+
 def test_half_rows_follow_f32_rows(built):
     """One iteration from zero arenas on the river and turn spots: the f16-row
     engine's decoded arenas against the f32-row engine's (same f16 arena
@@ -32,11 +58,10 @@ def test_half_rows_follow_f32_rows(built):
     only: regret matching turns near-zero regrets (actions a hand is
     indifferent between) into different strategies under any perturbation,
     so single entries part ways after the first iteration."""
-    from gtopen_amd import GpuSolver, Spot, workloads
+    from gtopen_amd import GpuSolver
 
-    for name, cfg in (("river", workloads.river_spot()), ("turn", workloads.turn_spot())):
-        spot = Spot(cfg)
-        _say(f"case: {name} spot, 1 iteration, f16 rows vs f32 rows (f16 arenas)")
+    for name, spot in _spots():
+        _say(f"case: {name} spot {tuple(spot.num_hands)}, 1 iteration, f16 rows vs f32 rows (f16 arenas)")
         a, b = GpuSolver(spot, storage="half"), GpuSolver(spot, storage="half_rows")
         a.iterate(1)
         b.iterate(1)
@@ -51,16 +76,18 @@ def test_half_rows_follow_f32_rows(built):
 
 
 def test_half_rows_trajectory_against_fp64(built):
-    """River spot for 200 iterations and turn spot for 100, exploitability every
-    25 against the FP64 oracle's trajectory, within 1e-3 of the pot (the
-    band of the BF16 showdown form, whose reach operands carry 8 bits; f16
-    rows carry 11)."""
+    """River spot for 200 iterations and turn spot for 100, exploitability
+    every 25 against the FP64 oracle's trajectory, within 1e-3 of the pot
+    (the band of the BF16 showdown form, whose reach operands carry 8 bits;
+    f16 rows carry 11). The odd-hand turn spot is left to the one-iteration
+    check: its trajectory is chaotic enough that the CPU f32 engine itself
+    strays 0.14% of the pot from FP64 by iteration 75 (f16 rows: at most
+    0.11% over 200 iterations, f16 arenas with f32 rows 0.18%)."""
     import oracle
-    from gtopen_amd import GpuSolver, Spot, workloads
+    from gtopen_amd import GpuSolver
 
-    for name, cfg, n in (("river", workloads.river_spot(), 200), ("turn", workloads.turn_spot(), 100)):
-        spot = Spot(cfg)
-        _say(f"case: {name} spot, f16 rows vs FP64 oracle, {n} iterations, check every 25")
+    for (name, spot), n in zip(_spots()[:2], (200, 100)):
+        _say(f"case: {name} spot {tuple(spot.num_hands)}, f16 rows vs FP64 oracle, {n} iterations, check every 25")
         g, ref = GpuSolver(spot, storage="half_rows"), oracle.Oracle(spot, precision=64)
         pot = spot.starting_pot
         for k in range(n // 25):
