@@ -614,6 +614,11 @@ __device__ __forceinline__ double dpp_zero(double x) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xF, false);
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_zero(float x) {
+    const int u = __builtin_bit_cast(int, x);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, u, CTRL, ROWS, 0xF, false));
+}
 __device__ __forceinline__ double readlane_f64(double x, int l) {
     const uint64_t u = __builtin_bit_cast(uint64_t, x);
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
@@ -621,7 +626,18 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Inclusive prefix sum over the wavefront's 64 lanes (all lanes active).
+// Inclusive prefix sum over the wavefront's 64 lanes (all lanes active),
+// in f64 or f32.
+template <class X>
+__device__ __forceinline__ X wave_incl_sum(X x) {
+    x += dpp_zero<0x111>(x);
+    x += dpp_zero<0x112>(x);
+    x += dpp_zero<0x114>(x);
+    x += dpp_zero<0x118>(x);
+    x += dpp_zero<0x142, 0xA>(x);
+    x += dpp_zero<0x143, 0xC>(x);
+    return x;
+}
 __device__ __forceinline__ double wave_incl_sum_f64(double x) {
     x += dpp_zero<0x111>(x);
     x += dpp_zero<0x112>(x);
@@ -634,6 +650,10 @@ __device__ __forceinline__ double wave_incl_sum_f64(double x) {
 
 // Sum over the wavefront (all lanes active), the same in every lane.
 __device__ __forceinline__ double wave_sum_f64(double v) { return readlane_f64(wave_incl_sum_f64(v), kWave - 1); }
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_f64(v); }
+__device__ __forceinline__ float wave_sum(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wave_incl_sum(v)), kWave - 1));
+}
 
 // Max over the wavefront (all lanes active), the same in every lane.
 __device__ __forceinline__ float wave_max_f32(float x) {
@@ -880,12 +900,28 @@ constexpr uint32_t kFoldNodes = 16;                     // fold nodes per wavefr
 constexpr size_t kFoldWaveBytes = 52 * 52 * 4 + 52 * 8;  // card matrix + per-card sums
 
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
+// Reach sums of the fold and showdown terminals: f64 over f32 reach rows
+// (the CPU engine's sums, game.rs:477-484, 517-574); f32 over f16 reach rows, whose own rounding
+// (2^-11 of each entry) is far above f32's accumulation error, at half the
+// LDS bytes of the showdown's random prefix reads.
+template <bool R16>
+struct ShowAcc {
+    using T = double;
+};
+template <>
+struct ShowAcc<true> {
+    using T = float;
+};
+
 __device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return hi * (hi - 1) / 2 + lo; }
 
 template <int ROWS = 0>
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
                                                     uint32_t count, int p, int swap) {
     constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
+    // totals and card sums in f64 (the CPU's, game.rs:477-484), f32 over
+    // f16 reach rows (as the showdown's prefix sums)
+    using Acc = typename ShowAcc<R16>::T;
     extern __shared__ double lds[];
     const int o = 1 - p;
     const uint32_t nho = T.nh[o], nhp = T.nh[p];
@@ -896,7 +932,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     const uint32_t wave = threadIdx.x >> 6;
     char* base = (char*)lds + wave * kFoldWaveBytes;
     float* M = (float*)base;
-    double* s_card = (double*)(base + 52 * 52 * 4);
+    Acc* s_card = (Acc*)(base + 52 * 52 * 4);
     // packed hand words (c1 | c2 << 8 | ...) of both players: static for the
     // wavefront's nodes; the opponent's as the two matrix positions of the
     // hand (a * 52 + b | (b * 52 + a) << 16)
@@ -930,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     for (uint32_t n = first; n < last; ++n) {
         const FoldRec r = recs[n];
         const bool live = S.alive[o][pick(r.own, o)] != 0;
-        double part = 0.0;
+        Acc part = 0;
         if (live) {
             wave_sync();  // the previous node's reads of M and s_card are done
 #pragma unroll
@@ -939,7 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
                     const float x = RowIn<R16>::dec(rh[u], rsc);
                     M[mo_pos[u] & 0xFFFFu] = x;
                     M[mo_pos[u] >> 16] = x;
-                    part += (double)x;
+                    part += (Acc)x;
                 }
         }
         // the next node's row, in flight during the rest of this node
@@ -957,11 +993,11 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
             if (V16 && lane == 0) S.vsc[r.cfv] = 1.0f;
             continue;
         }
-        const double tot = wave_sum_f64(part);
+        const Acc tot = wave_sum(part);
         wave_sync();
         if (lane < 52) {
             const float4* row = (const float4*)(M + lane * 52);
-            double acc = 0.0;
+            Acc acc = 0;
 #pragma unroll
             for (int t0 = 0; t0 < 13; t0 += 4) {
                 float4 v[4];
@@ -971,10 +1007,10 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
                     if (t0 + t < 13) {
-                        acc += (double)v[t].x;
-                        acc += (double)v[t].y;
-                        acc += (double)v[t].z;
-                        acc += (double)v[t].w;
+                        acc += (Acc)v[t].x;
+                        acc += (Acc)v[t].y;
+                        acc += (Acc)v[t].z;
+                        acc += (Acc)v[t].w;
                     }
             }
             s_card[lane] = acc;
@@ -1018,10 +1054,10 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 // LDS), then per node (all 256 lanes on every phase):
 //   1. the opponent's reach row (hand order) into LDS; the next node's row
 //      is loaded meanwhile,
-//   2. reach in strength order + its exclusive prefix P (f64, per-thread
-//      chunks and a workgroup scan),
+//   2. reach in strength order + its exclusive prefix P (f64, f32 over f16
+//      reach rows; per-thread chunks and a workgroup scan),
 //   3. per card, the exclusive prefix of the reach of the opponent hands
-//      holding it, in strength order (f64; four lanes per card, each a
+//      holding it, in strength order (as P; four lanes per card, each a
 //      quarter of the card's list, read from the strength-order row at
 //      positions held in registers for the run),
 //   4. per hand (the traverser's hands in strength order), O(1) lookups:
@@ -1041,6 +1077,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
                                                     const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
                                                     int swap) {
     constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
+    using Acc = typename ShowAcc<R16>::T;
     extern __shared__ double lds[];
     const uint32_t c = xcd_slice(blockIdx.x, gridDim.x);
     if (c >= n_chunks) return;
@@ -1052,9 +1089,9 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     const uint32_t mo = T.sd_cnt[o][b], mp = T.sd_cnt[p][b];
     const ShowLds L = show_lds_layout(mo, nhp, nho);
     char* base = (char*)lds;
-    double* P = (double*)(base + L.P);
-    double* cpre = (double*)(base + L.cpre);
-    double* wsum = (double*)(base + L.wsum);   // per wave: prefix scan total
+    Acc* P = (Acc*)(base + L.P);
+    Acc* cpre = (Acc*)(base + L.cpre);
+    Acc* wsum = (Acc*)(base + L.wsum);         // per wave: prefix scan total
     float* wmax = (float*)(wsum + kWaves);      // per wave: max |value| of the row
     float* rs = (float*)(base + L.rs);
     float* rhs = (float*)(base + L.rhs);
@@ -1179,7 +1216,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         if (pending) flush();
         // 2. strength order and the chunk sums of its exclusive prefix
         float v[kR];
-        double part = 0.0;
+        Acc part = 0;
 #pragma unroll
         for (int u = 0; u < kR; ++u) v[u] = rhs[oidx[u]];
 #pragma unroll
@@ -1188,25 +1225,25 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             const uint32_t k = lo + u;
             v[u] = k < hi ? v[u] : 0.0f;
             rs[k < hi ? k : mo] = v[u];
-            part += (double)v[u];
+            part += (Acc)v[u];
         }
-        const double incl = wave_incl_sum_f64(part);
-        const double excl = dpp_zero<0x138>(incl);  // the previous lane's; 0 in lane 0
+        const Acc incl = wave_incl_sum(part);
+        const Acc excl = dpp_zero<0x138>(incl);  // the previous lane's; 0 in lane 0
         if (lane == kWave - 1) wsum[wave] = incl;
         __syncthreads();
-        double before = 0.0, total = 0.0;
+        Acc before = 0, total = 0;
 #pragma unroll
         for (int w2 = 0; w2 < kWaves; ++w2) {
-            const double t = wsum[w2];
+            const Acc t = wsum[w2];
             before += (uint32_t)w2 < wave ? t : 0.0;
             total += t;
         }
-        double run = before + excl;
+        Acc run = before + excl;
 #pragma unroll
         for (int u = 0; u < kR; ++u) {
             const uint32_t k = lo + u;
             P[k < hi ? k : mo + 1] = run;
-            run += (double)v[u];
+            run += (Acc)v[u];
         }
         if (tid == 0) P[mo] = total;
         // 3. per-card exclusive prefixes in strength order
@@ -1214,20 +1251,20 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             float x[kSeg];
 #pragma unroll
             for (int u = 0; u < (int)kSeg; ++u) x[u] = rs[cw[u] & 0xFFFFu];
-            double a = 0.0;
+            Acc a = 0;
 #pragma unroll
-            for (int u = 0; u < (int)kSeg; ++u) a += (double)x[u];
+            for (int u = 0; u < (int)kSeg; ++u) a += (Acc)x[u];
             // exclusive scan over the card's four segments (lanes 4*card..+3,
             // one quad: quad_perm broadcasts each lane's sum to the quad)
-            const double a0 = dpp_zero<0x00>(a), a1 = dpp_zero<0x55>(a), a2 = dpp_zero<0xAA>(a), a3 = dpp_zero<0xFF>(a);
-            double y = 0.0;
+            const Acc a0 = dpp_zero<0x00>(a), a1 = dpp_zero<0x55>(a), a2 = dpp_zero<0xAA>(a), a3 = dpp_zero<0xFF>(a);
+            Acc y = 0;
             if (seg > 0) y += a0;
             if (seg > 1) y += a1;
             if (seg > 2) y += a2;
 #pragma unroll
             for (int u = 0; u < (int)kSeg; ++u) {
                 cpre[cw[u] >> 16] = y;
-                y += (double)x[u];
+                y += (Acc)x[u];
             }
             if (seg == 3) cpre[c0 + card + len] = ((a0 + a1) + a2) + a3;
         }
@@ -1243,9 +1280,9 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
             if (t >= mp) break;
             const uint4 e = ent[u];
             const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
-            const double* cp1 = cpre + (e.y & 0xFFFFu);
-            const double* cp2 = cpre + (e.y >> 16);
-            const double S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
+            const Acc* cp1 = cpre + (e.y & 0xFFFFu);
+            const Acc* cp2 = cpre + (e.y >> 16);
+            const Acc S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
             const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
             const float higher =
                 (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
