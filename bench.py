@@ -240,8 +240,8 @@ def main():
     if args.autotune > 0:
         chosen, cand = gpu.autotune(args.autotune)
         sched = {"chosen": chosen, "candidates_ms_per_iter": [round(x, 3) for x in cand],
-                 "candidates": "level sweeps: show_nodes 16/8/4 x terminals all-levels-first / per-level / "
-                               "in two parts; fused river sweep"}
+                 "candidates": "level sweeps: show_nodes 16/8/4 x terminals all-levels-first / per-level; "
+                               "fused river sweep"}
         log(f"bench: autotuner chose {chosen}; candidates ms/iter {sched['candidates_ms_per_iter']}")
     t0 = time.perf_counter()
     gpu.iterate(args.steps)

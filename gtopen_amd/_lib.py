@@ -100,7 +100,7 @@ class Schedule(ctypes.Structure):
     _fields_ = [("show_nodes", ctypes.c_uint32), ("per_level", ctypes.c_uint32), ("river", ctypes.c_uint32)]
 
 
-SCHEDULE_CANDIDATES = 10
+SCHEDULE_CANDIDATES = 7
 
 
 class RunOptions(ctypes.Structure):

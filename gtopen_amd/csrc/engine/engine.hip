@@ -146,8 +146,6 @@ class Engine {
         HIP_OK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
         HIP_OK(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
-        HIP_OK(hipStreamCreateWithFlags(&aux2_, hipStreamNonBlocking));
-        HIP_OK(hipEventCreateWithFlags(&join2_ev_, hipEventDisableTiming));
         lstream_ = stream_;
         plan_ = build_plan(v, o.isomorphism != 0);
         num_nodes_ = v.num_nodes;
@@ -180,8 +178,6 @@ class Engine {
         if (aux_) (void)hipStreamDestroy(aux_);
         if (fork_ev_) (void)hipEventDestroy(fork_ev_);
         if (join_ev_) (void)hipEventDestroy(join_ev_);
-        if (aux2_) (void)hipStreamDestroy(aux2_);
-        if (join2_ev_) (void)hipEventDestroy(join2_ev_);
         for (auto& e : evs_) (void)hipEventDestroy(e);
     }
 
@@ -760,7 +756,7 @@ class Engine {
         int c = -1;
         for (int k = 0; k < 3; ++k)
             if (kShowCaps[k] == sc.show_nodes) c = k;
-        if (c < 0 || sc.per_level > 2 || sc.river > 1) throw std::runtime_error("unknown schedule");
+        if (c < 0 || sc.per_level > 1 || sc.river > 1) throw std::runtime_error("unknown schedule");
         if (c == cap_ && sc.per_level == sched_.per_level && sc.river == sched_.river) return;
         HIP_OK(hipStreamSynchronize(stream_));
         drop_graph();
@@ -771,12 +767,11 @@ class Engine {
     // the fused river sweep runs (schedule, spot shape and engine options allow it)
     bool fused() const { return sched_.river && river_avail_ && show_mode_ == GTO_SHOWDOWN_EXACT && !locked_; }
 
-    // Candidate i of the autotuner: i < 9 level-synchronous sweeps with
-    // terminals placed by per_level = i / 3 (all levels first, per level,
-    // in two parts) and showdown runs of kShowCaps[i % 3] nodes; i = 9 the
-    // fused river sweep.
+    // Candidate i of the autotuner: i < 6 level-synchronous sweeps with
+    // per-level terminals for i >= 3 and showdown runs of kShowCaps[i % 3]
+    // nodes; i = 6 the fused river sweep.
     static gto_schedule candidate(int i) {
-        if (i >= 9) return gto_schedule{kShowCaps[0], 0u, 1u};
+        if (i >= 6) return gto_schedule{kShowCaps[0], 0u, 1u};
         return gto_schedule{kShowCaps[i % 3], (uint32_t)(i / 3), 0u};
     }
 
@@ -1047,10 +1042,6 @@ class Engine {
     // get two parallel branches); lstream_ is the stream launches go to
     hipStream_t aux_ = nullptr, lstream_ = nullptr;
     hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
-    // split terminals (schedule per_level 2): the shallow part's showdowns
-    // run on a third stream, joined by join2_ev_
-    hipStream_t aux2_ = nullptr;
-    hipEvent_t join2_ev_ = nullptr;
     Plan plan_;
     uint32_t num_nodes_ = 0;
     uint64_t data_size_[2] = {0, 0};
@@ -1077,11 +1068,6 @@ class Engine {
     ShowRec* show_all_ = nullptr;
     uint4* show_chunks_[3] = {nullptr, nullptr, nullptr};  // all-level runs per entry of kShowCaps
     uint32_t n_show_chunks_[3] = {0, 0, 0};
-    // the same terminals in two parts (deep levels first; Plan::term_split)
-    FoldRec* fold_split_ = nullptr;
-    ShowRec* show_split_ = nullptr;
-    uint4* show_split_chunks_[3] = {nullptr, nullptr, nullptr};
-    uint32_t n_show_split_chunks_[3] = {0, 0, 0};
     gto_schedule sched_{kShowCaps[0], 0, 0};
     // fused river sweeps: subtree records and programs, the LDS they need,
     // the upper tree's terminals; river-level reach rows are left stale by
@@ -1208,13 +1194,7 @@ class Engine {
                 show_chunks_[c] = (uint4*)upload_vec(plan_.show_chunks[c]);
             }
             show_chunks_mma_ = (uint4*)upload_vec(plan_.show_chunks_mma);
-            show_split_ = upload_vec(plan_.show_split);
-            for (int c = 0; c < 3; ++c) {
-                n_show_split_chunks_[c] = (uint32_t)(plan_.show_split_chunks[c].size() / 4);
-                show_split_chunks_[c] = (uint4*)upload_vec(plan_.show_split_chunks[c]);
-            }
         }
-        if (n_fold_all_) fold_split_ = upload_vec(plan_.fold_split);
         lv_.resize(plan_.levels.size());
         for (size_t l = 0; l < plan_.levels.size(); ++l) {
             const LevelLists& L = plan_.levels[l];
@@ -1536,82 +1516,6 @@ class Engine {
             launch_b(PK_UP_SHOW, show_mode_ == GTO_SHOWDOWN_FP8 ? k_up_show_mma<true> : k_up_show_mma<false>,
                      wave_grid(n_show_chunks_mma_, 1), kMmaBlock, show_mma_lds_, T_, S_, (const ShowRec*)show_all_,
                      (const uint4*)show_chunks_mma_, n_show_chunks_mma_, p, swap, show_mma_staged_ ? 1 : 0);
-        // split terminals: the deep part (levels >= term_split) like the
-        // all-level launches below; the shallow part on two side streams
-        // beside the kernels of levels >= term_split - 1, which read no
-        // shallow terminal, and joined before level term_split - 2
-        const bool split = sched_.per_level == 2;
-        bool split_joined = !split || prof_;
-        if (split) {
-            const uint32_t nfd = plan_.n_fold_deep, nfs = n_fold_all_ - plan_.n_fold_deep;
-            const uint32_t ncd = mma ? 0u : plan_.n_show_deep_chunks[cap_];
-            const uint32_t ncs = mma ? 0u : n_show_split_chunks_[cap_] - plan_.n_show_deep_chunks[cap_];
-            const uint4* ck = (const uint4*)show_split_chunks_[cap_];
-            auto fold_part = [&](uint32_t first, uint32_t n) {
-                if (n)
-                    launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(n)), fold_lds_[p], T_, S_,
-                           (const FoldRec*)fold_split_ + first, n, p, swap);
-            };
-            auto show_part = [&](uint32_t first, uint32_t n) {
-                if (n)
-                    launch(PK_UP_SHOW, show_kernel(rows), wave_grid(n, 1), show_lds_[p], T_, S_,
-                           (const ShowRec*)show_split_, ck + first, n, p, swap);
-            };
-            if (prof_) {
-                // kernels timed one by one: one stream, deep part first
-                fold_part(0, nfd);
-                show_part(0, ncd);
-                fold_part(nfd, nfs);
-                show_part(ncd, ncs);
-            } else {
-                std::exception_ptr err;
-                HIP_OK(hipEventRecord(fork_ev_, stream_));
-                HIP_OK(hipStreamWaitEvent(aux_, fork_ev_, 0));
-                try {
-                    lstream_ = aux_;
-                    fold_part(0, nfd);
-                    lstream_ = stream_;
-                    show_part(0, ncd);
-                } catch (...) {
-                    err = std::current_exception();
-                }
-                lstream_ = stream_;
-                {
-                    const hipError_t r1 = hipEventRecord(join_ev_, aux_);
-                    const hipError_t r2 = r1 == hipSuccess ? hipStreamWaitEvent(stream_, join_ev_, 0) : r1;
-                    if (!err) HIP_OK(r2);
-                }
-                if (err) std::rethrow_exception(err);
-                HIP_OK(hipEventRecord(fork_ev_, stream_));
-                HIP_OK(hipStreamWaitEvent(aux_, fork_ev_, 0));
-                HIP_OK(hipStreamWaitEvent(aux2_, fork_ev_, 0));
-                try {
-                    lstream_ = aux_;
-                    fold_part(nfd, nfs);
-                    lstream_ = aux2_;
-                    show_part(ncd, ncs);
-                } catch (...) {
-                    err = std::current_exception();
-                }
-                lstream_ = stream_;
-                const hipError_t r1 = hipEventRecord(join_ev_, aux_);
-                const hipError_t r2 = hipEventRecord(join2_ev_, aux2_);
-                if (err) {
-                    // join both branches so a capture in progress can end
-                    if (r1 == hipSuccess) (void)hipStreamWaitEvent(stream_, join_ev_, 0);
-                    if (r2 == hipSuccess) (void)hipStreamWaitEvent(stream_, join2_ev_, 0);
-                    std::rethrow_exception(err);
-                }
-                HIP_OK(r1);
-                HIP_OK(r2);
-            }
-        }
-        auto join_split = [&]() {
-            if (split_joined) return;
-            split_joined = true;
-            HIP_OK(hipStreamWaitEvent(stream_, join_ev_, 0));
-            HIP_OK(hipStreamWaitEvent(stream_, join2_ev_, 0));
-        };
         if (!sched_.per_level) {
             // terminals of every level first: they read only the down pass.
             // Fold and showdown are independent: the fold launch runs on the
@@ -1653,8 +1557,7 @@ class Engine {
         }
         for (size_t l = lv_.size(); l-- > 0;) {
             const DevLevel& D = lv_[l];
-            if (split && l + 1 < plan_.term_split) join_split();
-            if (sched_.per_level == 1) {
+            if (sched_.per_level) {
                 // this level's terminals, just before the kernels that read them
                 if (D.n_fold)
                     launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(D.n_fold)), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
@@ -1705,7 +1608,6 @@ class Engine {
                 }
             }
         }
-        join_split();
     }
 
     // One DCFR iteration: traverser 0 then 1 (cfr.rs:170-190). Each sweep

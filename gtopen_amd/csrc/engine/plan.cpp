@@ -43,61 +43,6 @@ void runs(const std::vector<ShowRec>& recs, uint32_t cap, std::vector<uint32_t>&
     }
 }
 
-// Terminals in two parts for the split schedule: the up sweep waits for the
-// deep part (levels >= s) before its first level and for the shallow part
-// only before level s - 2, whose nodes are the first with terminal children
-// above level s - 1; meanwhile the shallow part runs beside the action and
-// chance kernels of levels >= s - 1. s maximizes the smaller of the shallow
-// terminals' cost and those kernels' cost, with per-unit costs from the
-// flop SRP kernel split (ns per showdown node, per fold node, per arena
-// entry of an action node).
-void split_terminals(Plan& pl) {
-    const size_t nl = pl.levels.size();
-    std::vector<double> term(nl, 0.0), act(nl, 0.0);
-    for (size_t l = 0; l < nl; ++l) {
-        const LevelLists& ll = pl.levels[l];
-        term[l] = 4.8 * ll.show_rec.size() + 2.9 * ll.fold_rec.size();
-        for (int q = 0; q < 2; ++q)
-            for (const ActRec& r : ll.act_rec[q]) act[l] += 0.007 * r.na * pl.nh[q];
-    }
-    uint32_t best = (uint32_t)nl;
-    double best_v = -1.0;
-    for (size_t s = 2; s < nl; ++s) {
-        double t = 0.0, a = 0.0;
-        for (size_t l = 0; l < s; ++l) t += term[l];
-        for (size_t l = s - 1; l < nl; ++l) a += act[l];
-        if (std::min(t, a) > best_v) {
-            best_v = std::min(t, a);
-            best = (uint32_t)s;
-        }
-    }
-    pl.term_split = best;
-    std::vector<ShowRec> shallow_show;
-    std::vector<FoldRec> shallow_fold;
-    for (size_t l = 0; l < nl; ++l) {
-        const LevelLists& ll = pl.levels[l];
-        auto& fd = l >= best ? pl.fold_split : shallow_fold;
-        auto& sd = l >= best ? pl.show_split : shallow_show;
-        fd.insert(fd.end(), ll.fold_rec.begin(), ll.fold_rec.end());
-        sd.insert(sd.end(), ll.show_rec.begin(), ll.show_rec.end());
-    }
-    auto by_board = [](const ShowRec& a, const ShowRec& b) { return a.board < b.board; };
-    std::stable_sort(pl.show_split.begin(), pl.show_split.end(), by_board);
-    std::stable_sort(shallow_show.begin(), shallow_show.end(), by_board);
-    pl.n_fold_deep = (uint32_t)pl.fold_split.size();
-    pl.fold_split.insert(pl.fold_split.end(), shallow_fold.begin(), shallow_fold.end());
-    const uint32_t n_deep = (uint32_t)pl.show_split.size();
-    for (int c = 0; c < 3; ++c) {
-        runs(pl.show_split, kShowCaps[c], pl.show_split_chunks[c]);
-        pl.n_show_deep_chunks[c] = (uint32_t)(pl.show_split_chunks[c].size() / 4);
-        std::vector<uint32_t> sh;
-        runs(shallow_show, kShowCaps[c], sh);
-        for (size_t k = 0; k < sh.size(); k += 4) sh[k + 1] += n_deep;
-        pl.show_split_chunks[c].insert(pl.show_split_chunks[c].end(), sh.begin(), sh.end());
-    }
-    pl.show_split.insert(pl.show_split.end(), shallow_show.begin(), shallow_show.end());
-}
-
 // The program of one river subtree for traverser p: its depth-first walk as
 // RiverOps with every LDS row assigned on a stack (a row is live from its
 // op until the walk leaves the node that needed it).
@@ -670,7 +615,6 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
         for (LevelLists& ll : pl.levels) runs(ll.show_rec, kShowCaps[c], ll.show_chunks[c]);
     }
     runs(pl.show_all, kMmaNodes, pl.show_chunks_mma);
-    split_terminals(pl);
 
     build_river(v, pl, dmask);
     return pl;

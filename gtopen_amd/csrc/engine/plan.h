@@ -103,15 +103,6 @@ struct Plan {
     std::vector<uint32_t> show_chunks[3];
     // the same runs capped at kMmaNodes nodes (the MFMA showdown's N tile)
     std::vector<uint32_t> show_chunks_mma;
-    // the terminals in two parts (schedule per_level 2): levels >= term_split
-    // (the deep part, first in each list) and the levels above; showdown
-    // runs per board within each part, the deep part's runs first
-    uint32_t term_split = 0;
-    std::vector<FoldRec> fold_split;
-    uint32_t n_fold_deep = 0;
-    std::vector<ShowRec> show_split;
-    std::vector<uint32_t> show_split_chunks[3];
-    uint32_t n_show_deep_chunks[3] = {0, 0, 0};
 
     // fused river sweeps: the subtrees, their programs (deduplicated), the
     // LDS rows the programs need, and the terminals outside river subtrees

@@ -12,7 +12,7 @@ from .query import QueryMixin
 
 
 def schedule_dict(sc):
-    return {"show_nodes": int(sc.show_nodes), "per_level": int(sc.per_level), "river": bool(sc.river)}
+    return {"show_nodes": int(sc.show_nodes), "per_level": bool(sc.per_level), "river": bool(sc.river)}
 
 
 class Progress(NamedTuple):
@@ -53,15 +53,13 @@ class GpuSolver(QueryMixin):
         if showdown != "exact":
             self.set_showdown(showdown)
 
-    def set_schedule(self, show_nodes=16, per_level=0, river=False):
+    def set_schedule(self, show_nodes=16, per_level=False, river=False):
         """Launch schedule (results are identical under every schedule):
         river = fused river sweeps (one workgroup per river subtree, its rows
         in LDS) where the engine allows them; for the level sweeps, showdown
-        nodes per workgroup (16, 8 or 4) and where terminals launch
-        (per_level 0: every level's at once before the level loop; 1: each
-        level's with that level; 2: in two parts, the shallow levels' on
-        side streams beside the deepest levels of the sweep)."""
-        sc = L.Schedule(show_nodes, int(per_level), int(bool(river)))
+        nodes per workgroup (16, 8 or 4) and whether each level's terminals
+        launch with that level."""
+        sc = L.Schedule(show_nodes, int(bool(per_level)), int(bool(river)))
         L.check(L.lib().gto_hip_set_schedule(self._h, ctypes.byref(sc)), "set schedule")
 
     def schedule(self):

@@ -308,11 +308,7 @@ int gto_hip_set_showdown(gto_hip* h, int mode);
  * workgroup of the exact showdown kernel (16, 8 or 4). per_level: 0 = the
  * terminals of every level in one launch each per sweep, before the level
  * loop (the default); 1 = each level's terminals launched with that level,
- * just before the chance and action kernels that read their values; 2 =
- * the terminals in two parts, the deep levels' first, then the shallow
- * levels' on two side streams (fold, showdown) beside the up sweep's
- * deepest levels, which read none of them (the split level balances the
- * two; LDS-bound terminal kernels overlap HBM-bound action kernels). No
+ * just before the chance and action kernels that read their values. No
  * counterpart in the reference (its kernels have one fixed schedule). */
 typedef struct gto_schedule {
     uint32_t show_nodes;
@@ -330,8 +326,8 @@ int gto_hip_set_schedule(gto_hip* h, const gto_schedule* s);
 int gto_hip_get_schedule(const gto_hip* h, gto_schedule* s);
 
 /* Autotuner: for each of the GTO_SCHEDULE_CANDIDATES schedules (candidate
- * i < 9: level sweeps with show_nodes = {16, 8, 4}[i % 3], per_level =
- * i / 3; candidate 9: the fused river sweep), one warm-up
+ * i < 6: level sweeps with show_nodes = {16, 8, 4}[i % 3], per_level =
+ * i / 3; candidate 6: the fused river sweep), one warm-up
  * iteration (which captures the schedule's graph) and `iters` timed DCFR
  * iterations; the fastest schedule is then applied. The iterations are
  * ordinary iterations of the solve: it advances by
@@ -340,7 +336,7 @@ int gto_hip_get_schedule(const gto_hip* h, gto_schedule* s);
  * batch); best (may be NULL): the chosen schedule. The batch form applies
  * each candidate to every engine of the batch and times the batch's
  * iterations. */
-#define GTO_SCHEDULE_CANDIDATES 10
+#define GTO_SCHEDULE_CANDIDATES 7
 int gto_hip_autotune(gto_hip* h, uint32_t iters, gto_schedule* best, float* ms);
 int gto_batch_autotune(gto_batch* b, uint32_t iters, gto_schedule* best, float* ms);
 
