@@ -1569,44 +1569,43 @@ class Engine {
             if (D.n_chance)
                 launch_b(PK_UP_CHANCE, (rows & kRows16Value) ? k_up_chance<true> : k_up_chance<false>, wave_grid(D.n_chance, 1),
                          kChanceBlock, 0, T_, S_, (const ChanceRec*)D.chance, D.n_chance, p);
-            for (int q = 0; q < 2; ++q) {
-                if (!D.n_act[q]) continue;
-                const ActRec* recs = D.act[q];
-                const uint32_t n = D.n_act[q], g = wave_grid(n);
-                if (half_) {
-                    switch (mode) {
-                        case 0:
-                            if (rows == kRows16Reach)
-                                launch(PK_UP_ACTION, k_up_action<0, kStoreH, kRows16Reach>, g, 0, T_, S_, recs, n, p, q);
-                            else if (rows)
-                                launch(PK_UP_ACTION, k_up_action<0, kStoreH, kRows16Reach | kRows16Value>, g, 0, T_, S_,
-                                       recs, n, p, q);
-                            else launch(PK_UP_ACTION, k_up_action<0, kStoreH>, g, 0, T_, S_, recs, n, p, q);
-                            break;
-                        case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
-                        default: launch(PK_UP_ACTION, k_up_action<3, kStoreH>, g, 0, T_, S_, recs, n, p, q); break;
-                    }
-                } else if (compressed_) {
-                    switch (mode) {
-                        case 0: launch(PK_UP_ACTION, k_up_action<0, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
-                        default: launch(PK_UP_ACTION, k_up_action<3, kStoreQ>, g, 0, T_, S_, recs, n, p, q); break;
-                    }
-                } else {
-                    switch (mode) {
-                        case 0:
-                            if (pred_) launch(PK_UP_ACTION, k_up_action<0, kStoreF32P>, g, 0, T_, S_, recs, n, p, q);
-                            else launch(PK_UP_ACTION, k_up_action<0, kStoreF32>, g, 0, T_, S_, recs, n, p, q);
-                            break;
-                        case 1: launch(PK_UP_ACTION, k_up_action<1, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 2: launch(PK_UP_ACTION, k_up_action<2, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
-                        case 4: launch(PK_UP_ACTION, k_up_action<4, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
-                        default: launch(PK_UP_ACTION, k_up_action<3, kStoreF32>, g, 0, T_, S_, recs, n, p, q); break;
-                    }
-                }
+            // both players' action nodes of the level in one launch
+            const uint32_t n0 = D.n_act[0], n1 = D.n_act[1];
+            if (n0 + n1)
+                launch(PK_UP_ACTION, up_kernel(mode, rows), wave_grid(n0 + n1), 0, T_, S_, (const ActRec*)D.act[0], n0,
+                       (const ActRec*)D.act[1], n1, p);
+        }
+    }
+
+    using UpFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, const ActRec*, uint32_t, int);
+    // the up-sweep action kernel for a sweep mode (0 DCFR update, 1 best
+    // response, 2 average, 3 current, 4 given strategy) and the sweep's rows
+    UpFn up_kernel(int mode, int rows) const {
+        if (half_) {
+            switch (mode) {
+                case 0:
+                    if (rows == kRows16Reach) return k_up_action2<0, kStoreH, kRows16Reach>;
+                    if (rows) return k_up_action2<0, kStoreH, kRows16Reach | kRows16Value>;
+                    return k_up_action2<0, kStoreH>;
+                case 1: return k_up_action2<1, kStoreH>;
+                case 2: return k_up_action2<2, kStoreH>;
+                default: return k_up_action2<3, kStoreH>;
             }
+        }
+        if (compressed_) {
+            switch (mode) {
+                case 0: return k_up_action2<0, kStoreQ>;
+                case 1: return k_up_action2<1, kStoreQ>;
+                case 2: return k_up_action2<2, kStoreQ>;
+                default: return k_up_action2<3, kStoreQ>;
+            }
+        }
+        switch (mode) {
+            case 0: return pred_ ? k_up_action2<0, kStoreF32P> : k_up_action2<0, kStoreF32>;
+            case 1: return k_up_action2<1, kStoreF32>;
+            case 2: return k_up_action2<2, kStoreF32>;
+            case 4: return k_up_action2<4, kStoreF32>;
+            default: return k_up_action2<3, kStoreF32>;
         }
     }
 

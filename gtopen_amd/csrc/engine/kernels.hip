@@ -2948,6 +2948,30 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
     }
 }
 
+// Both players' action nodes of one level in one launch (they read only the
+// level below and write disjoint rows and arenas): wavefront m takes item
+// k0(m) = floor(m c0 / (c0 + c1)) of player 0's list when k0(m+1) > k0(m),
+// else item m - k0(m) of player 1's, so every stretch of the grid (each
+// XCD's slice among them) holds both lists in proportion and the lighter
+// opponent-node sums fill in beside the traverser's updates.
+template <int MODE, int STORE, int ROWS = 0>
+__global__ __launch_bounds__(kBlock) void k_up_action2(DevTree T, DevState S, const ActRec* __restrict__ r0,
+                                                       uint32_t c0, const ActRec* __restrict__ r1, uint32_t c1, int p) {
+    const uint32_t m = wave_item();
+    const uint32_t tot = c0 + c1;
+    if (m >= tot) return;
+    const uint32_t k0 = (uint32_t)(((uint64_t)m * c0) / tot);
+    const uint32_t k1 = (uint32_t)(((uint64_t)(m + 1) * c0) / tot);
+    const int q = __builtin_amdgcn_readfirstlane(k1 > k0 ? 0 : 1);
+    const uint32_t lane = threadIdx.x & 63;
+    const ActRec r = q == 0 ? r0[k0] : r1[m - k0];
+    float vm = up_action_node<MODE, STORE, ROWS>(T, S, r, p, q, lane);
+    if (S.vmax) {
+        vm = wave_max_f32(vm);
+        if (lane == 0) S.vmax[r.cfv] = vm;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Game-value gradient with respect to actor q's behavioural strategy:
 // dU/dsigma(node, a, i) = reach_q(node, i) * (chance weight of the path) *
