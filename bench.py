@@ -75,7 +75,7 @@ def measured_traffic(args, kind):
     if kind == "river":
         names = [k for k in prof["kernels"] if k.startswith("k_river<") and k.endswith(f", {store}>")]
     elif kind == "up_action":
-        names = [f"k_up_action<0, {store}, {x}>" for x in (("3", "true") if f16 else ("0", "false"))]
+        names = [f"k_up_action{v}<0, {store}, {x}>" for v in ("2", "") for x in (("3", "true") if f16 else ("0", "false"))]
     elif kind == "down_action":
         names = ["k_down_action<%d, %s>" % ({0: 0, 1: 2, 2: 5}[store], "true" if f16 else "false")]
     else:
