@@ -77,7 +77,8 @@ def measured_traffic(args, kind):
     elif kind == "up_action":
         names = [f"k_up_action{v}<0, {store}, {x}>" for v in ("2", "") for x in (("3", "true") if f16 else ("0", "false"))]
     elif kind == "down_action":
-        names = ["k_down_action<%d, %s>" % ({0: 0, 1: 2, 2: 5}[store], "true" if f16 else "false")]
+        names = ["k_down_%s<%d, %s>" % (v, {0: 0, 1: 2, 2: 5}[store], "true" if f16 else "false")
+                 for v in ("level", "action")]
     else:
         names = []
     key = next((k for k in names if k in prof["kernels"]), None)

@@ -1388,31 +1388,38 @@ class Engine {
     void launch_down(int mask, int kind, bool upper = false, bool h16 = false) {
         for (size_t l = 0; l < lv_.size(); ++l) {
             const DevLevel& D = lv_[l];
+            const uint32_t ne = upper ? D.n_edge_up : D.n_edge;
+            const EdgeRec* er = (const EdgeRec*)(upper ? D.edge_up : D.edge);
+            // the level's chance edges ride along with its first action launch
+            bool edges_done = ne == 0;
             for (int q = 0; q < 2; ++q) {
                 const uint32_t n = upper ? D.n_act_up[q] : D.n_act[q];
                 if (!((mask >> q) & 1) || !n) continue;
                 const ActRec* recs = upper ? D.act_up[q] : D.act[q];
-                const uint32_t g = wave_grid(n);
-                if (kind == kRawF32) {
-                    launch(PK_DOWN_ACTION, k_down_action<kRawF32>, g, 0, T_, S_, recs, n, q);
-                } else if (kind == kAverageF32) {
-                    if (half_) launch(PK_DOWN_ACTION, k_down_action<kAverageH>, g, 0, T_, S_, recs, n, q);
-                    else if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kAverageQ>, g, 0, T_, S_, recs, n, q);
-                    else launch(PK_DOWN_ACTION, k_down_action<kAverageF32>, g, 0, T_, S_, recs, n, q);
-                } else {
-                    if (h16) launch(PK_DOWN_ACTION, k_down_action<kCurrentH, true>, g, 0, T_, S_, recs, n, q);
-                    else if (half_) launch(PK_DOWN_ACTION, k_down_action<kCurrentH>, g, 0, T_, S_, recs, n, q);
-                    else if (compressed_) launch(PK_DOWN_ACTION, k_down_action<kCurrentQ>, g, 0, T_, S_, recs, n, q);
-                    else launch(PK_DOWN_ACTION, k_down_action<kCurrentF32>, g, 0, T_, S_, recs, n, q);
-                }
+                const uint32_t e = edges_done ? 0u : ne;
+                launch(PK_DOWN_ACTION, down_kernel(kind, h16), wave_grid(n + e), 0, T_, S_, recs, n, q, er, e, mask);
+                edges_done = true;
             }
-            const uint32_t ne = upper ? D.n_edge_up : D.n_edge;
-            if (ne) {
-                const EdgeRec* er = (const EdgeRec*)(upper ? D.edge_up : D.edge);
+            if (!edges_done) {
                 if (h16) launch(PK_DOWN_CHANCE, k_down_chance<true>, wave_grid(ne), 0, T_, S_, er, ne, mask);
                 else launch(PK_DOWN_CHANCE, k_down_chance<false>, wave_grid(ne), 0, T_, S_, er, ne, mask);
             }
         }
+    }
+
+    using DownFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, int, const EdgeRec*, uint32_t, int);
+    // the down-sweep level kernel for a strategy kind (current / average /
+    // given) of this engine's arenas; h16: f16 reach rows (DCFR sweeps of
+    // f16-row engines)
+    DownFn down_kernel(int kind, bool h16) const {
+        if (h16) {
+            if (kind != kCurrentF32) throw std::runtime_error("f16 reach rows serve the DCFR sweeps only");
+            return k_down_level<kCurrentH, true>;
+        }
+        if (kind == kRawF32) return k_down_level<kRawF32>;
+        if (kind == kAverageF32)
+            return half_ ? k_down_level<kAverageH> : compressed_ ? k_down_level<kAverageQ> : k_down_level<kAverageF32>;
+        return half_ ? k_down_level<kCurrentH> : compressed_ ? k_down_level<kCurrentQ> : k_down_level<kCurrentF32>;
     }
 
     // mode 0 = DCFR update, 1 = best response, 2 = average values, 3 = current

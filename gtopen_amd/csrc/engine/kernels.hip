@@ -717,12 +717,8 @@ __global__ void k_begin_iteration(DevState S) {
 // the zero-reach pruning test of cfr.rs:424-430 for the subtree below.
 // ---------------------------------------------------------------------------
 template <int KIND, bool H16 = false>
-__global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, const ActRec* __restrict__ recs,
-                                                        uint32_t count, int q) {
-    const uint32_t w = wave_item();
-    if (w >= count) return;
-    const uint32_t lane = threadIdx.x & 63;
-    const ActRec r = recs[w];
+__device__ __forceinline__ void down_action_node(const DevTree& T, const DevState& S, const ActRec& r, int q,
+                                                 uint32_t lane) {
     const uint32_t nh = T.nh[q];
     const int na = r.na;
     const uint32_t* kid_own = T.kids + r.kid + na;
@@ -832,12 +828,8 @@ __global__ __launch_bounds__(kBlock) void k_down_action(DevTree T, DevState S, c
 // hold the dealt card zeroed (cfr.rs:615-629), for the players in `mask`.
 // ---------------------------------------------------------------------------
 template <bool H16 = false>
-__global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, const EdgeRec* __restrict__ recs,
-                                                        uint32_t count, int mask) {
-    const uint32_t w = wave_item();
-    if (w >= count) return;
-    const uint32_t lane = threadIdx.x & 63;
-    const EdgeRec r = recs[w];
+__device__ __forceinline__ void down_edge(const DevTree& T, const DevState& S, const EdgeRec& r, int mask,
+                                          uint32_t lane) {
     const uint64_t cm = 1ull << r.card;
     for (int q = 0; q < 2; ++q) {
         if (!((mask >> q) & 1)) continue;
@@ -878,6 +870,32 @@ __global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, c
             if constexpr (H16) S.rsc[q][pick(r.dst, q)] = enc.sc;
         }
     }
+}
+
+template <bool H16 = false>
+__global__ __launch_bounds__(kBlock) void k_down_chance(DevTree T, DevState S, const EdgeRec* __restrict__ recs,
+                                                        uint32_t count, int mask) {
+    const uint32_t w = wave_item();
+    if (w >= count) return;
+    down_edge<H16>(T, S, recs[w], mask, threadIdx.x & 63);
+}
+
+// A level of the down sweep in one launch: actor q's action nodes and the
+// chance edges below the level's chance nodes (independent: each reads the
+// level's own rows and writes its children's), the two lists interleaved
+// in proportion over the grid as in k_up_action2.
+template <int KIND, bool H16 = false>
+__global__ __launch_bounds__(kBlock) void k_down_level(DevTree T, DevState S, const ActRec* __restrict__ acts,
+                                                       uint32_t na, int q, const EdgeRec* __restrict__ edges,
+                                                       uint32_t ne, int mask) {
+    const uint32_t m = wave_item();
+    const uint32_t tot = na + ne;
+    if (m >= tot) return;
+    const uint32_t k0 = (uint32_t)(((uint64_t)m * na) / tot);
+    const uint32_t k1 = (uint32_t)(((uint64_t)(m + 1) * na) / tot);
+    const uint32_t lane = threadIdx.x & 63;
+    if (__builtin_amdgcn_readfirstlane(k1 > k0 ? 1u : 0u)) down_action_node<KIND, H16>(T, S, acts[k0], q, lane);
+    else down_edge<H16>(T, S, edges[m - k0], mask, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1322,7 +1340,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
 // terminals (and the opponent's all-zero test after each of its actions)
 // read across hands, behind workgroup barriers. Every per-hand expression,
 // the fold and showdown association included, is the level-sweep kernels'
-// (k_down_action, k_up_fold, k_up_show<kR, kSeg>, k_up_action<0, STORE>),
+// (down_action_node, k_up_fold, k_up_show<kR, kSeg>, up_action_node<0, STORE>),
 // so the fused and level-synchronous sweeps give bit-identical results.
 // ---------------------------------------------------------------------------
 template <int KIND, int NA>
