@@ -79,6 +79,9 @@ const char* kProfNames[GTO_PROF_KINDS] = {"root", "down_action", "down_chance", 
 struct DevLevel {
     ActRec* act[2] = {nullptr, nullptr};
     uint32_t n_act[2] = {0, 0};
+    // q's nodes that no node of the other player absorbs (Plan act_keep)
+    ActRec* act_keep[2] = {nullptr, nullptr};
+    uint32_t n_act_keep[2] = {0, 0};
     FoldRec* fold = nullptr;
     uint32_t n_fold = 0;
     ShowRec* show = nullptr;
@@ -1127,6 +1130,12 @@ class Engine {
         T_.lock_sigma = nullptr;
         T_.lock_bound = 1.0f;
         T_.kids = upload_vec(plan_.kids);
+        if (!plan_.xkids.empty()) T_.xkids = upload_vec(plan_.xkids);
+        {
+            float* z = (float*)dalloc(sizeof(float) * std::max<uint32_t>(plan_.nh_max, 1));
+            HIP_OK(hipMemset(z, 0, sizeof(float) * std::max<uint32_t>(plan_.nh_max, 1)));
+            T_.zero_row = z;
+        }
         T_.chr_card = upload_vec(plan_.chr_card);
         T_.chr_cfv = upload_vec(plan_.chr_cfv);
         T_.chr_perm = upload_vec(plan_.chr_perm);
@@ -1202,6 +1211,8 @@ class Engine {
             for (int q = 0; q < 2; ++q) {
                 D.n_act[q] = (uint32_t)L.act_rec[q].size();
                 if (D.n_act[q]) D.act[q] = upload_vec(L.act_rec[q]);
+                D.n_act_keep[q] = (uint32_t)L.act_keep[q].size();
+                if (D.n_act_keep[q]) D.act_keep[q] = upload_vec(L.act_keep[q]);
             }
             D.n_fold = (uint32_t)L.fold_rec.size();
             if (D.n_fold) D.fold = upload_vec(L.fold_rec);
@@ -1517,6 +1528,9 @@ class Engine {
     void launch_up(int p, int mode, int swap = 0) {
         // the DCFR sweeps of f16-row engines read and write f16 rows
         const int rows = mode == 0 ? rows_ : 0;
+        // DCFR sweeps of f32 engines without locks: the traverser's nodes sum
+        // their absorbed opponent children's children themselves
+        const bool absorb = mode == 0 && rows == 0 && !half_ && !compressed_ && !pred_ && !locked_ && T_.xkids;
         const FoldFn fold_fn = fold_kernel(rows);
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
         if (mma)
@@ -1576,15 +1590,20 @@ class Engine {
             if (D.n_chance)
                 launch_b(PK_UP_CHANCE, (rows & kRows16Value) ? k_up_chance<true> : k_up_chance<false>, wave_grid(D.n_chance, 1),
                          kChanceBlock, 0, T_, S_, (const ChanceRec*)D.chance, D.n_chance, p);
-            // both players' action nodes of the level in one launch
-            const uint32_t n0 = D.n_act[0], n1 = D.n_act[1];
+            // both players' action nodes of the level in one launch; with
+            // absorbed opponent nodes only the opponent's kept nodes
+            const ActRec* l0 = D.act[0];
+            const ActRec* l1 = D.act[1];
+            uint32_t n0 = D.n_act[0], n1 = D.n_act[1];
+            if (absorb && p == 0) l1 = D.act_keep[1], n1 = D.n_act_keep[1];
+            if (absorb && p == 1) l0 = D.act_keep[0], n0 = D.n_act_keep[0];
             if (n0 + n1)
-                launch(PK_UP_ACTION, up_kernel(mode, rows), wave_grid(n0 + n1), 0, T_, S_, (const ActRec*)D.act[0], n0,
-                       (const ActRec*)D.act[1], n1, p);
+                launch(PK_UP_ACTION, up_kernel(mode, rows), wave_grid(n0 + n1), 0, T_, S_, l0, n0, l1, n1, p,
+                       absorb ? 1 : 0);
         }
     }
 
-    using UpFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, const ActRec*, uint32_t, int);
+    using UpFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, const ActRec*, uint32_t, int, int);
     // the up-sweep action kernel for a sweep mode (0 DCFR update, 1 best
     // response, 2 average, 3 current, 4 given strategy) and the sweep's rows
     UpFn up_kernel(int mode, int rows) const {

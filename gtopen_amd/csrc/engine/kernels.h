@@ -37,6 +37,8 @@ struct DevTree {
     const float* lock_sigma;
     float lock_bound;          // max(1, largest lock sigma): a locked child's reach / value bound factor
     const uint32_t* kids;      // ActRec children: value rows, then reach slots
+    const uint32_t* xkids;     // ActRec absorbed opponent children (records.h)
+    const float* zero_row;     // nh_max zeros: the rows an absorbed sum pads with
     const uint8_t* chr_card;   // ChanceRec children: dealt card
     const uint32_t* chr_cfv;   // ChanceRec children: value row (the orbit representative's)
     const uint8_t* chr_perm;   // ChanceRec children: suit symmetry onto the representative (0 = identity)

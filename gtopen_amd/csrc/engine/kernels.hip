@@ -2443,10 +2443,14 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
 //     strategy itself (kRawF32), no update.
 // Subtrees whose opponent reach is all zero produce zeros and no update.
 // ---------------------------------------------------------------------------
+// Hands per lane in one step of the f32 update at a node with absorbed
+// opponent children (up to kMaxAbsorb rows per action are read per hand).
+constexpr int kStepAbsorb = 2;
+
 template <int MODE, int STORE, int ROWS = 0>
 // Returns this lane's max |value| over the row entries it wrote.
 __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState& S, const ActRec& r, int p, int q,
-                                                uint32_t lane) {
+                                                uint32_t lane, int absorb = 0) {
     constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
     static_assert(ROWS == 0 || (MODE == 0 && STORE == kStoreH), "f16 rows serve the DCFR sweeps of f16 arenas");
     float vm = 0.0f;
@@ -2617,6 +2621,87 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
             float* __restrict__ R = S.R[p] + off;
             float* __restrict__ Sm = S.S[p] + off;
             float* __restrict__ Qp = PRED ? S.Q[p] + off : nullptr;
+            if constexpr (!PRED && ROWS == 0) {
+                if (absorb && r.xk != kNoAbsorb) {
+                    // absorbed opponent children: the child's value is the sum
+                    // of its own children's rows, added in action order from 0
+                    // as the opponent-node sum does (padding rows are zeros,
+                    // which leave the sum unchanged); a dead opponent reach
+                    // at the child gives 0
+                    const uint32_t* __restrict__ xk = T.xkids + r.xk;
+                    bool done = false;
+                    dispatch_na4(na, [&](auto NAc) {
+                        constexpr int NA = decltype(NAc)::value;
+                        if constexpr (NA > 0) {
+                            done = true;
+                            const float* __restrict__ src[NA][kMaxAbsorb];
+                            uint32_t xp = 0;
+#pragma unroll
+                            for (int a = 0; a < NA; ++a) {
+                                const uint32_t c = xk[xp++];
+                                if (c == 0) {
+                                    src[a][0] = S.cfv + (size_t)kid_cfv[a] * nh_max;
+#pragma unroll
+                                    for (int b = 1; b < kMaxAbsorb; ++b) src[a][b] = T.zero_row;
+                                } else {
+                                    const bool live = S.alive[o][xk[xp++]] != 0;
+#pragma unroll
+                                    for (int b = 0; b < kMaxAbsorb; ++b)
+                                        src[a][b] = (live && (uint32_t)b < c) ? S.cfv + (size_t)xk[xp + b] * nh_max
+                                                                             : T.zero_row;
+                                    xp += c;
+                                }
+                            }
+                            for (uint32_t i0 = lane; i0 < nhp; i0 += kStepAbsorb * kWave) {
+                                HandSigma<NA, kCurrentF32> hs[kStepAbsorb];
+                                float x[kStepAbsorb][NA][kMaxAbsorb], sm[kStepAbsorb][NA], reach[kStepAbsorb];
+                                bool ok[kStepAbsorb];
+#pragma unroll
+                                for (int h = 0; h < kStepAbsorb; ++h) {
+                                    const uint32_t i = i0 + h * kWave;
+                                    ok[h] = i < nhp;
+                                    const uint32_t ii = ok[h] ? i : i0;
+#pragma unroll
+                                    for (int a = 0; a < NA; ++a) {
+                                        hs[h].raw[a] = R[(size_t)a * nhp + ii];
+                                        sm[h][a] = Sm[(size_t)a * nhp + ii];
+#pragma unroll
+                                        for (int b = 0; b < kMaxAbsorb; ++b) x[h][a][b] = src[a][b][ii];
+                                    }
+                                    reach[h] = rp[ii];
+                                }
+#pragma unroll
+                                for (int h = 0; h < kStepAbsorb; ++h) {
+                                    if (!ok[h]) continue;
+                                    const uint32_t i = i0 + h * kWave;
+                                    float v[NA];
+#pragma unroll
+                                    for (int a = 0; a < NA; ++a) {
+                                        float t = 0.0f;
+#pragma unroll
+                                        for (int b = 0; b < kMaxAbsorb; ++b) t += x[h][a][b];
+                                        v[a] = t;
+                                    }
+                                    hs[h].finish();
+                                    float acc = 0.0f;
+#pragma unroll
+                                    for (int a = 0; a < NA; ++a) acc += hs[h].sig[a] * v[a];
+#pragma unroll
+                                    for (int a = 0; a < NA; ++a) {
+                                        const float rv = hs[h].raw[a];
+                                        const float d = rv > 0.0f ? pos : neg;
+                                        R[(size_t)a * nhp + i] = rv * d + (v[a] - acc);
+                                        Sm[(size_t)a * nhp + i] = sm[h][a] * ds + reach[h] * hs[h].sig[a];
+                                    }
+                                    out.set(i, acc);
+                                    vm = fmaxf(vm, fabsf(acc));
+                                }
+                            }
+                        }
+                    });
+                    if (done) return vm;
+                }
+            }
             dispatch_na4(na, [&](auto NAc) {
                 constexpr int NA = decltype(NAc)::value;
                 if constexpr (NA > 0) {
@@ -2974,7 +3059,8 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
 // opponent-node sums fill in beside the traverser's updates.
 template <int MODE, int STORE, int ROWS = 0>
 __global__ __launch_bounds__(kBlock) void k_up_action2(DevTree T, DevState S, const ActRec* __restrict__ r0,
-                                                       uint32_t c0, const ActRec* __restrict__ r1, uint32_t c1, int p) {
+                                                       uint32_t c0, const ActRec* __restrict__ r1, uint32_t c1, int p,
+                                                       int absorb) {
     const uint32_t m = wave_item();
     const uint32_t tot = c0 + c1;
     if (m >= tot) return;
@@ -2983,7 +3069,7 @@ __global__ __launch_bounds__(kBlock) void k_up_action2(DevTree T, DevState S, co
     const int q = __builtin_amdgcn_readfirstlane(k1 > k0 ? 0 : 1);
     const uint32_t lane = threadIdx.x & 63;
     const ActRec r = q == 0 ? r0[k0] : r1[m - k0];
-    float vm = up_action_node<MODE, STORE, ROWS>(T, S, r, p, q, lane);
+    float vm = up_action_node<MODE, STORE, ROWS>(T, S, r, p, q, lane, absorb);
     if (S.vmax) {
         vm = wave_max_f32(vm);
         if (lane == 0) S.vmax[r.cfv] = vm;

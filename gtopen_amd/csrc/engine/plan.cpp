@@ -534,6 +534,23 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
             }
         }
     }
+    // absorbed opponent nodes: an action node of one player whose parent is
+    // an action node of the other (and with at most kMaxAbsorb actions)
+    std::vector<uint8_t> absorbed(v.num_nodes, 0);
+    for (uint32_t l = 0; l < L; ++l)
+        for (int q = 0; q < 2; ++q)
+            for (uint32_t n : pl.levels[l].act[q]) {
+                const uint32_t cs = v.node_children_start[n];
+                if (v.node_num_children[n] > (uint32_t)kMaxAbsorbParent) continue;
+                for (uint32_t a = 0; a < v.node_num_children[n]; ++a) {
+                    const uint32_t m = v.children[cs + a];
+                    if (v.node_kind[m] == GTO_KIND_ACTION && v.node_player[m] != q &&
+                        v.node_num_children[m] <= (uint32_t)kMaxAbsorb) {
+                        absorbed[m] = 1;
+                        ++pl.absorbed[1 - q];
+                    }
+                }
+            }
     // work records
     for (uint32_t l = 0; l < L; ++l) {
         LevelLists& ll = pl.levels[l];
@@ -553,7 +570,25 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
                 const uint32_t cs = v.node_children_start[n];
                 for (uint32_t a = 0; a < r.na; ++a) pl.kids.push_back(pl.cfv_slot[v.children[cs + a]]);
                 for (uint32_t a = 0; a < r.na; ++a) pl.kids.push_back(pl.owner[q][v.children[cs + a]]);
+                r.xk = kNoAbsorb;
+                for (uint32_t a = 0; a < r.na; ++a)
+                    if (absorbed[v.children[cs + a]]) r.xk = 0;
+                if (r.xk == 0) {
+                    r.xk = (uint32_t)pl.xkids.size();
+                    for (uint32_t a = 0; a < r.na; ++a) {
+                        const uint32_t m = v.children[cs + a];
+                        if (!absorbed[m]) {
+                            pl.xkids.push_back(0u);
+                            continue;
+                        }
+                        const uint32_t ms = v.node_children_start[m], mn = v.node_num_children[m];
+                        pl.xkids.push_back(mn);
+                        pl.xkids.push_back(pl.owner[1 - q][m]);
+                        for (uint32_t b = 0; b < mn; ++b) pl.xkids.push_back(pl.cfv_slot[v.children[ms + b]]);
+                    }
+                }
                 ll.act_rec[q].push_back(r);
+                if (!absorbed[n]) ll.act_keep[q].push_back(r);
             }
 
         for (uint32_t n : ll.fold) {
@@ -602,6 +637,10 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
             ll.edge_rec.push_back(r);
         }
     }
+
+    // the absorbed-children reader may form (not follow) addresses up to
+    // kMaxAbsorb entries past a node's list
+    if (!pl.xkids.empty()) pl.xkids.insert(pl.xkids.end(), (size_t)kMaxAbsorb, 0u);
 
     // all-level terminal lists; showdown runs per board
     for (const LevelLists& ll : pl.levels) {

@@ -19,6 +19,10 @@ struct LevelLists {
     std::vector<uint32_t> edge_parent, edge_child, edge_card;
     // work records, in the same order as the lists above
     std::vector<ActRec> act_rec[2];
+    // q's action nodes that no action node of the other player absorbs
+    // (their parent is a chance node or the root): the opponent's list of a
+    // DCFR sweep with absorbed opponent nodes
+    std::vector<ActRec> act_keep[2];
     // the same without the nodes inside river subtrees (fused river sweeps)
     std::vector<ActRec> act_up[2];
     std::vector<EdgeRec> edge_up;
@@ -84,6 +88,9 @@ struct Plan {
 
     // children of action nodes referenced by ActRec::kid
     std::vector<uint32_t> kids;
+    // absorbed opponent children referenced by ActRec::xk (records.h)
+    std::vector<uint32_t> xkids;
+    uint64_t absorbed[2] = {0, 0};  // q's action nodes absorbed by the other player's nodes
     // chance children referenced by ChanceRec: dealt card and value row
     std::vector<uint8_t> chr_card;
     std::vector<uint32_t> chr_cfv;

@@ -19,9 +19,24 @@ struct ActRec {
     uint32_t na;        // number of actions
     uint32_t kid;       // index into the kids array
     float cw;           // product of the chance probabilities above the node
-    uint32_t pad;       // keeps the record at 48 bytes
+    uint32_t xk;        // absorbed children (kNoAbsorb: none): index into DevTree::xkids
 };
 static_assert(sizeof(ActRec) == 48, "ActRec layout");
+
+// Absorbed opponent nodes. In the DCFR sweep whose traverser acts at node n,
+// a child of n that is the opponent's action node m (with at most
+// kMaxAbsorb actions) needs no value row of its own: m's value is the sum of
+// its children's rows (the opponent's reach is already in them), so n adds
+// them up itself. xkids[n.xk ..] holds, per action a of n: 0 (the child's own
+// row, kids[kid + a]), or the count c of m's actions, then m's reach slot for
+// the opponent (whose alive flag zeroes the value) and the c value rows of
+// m's children in action order.
+// Only nodes n with at most kMaxAbsorbParent actions absorb (the update's
+// per-action-count specialisations, 1-4, read xkids; wider nodes read their
+// children's own rows).
+constexpr uint32_t kNoAbsorb = 0xFFFFFFFFu;
+constexpr int kMaxAbsorb = 3;
+constexpr int kMaxAbsorbParent = 4;
 
 struct FoldRec {
     uint32_t cfv;
