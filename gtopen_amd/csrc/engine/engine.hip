@@ -1529,8 +1529,10 @@ class Engine {
         // the DCFR sweeps of f16-row engines read and write f16 rows
         const int rows = mode == 0 ? rows_ : 0;
         // DCFR sweeps of f32 engines without locks: the traverser's nodes sum
-        // their absorbed opponent children's children themselves
-        const bool absorb = mode == 0 && rows == 0 && !half_ && !compressed_ && !pred_ && !locked_ && T_.xkids;
+        // their absorbed opponent children's children themselves (f16 arenas
+        // would bound those sums by their children's maxima, a different
+        // scale than the child row's own maximum, so they keep the rows)
+        const bool absorb = mode == 0 && !compressed_ && !pred_ && !locked_ && T_.xkids;
         const FoldFn fold_fn = fold_kernel(rows);
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
         if (mma)

@@ -2447,6 +2447,45 @@ __global__ __launch_bounds__(kChanceBlock) void k_up_chance(DevTree T, DevState 
 // opponent children (up to kMaxAbsorb rows per action are read per hand).
 constexpr int kStepAbsorb = 2;
 
+// Value sources of node r's actions when its opponent children are absorbed
+// (records.h): per action a, kMaxAbsorb f32 rows whose sum, taken from 0 in
+// order, is the child's value (the child's own row and zero rows, or the
+// absorbed child's children's rows and zero rows, or zero rows when the
+// opponent's reach at the absorbed child is dead; adding 0 changes no sum).
+// Returns max over actions of a bound of |value| (sums of the rows' recorded
+// maxima) when S.vmax is kept, else 0.
+template <int NA>
+__device__ __forceinline__ float absorbed_sources(const DevTree& T, const DevState& S, const ActRec& r, int o,
+                                                  const float* (&src)[NA][kMaxAbsorb]) {
+    const uint32_t* __restrict__ xk = T.xkids + r.xk;
+    const uint32_t* kid_cfv = T.kids + r.kid;
+    const size_t nh_max = T.nh_max;
+    uint32_t xp = 0;
+    float mv = 0.0f;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const uint32_t c = xk[xp++];
+        float bound = 0.0f;
+        if (c == 0) {
+            src[a][0] = S.cfv + (size_t)kid_cfv[a] * nh_max;
+#pragma unroll
+            for (int b = 1; b < kMaxAbsorb; ++b) src[a][b] = T.zero_row;
+            if (S.vmax) bound = S.vmax[kid_cfv[a]];
+        } else {
+            const bool live = S.alive[o][xk[xp++]] != 0;
+#pragma unroll
+            for (int b = 0; b < kMaxAbsorb; ++b) {
+                const bool use = live && (uint32_t)b < c;
+                src[a][b] = use ? S.cfv + (size_t)xk[xp + b] * nh_max : T.zero_row;
+                if (S.vmax && use) bound += S.vmax[xk[xp + b]];
+            }
+            xp += c;
+        }
+        mv = fmaxf(mv, bound);
+    }
+    return mv;
+}
+
 template <int MODE, int STORE, int ROWS = 0>
 // Returns this lane's max |value| over the row entries it wrote.
 __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState& S, const ActRec& r, int p, int q,
@@ -2628,30 +2667,13 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     // as the opponent-node sum does (padding rows are zeros,
                     // which leave the sum unchanged); a dead opponent reach
                     // at the child gives 0
-                    const uint32_t* __restrict__ xk = T.xkids + r.xk;
                     bool done = false;
                     dispatch_na4(na, [&](auto NAc) {
                         constexpr int NA = decltype(NAc)::value;
                         if constexpr (NA > 0) {
                             done = true;
-                            const float* __restrict__ src[NA][kMaxAbsorb];
-                            uint32_t xp = 0;
-#pragma unroll
-                            for (int a = 0; a < NA; ++a) {
-                                const uint32_t c = xk[xp++];
-                                if (c == 0) {
-                                    src[a][0] = S.cfv + (size_t)kid_cfv[a] * nh_max;
-#pragma unroll
-                                    for (int b = 1; b < kMaxAbsorb; ++b) src[a][b] = T.zero_row;
-                                } else {
-                                    const bool live = S.alive[o][xk[xp++]] != 0;
-#pragma unroll
-                                    for (int b = 0; b < kMaxAbsorb; ++b)
-                                        src[a][b] = (live && (uint32_t)b < c) ? S.cfv + (size_t)xk[xp + b] * nh_max
-                                                                             : T.zero_row;
-                                    xp += c;
-                                }
-                            }
+                            const float* src[NA][kMaxAbsorb];
+                            (void)absorbed_sources<NA>(T, S, r, o, src);
                             for (uint32_t i0 = lane; i0 < nhp; i0 += kStepAbsorb * kWave) {
                                 HandSigma<NA, kCurrentF32> hs[kStepAbsorb];
                                 float x[kStepAbsorb][NA][kMaxAbsorb], sm[kStepAbsorb][NA], reach[kStepAbsorb];
