@@ -97,7 +97,8 @@ class HipProfile(ctypes.Structure):
 
 
 class Schedule(ctypes.Structure):
-    _fields_ = [("show_nodes", ctypes.c_uint32), ("per_level", ctypes.c_uint32), ("river", ctypes.c_uint32)]
+    _fields_ = [("show_nodes", ctypes.c_uint32), ("per_level", ctypes.c_uint32), ("river", ctypes.c_uint32),
+                ("absorb", ctypes.c_uint32)]
 
 
 SCHEDULE_CANDIDATES = 7

@@ -759,8 +759,9 @@ class Engine {
         int c = -1;
         for (int k = 0; k < 3; ++k)
             if (kShowCaps[k] == sc.show_nodes) c = k;
-        if (c < 0 || sc.per_level > 1 || sc.river > 1) throw std::runtime_error("unknown schedule");
-        if (c == cap_ && sc.per_level == sched_.per_level && sc.river == sched_.river) return;
+        if (c < 0 || sc.per_level > 1 || sc.river > 1 || sc.absorb > 1) throw std::runtime_error("unknown schedule");
+        if (c == cap_ && sc.per_level == sched_.per_level && sc.river == sched_.river && sc.absorb == sched_.absorb)
+            return;
         HIP_OK(hipStreamSynchronize(stream_));
         drop_graph();
         cap_ = c;
@@ -774,8 +775,8 @@ class Engine {
     // per-level terminals for i >= 3 and showdown runs of kShowCaps[i % 3]
     // nodes; i = 6 the fused river sweep.
     static gto_schedule candidate(int i) {
-        if (i >= 6) return gto_schedule{kShowCaps[0], 0u, 1u};
-        return gto_schedule{kShowCaps[i % 3], (uint32_t)(i / 3), 0u};
+        if (i >= 6) return gto_schedule{kShowCaps[0], 0u, 1u, 1u};
+        return gto_schedule{kShowCaps[i % 3], (uint32_t)(i / 3), 0u, 1u};
     }
 
     // GPU milliseconds per iteration of `iters` iterations after a warm-up
@@ -1071,7 +1072,7 @@ class Engine {
     ShowRec* show_all_ = nullptr;
     uint4* show_chunks_[3] = {nullptr, nullptr, nullptr};  // all-level runs per entry of kShowCaps
     uint32_t n_show_chunks_[3] = {0, 0, 0};
-    gto_schedule sched_{kShowCaps[0], 0, 0};
+    gto_schedule sched_{kShowCaps[0], 0, 0, 1};
     // fused river sweeps: subtree records and programs, the LDS they need,
     // the upper tree's terminals; river-level reach rows are left stale by
     // fused sweeps (level sweeps recompute both players' rows after them)
@@ -1532,7 +1533,7 @@ class Engine {
         // their absorbed opponent children's children themselves (f16 arenas
         // would bound those sums by their children's maxima, a different
         // scale than the child row's own maximum, so they keep the rows)
-        const bool absorb = mode == 0 && !compressed_ && !pred_ && !locked_ && T_.xkids;
+        const bool absorb = mode == 0 && sched_.absorb && !compressed_ && !pred_ && !locked_ && T_.xkids;
         const FoldFn fold_fn = fold_kernel(rows);
         const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
         if (mma)

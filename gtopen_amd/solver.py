@@ -12,7 +12,8 @@ from .query import QueryMixin
 
 
 def schedule_dict(sc):
-    return {"show_nodes": int(sc.show_nodes), "per_level": bool(sc.per_level), "river": bool(sc.river)}
+    return {"show_nodes": int(sc.show_nodes), "per_level": bool(sc.per_level), "river": bool(sc.river),
+            "absorb": bool(sc.absorb)}
 
 
 class Progress(NamedTuple):
@@ -53,13 +54,14 @@ class GpuSolver(QueryMixin):
         if showdown != "exact":
             self.set_showdown(showdown)
 
-    def set_schedule(self, show_nodes=16, per_level=False, river=False):
+    def set_schedule(self, show_nodes=16, per_level=False, river=False, absorb=True):
         """Launch schedule (results are identical under every schedule):
         river = fused river sweeps (one workgroup per river subtree, its rows
         in LDS) where the engine allows them; for the level sweeps, showdown
-        nodes per workgroup (16, 8 or 4) and whether each level's terminals
-        launch with that level."""
-        sc = L.Schedule(show_nodes, int(bool(per_level)), int(bool(river)))
+        nodes per workgroup (16, 8 or 4), whether each level's terminals
+        launch with that level, and whether traverser nodes absorb their
+        opponent children (f32 DCFR sweeps without locks)."""
+        sc = L.Schedule(show_nodes, int(bool(per_level)), int(bool(river)), int(bool(absorb)))
         L.check(L.lib().gto_hip_set_schedule(self._h, ctypes.byref(sc)), "set schedule")
 
     def schedule(self):

@@ -321,6 +321,13 @@ typedef struct gto_schedule {
      * with 0, the default) every level is swept level-synchronously.
      * Results are bit for bit those of the level sweeps. */
     uint32_t river;
+    /* 1 (the default) = absorbed opponent nodes in the DCFR sweeps of f32
+     * engines without locks: an opponent action node with at most 3
+     * actions under a traverser node with at most 4 writes no value row;
+     * its parent adds up the node's children's rows itself, in the order
+     * the node would. 0 = every node writes its row. Bit-identical either
+     * way; other engines and sweeps ignore it. */
+    uint32_t absorb;
 } gto_schedule;
 int gto_hip_set_schedule(gto_hip* h, const gto_schedule* s);
 int gto_hip_get_schedule(const gto_hip* h, gto_schedule* s);

@@ -2,7 +2,8 @@
 launched: the fused river sweep (one workgroup per river subtree with its
 rows in LDS) or the level sweeps, and for the level sweeps the terminal
 launches (runs of 16, 8 or 4 showdown nodes per workgroup; all levels'
-terminals first, or each level's with that level). Every schedule, and any
+terminals first, or each level's with that level), and whether traverser
+nodes absorb their opponent children (f32 sweeps). Every schedule, and any
 sequence of schedules chosen while tuning, must leave the arenas
 bit-identical to the default schedule's."""
 import numpy as np
@@ -12,6 +13,13 @@ pytestmark = pytest.mark.gpu
 
 CANDIDATES = [(16, False, False), (8, False, False), (4, False, False), (16, True, False), (8, True, False),
               (4, True, False), (16, False, True)]
+# the autotuner's candidates (absorbed opponent nodes on) and the level
+# sweeps without absorbed opponent nodes
+SCHEDULES = [c + (True,) for c in CANDIDATES] + [(16, False, False, False)]
+
+
+def _key(sc):
+    return (sc["show_nodes"], sc["per_level"], sc["river"])
 
 
 def _say(msg):
@@ -29,22 +37,23 @@ def _same(a, b):
 
 @pytest.mark.parametrize("storage", ["f32", "half", "half_rows"])
 def test_every_schedule_gives_the_same_arenas(built, storage):
-    """Level sweeps under the 6 terminal schedules and the fused river sweep,
-    20 iterations each, on the river spot (one subtree: the whole tree),
-    the turn spot (48 river subtrees per river chance node, raises to a cap
-    of 10) and a report flop (1176 hands: six hands per thread)."""
+    """Level sweeps under the 6 terminal schedules (and without absorbed
+    opponent nodes) and the fused river sweep, 20 iterations each, on the
+    river spot (one subtree: the whole tree), the turn spot (48 river
+    subtrees per river chance node, raises to a cap of 10) and a report flop
+    (1176 hands: six hands per thread)."""
     from gtopen_amd import GpuSolver, Spot, workloads
 
     flop = workloads.batched_flops(n=184)[1]
     for name, cfg in (("river", workloads.river_spot()), ("turn", workloads.turn_spot()), ("report flop", flop)):
         spot = Spot(cfg)
         _say(f"case: {name} spot ({spot.num_nodes} nodes, hands {spot.num_hands}), {storage} arenas, "
-             f"20 iterations under each of {len(CANDIDATES)} schedules")
+             f"20 iterations under each of {len(SCHEDULES)} schedules")
         ref = None
-        for show_nodes, per_level, river in CANDIDATES:
+        for show_nodes, per_level, river, absorb in SCHEDULES:
             g = GpuSolver(spot, storage=storage)
-            g.set_schedule(show_nodes, per_level, river)
-            assert g.schedule() == {"show_nodes": show_nodes, "per_level": per_level, "river": river}
+            g.set_schedule(show_nodes, per_level, river, absorb)
+            assert g.schedule() == {"show_nodes": show_nodes, "per_level": per_level, "river": river, "absorb": absorb}
             if river and storage != "half_rows":  # f16 rows: level sweeps only
                 assert g.river_entries > 0
             g.iterate(20)
@@ -54,7 +63,7 @@ def test_every_schedule_gives_the_same_arenas(built, storage):
             if ref is None:
                 ref = (a, e)
             else:
-                assert _same(ref[0], a), f"schedule ({show_nodes}, {per_level}, {river}) changed the arenas"
+                assert _same(ref[0], a), f"schedule ({show_nodes}, {per_level}, {river}, {absorb}) changed the arenas"
                 assert e == ref[1]
 
 
@@ -92,7 +101,7 @@ def test_autotune_is_an_ordinary_part_of_the_solve(built):
     g = GpuSolver(spot)
     best, ms = g.autotune(1)
     _say(f"  chose {best}; ms per iteration per candidate {['%.3f' % x for x in ms]}")
-    assert (best["show_nodes"], best["per_level"], best["river"]) in CANDIDATES
+    assert _key(best) in CANDIDATES and best["absorb"]
     assert g.schedule() == best
     assert len(ms) == len(CANDIDATES) and all(x > 0 for x in ms)
     assert g.iteration == 2 * len(CANDIDATES)
@@ -116,7 +125,7 @@ def test_batch_autotune(built):
     b = Batch(gs)
     best, ms = b.autotune(1)
     _say(f"  chose {best}; ms per batch iteration per candidate {['%.3f' % x for x in ms]}")
-    assert (best["show_nodes"], best["per_level"], best["river"]) in CANDIDATES
+    assert _key(best) in CANDIDATES and best["absorb"]
     assert all(g.schedule() == best for g in gs)
     b.iterate(4)
     b.close()
