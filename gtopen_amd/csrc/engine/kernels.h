@@ -143,7 +143,7 @@ __host__ __device__ inline MmaLds mma_lds_layout(uint32_t mo, uint32_t mp, uint3
     L.B = 0;                              // 32 x rs elements
     L.bsc = L.B + (fp8 ? 1 : 2) * 32 * L.rs;  // 32 f32: per-node inverse scale of B
     L.nodes = L.bsc + 32 * 4;             // 32 ShowNode
-    L.vmx = L.nodes + 32 * 24;            // 32 u32
+    L.vmx = L.nodes + 32 * sizeof(ShowNode);  // 32 u32
     L.onb = L.vmx + 32 * 4;               // 42 u32: bitmap of p's hands on the board (nout == 0)
     L.lbub = L.onb + 42 * 4;              // mp u32: rank bounds of p's sorted position i
     L.oidx = L.lbub + 4 * mp;             // kpad u16: opponent hand of sorted position k
