@@ -534,8 +534,9 @@ Plan build_plan(const gto_spot_view& v, bool iso) {
             }
         }
     }
-    // absorbed opponent nodes: an action node of one player whose parent is
-    // an action node of the other (and with at most kMaxAbsorb actions)
+    // absorbed opponent nodes (records.h): an action node of one player with
+    // at most kMaxAbsorb actions whose parent is an action node of the other
+    // player with at most kMaxAbsorbParent actions
     std::vector<uint8_t> absorbed(v.num_nodes, 0);
     for (uint32_t l = 0; l < L; ++l)
         for (int q = 0; q < 2; ++q)
