@@ -58,11 +58,24 @@ DTYPE = {"f32": "f32", "compressed": "i16/u16 arenas, f32 math", "half": "f16 ar
          "half_rows": "f16 arenas and rows, f32 math"}
 
 
-def traffic_profile(args):
+def traffic_profile(args, storage=None):
     """The committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
     workload (profiles/<config>_<storage>_traffic.json), if any."""
-    path = os.path.join(ROOT, "profiles", f"{args.config}_{args.storage}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{args.config}_{storage or args.storage}_traffic.json")
     return json.load(open(path)) if os.path.exists(path) else None
+
+
+def iteration_traffic_of(args, storage, D):
+    """HBM bytes of a whole DCFR iteration from the committed counter
+    profile of this workload and storage form, against the algorithmic arena
+    bytes (20 B per entry with f32 arenas, 10 B with 16-bit arenas)."""
+    tp = traffic_profile(args, storage)
+    if tp is None:
+        return None
+    alg = (20.0 if storage == "f32" else 10.0) * D
+    tb = sum(k["read_bytes_per_iter"] + k["write_bytes_per_iter"] for k in tp["kernels"].values())
+    return {"bytes": round(tb), "algorithmic_bytes": alg, "ratio": round(tb / alg, 3),
+            "source": f"profiles/{args.config}_{storage}_traffic.json (all kernels of the profiled run, per iteration)"}
 
 
 def measured_traffic(args, kind):
@@ -263,14 +276,7 @@ def main():
     whole = 5.0 * eb * D / (ms_per_step * 1e-3) / 1e9
     gpu.close()
     # HBM bytes of a whole iteration from the committed counter profile
-    tp = traffic_profile(args)
-    iteration_traffic = None
-    if tp is not None:
-        tb = sum(k["read_bytes_per_iter"] + k["write_bytes_per_iter"] for k in tp["kernels"].values())
-        iteration_traffic = {"bytes": round(tb), "algorithmic_bytes": 5.0 * eb * D,
-                             "ratio": round(tb / (5.0 * eb * D), 3),
-                             "source": f"profiles/{args.config}_{args.storage}_traffic.json "
-                                       "(all kernels of the profiled run, per iteration)"}
+    iteration_traffic = iteration_traffic_of(args, args.storage, D)
 
     judge = None
     if not args.no_target and not args.no_fp64:
@@ -308,7 +314,8 @@ def main():
                           "iters_to_target": its_t if reached else None,
                           "exploitability_pct_at_stop": round(e_t, 4),
                           "iterations_at_stop": its_t,
-                          "fp64_exploitability_pct": None if e64 is None else round(e64, 4)}
+                          "fp64_exploitability_pct": None if e64 is None else round(e64, 4),
+                          "iteration_traffic": iteration_traffic_of(args, st, D) if sd == "exact" else None}
             log(f"bench: {key}: {ms:.3f} ms/iter; " + (
                 f"{its_t} iterations to {e_t:.4f}% pot in {sec:.3f}s" if reached
                 else f"target not reached: {e_t:.4f}% pot after {its_t} iterations")
