@@ -723,11 +723,14 @@ class Engine {
     }
 
     void set_showdown(int mode) {
-        if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16 && mode != GTO_SHOWDOWN_FP8)
+        if (mode != GTO_SHOWDOWN_EXACT && mode != GTO_SHOWDOWN_BF16 && mode != GTO_SHOWDOWN_FP8 &&
+            mode != GTO_SHOWDOWN_SEQUENTIAL)
             throw std::runtime_error("unknown showdown mode");
+        if (mode == GTO_SHOWDOWN_SEQUENTIAL && rows_)
+            throw std::invalid_argument("the sequential sums are f64 sums of f32 reach rows: not with f16 rows");
         if (mode != GTO_SHOWDOWN_EXACT && rows_)
             throw std::invalid_argument("the matrix-core showdown forms read f32 rows: not with f16 rows");
-        if (mode != GTO_SHOWDOWN_EXACT) {
+        if (mode == GTO_SHOWDOWN_BF16 || mode == GTO_SHOWDOWN_FP8) {
             const bool fp8 = mode == GTO_SHOWDOWN_FP8;
             build_mma_tables();
             // assemble output rows in LDS when they fit
@@ -1189,11 +1192,14 @@ class Engine {
         if (lds_max > kLdsBudget)
             throw std::runtime_error("showdown tables exceed the LDS budget (too many hands per board)");
         for (const int rows : {0, kRows16Reach, kRows16Reach | kRows16Value})
-            HIP_OK(hipFuncSetAttribute((const void*)show_kernel(rows), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)std::max(show_lds_[0], show_lds_[1])));
-        for (const int rows : {0, kRows16Reach, kRows16Reach | kRows16Value})
-            HIP_OK(hipFuncSetAttribute((const void*)fold_kernel(rows), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)std::max(fold_lds_[0], fold_lds_[1])));
+            for (const bool seq : {false, true}) {
+                HIP_OK(hipFuncSetAttribute((const void*)show_kernel(rows, seq),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)std::max(show_lds_[0], show_lds_[1])));
+                HIP_OK(hipFuncSetAttribute((const void*)fold_kernel(rows, seq),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)std::max(fold_lds_[0], fold_lds_[1])));
+            }
         n_fold_all_ = (uint32_t)plan_.fold_all.size();
         if (n_fold_all_) fold_all_ = upload_vec(plan_.fold_all);
         n_show_chunks_mma_ = (uint32_t)(plan_.show_chunks_mma.size() / 4);
@@ -1509,21 +1515,24 @@ class Engine {
     }
 
     // rows: the f16 rows of the sweep (0, kRows16Reach, or both kinds)
-    template <int ROWS>
+    // fold and showdown f64 sums in the CPU engine's order (f32 rows only)
+    bool seq_sums() const { return show_mode_ == GTO_SHOWDOWN_SEQUENTIAL; }
+    template <int ROWS, bool SEQ = false>
     ShowFn show_kernel_of() const {
-        if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4, ROWS>;
-        return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg, ROWS> : k_up_show<3, kMaxCardList / 4, ROWS>;
+        if (std::max(plan_.nh[0], plan_.nh[1]) > 3u * kBlock) return k_up_show<kRowPer, kMaxCardList / 4, ROWS, SEQ>;
+        return max_card_list_ <= 4u * kShortSeg ? k_up_show<3, kShortSeg, ROWS, SEQ>
+                                                 : k_up_show<3, kMaxCardList / 4, ROWS, SEQ>;
     }
-    ShowFn show_kernel(int rows) const {
+    ShowFn show_kernel(int rows, bool seq = false) const {
         if (rows == kRows16Reach) return show_kernel_of<kRows16Reach>();
         if (rows) return show_kernel_of<kRows16Reach | kRows16Value>();
-        return show_kernel_of<0>();
+        return seq ? show_kernel_of<0, true>() : show_kernel_of<0>();
     }
     using FoldFn = void (*)(DevTree, DevState, const FoldRec*, uint32_t, int, int);
-    static FoldFn fold_kernel(int rows) {
+    static FoldFn fold_kernel(int rows, bool seq = false) {
         if (rows == kRows16Reach) return k_up_fold<kRows16Reach>;
         if (rows) return k_up_fold<kRows16Reach | kRows16Value>;
-        return k_up_fold<0>;
+        return seq ? k_up_fold<0, true> : k_up_fold<0>;
     }
 
     void launch_up(int p, int mode, int swap = 0) {
@@ -1534,8 +1543,10 @@ class Engine {
         // would bound those sums by their children's maxima, a different
         // scale than the child row's own maximum, so they keep the rows)
         const bool absorb = mode == 0 && sched_.absorb && !compressed_ && !pred_ && !locked_ && T_.xkids;
-        const FoldFn fold_fn = fold_kernel(rows);
-        const bool mma = n_show_chunks_mma_ && show_mode_ != GTO_SHOWDOWN_EXACT && (mode == 0 || mode == 3);
+        const FoldFn fold_fn = fold_kernel(rows, seq_sums());
+        const ShowFn show_fn = show_kernel(rows, seq_sums());
+        const bool mma = n_show_chunks_mma_ && (show_mode_ == GTO_SHOWDOWN_BF16 || show_mode_ == GTO_SHOWDOWN_FP8) &&
+                         (mode == 0 || mode == 3);
         if (mma)
             launch_b(PK_UP_SHOW, show_mode_ == GTO_SHOWDOWN_FP8 ? k_up_show_mma<true> : k_up_show_mma<false>,
                      wave_grid(n_show_chunks_mma_, 1), kMmaBlock, show_mma_lds_, T_, S_, (const ShowRec*)show_all_,
@@ -1566,7 +1577,7 @@ class Engine {
             lstream_ = stream_;
             if (!err) try {
                     if (show)
-                        launch(PK_UP_SHOW, show_kernel(rows), wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
+                        launch(PK_UP_SHOW, show_fn, wave_grid(n_show_chunks_[cap_], 1), show_lds_[p], T_, S_,
                                (const ShowRec*)show_all_, (const uint4*)show_chunks_[cap_], n_show_chunks_[cap_],
                                p, swap);
                 } catch (...) {
@@ -1587,7 +1598,7 @@ class Engine {
                     launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(D.n_fold)), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
                            D.n_fold, p, swap);
                 if (!mma && D.n_show_chunks[cap_])
-                    launch(PK_UP_SHOW, show_kernel(rows), wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
+                    launch(PK_UP_SHOW, show_fn, wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
                            (const ShowRec*)D.show, (const uint4*)D.show_chunks[cap_], D.n_show_chunks[cap_], p, swap);
             }
             if (D.n_chance)

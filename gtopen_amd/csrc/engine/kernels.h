@@ -72,7 +72,7 @@ struct DevTree {
 // strength-order reach and its prefix P, per-card prefixes, the value row)
 // are rebuilt per node.
 struct ShowLds {
-    uint32_t P, cpre, creg, wsum, rs, coff, nodes, O, rhs, bytes;
+    uint32_t P, cpre, creg, wsum, exr, rs, coff, nodes, O, rhs, bytes;
 };
 // per node of a run, staged once: value row, opponent reach slot, live
 // flag, win, lose, tie
@@ -90,7 +90,8 @@ __host__ __device__ inline ShowLds show_lds_layout(uint32_t mo, uint32_t nhp, ui
     L.cpre = L.P + 8 * (mo + 2);                // creg + 1 doubles
     L.creg = creg;
     L.wsum = L.cpre + 8 * (creg + 1);           // per wave: scan total (double), row max (float)
-    L.rs = L.wsum + 48;                         // mo + 1 floats: reach in strength order, then 0
+    L.exr = L.wsum + 48;                        // per wave: a float (sequential sums' exactness test)
+    L.rs = L.exr + 16;                          // mo + 1 floats: reach in strength order, then 0
     L.coff = L.rs + 4 * (mo + 1);               // 53 u32
     L.nodes = L.coff + 53 * 4;                  // up to 16 ShowNode
     L.O = L.nodes + 16 * sizeof(ShowNode);      // nhp floats: the value row in hand order

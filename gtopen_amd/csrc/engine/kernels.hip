@@ -898,14 +898,44 @@ __global__ __launch_bounds__(kBlock) void k_down_level(DevTree T, DevState S, co
     else down_edge<H16>(T, S, edges[m - k0], mask, lane);
 }
 
+// Exactness test for the f64 sums of the fold and showdown terminals. The
+// nonzero entries of an opponent reach row are all multiples of u, the
+// smallest of their lowest set bits; so is every sum the terminals form
+// from them (totals, prefixes, per-card sums, and the differences of
+// those), whose magnitude is at most S, the row's total (entries are >= 0).
+// When S < 2^53 u each of these is exact in f64 in any order of addition,
+// and the GPU's chunked sums equal the CPU engine's sequential ones bit for
+// bit. S is bounded by the f64 total as computed (its relative error is far
+// below 2^-40). lsb_exp: the exponent of x's lowest set bit (x > 0,
+// finite).
+__device__ __forceinline__ int lsb_exp(float x) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, x);
+    const uint32_t e = (b >> 23) & 0xFFu, m = e ? (b & 0x7FFFFFu) | 0x800000u : b & 0x7FFFFFu;
+    return (int)(e ? e : 1u) - 150 + __builtin_ctz(m);
+}
+__device__ __forceinline__ bool exact_in_any_order(double total, int lsb_min) {
+    return total * (1.0 + 0x1p-40) < ldexp(1.0, 53 + lsb_min);
+}
+
+// t + x(lane 0) + x(lane 1) + ... + x(lane m - 1), one addition at a time
+// (the same in every lane)
+__device__ __forceinline__ double add_lanes_in_order(double t, float x, uint32_t m) {
+    const int xi = __builtin_bit_cast(int, x);
+#pragma unroll 8
+    for (uint32_t l = 0; l < m; ++l) t += (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(xi, (int)l));
+    return t;
+}
+
 // ---------------------------------------------------------------------------
 // Up sweep, fold terminals (game.rs:470-496), all levels in one launch:
 // value = amount * (opponent reach compatible with the hand), with f64
-// totals. One wavefront takes kFoldNodes consecutive nodes. The opponent's
-// row is scattered into a symmetric 52 x 52 card matrix in LDS, M[a][b] =
-// M[b][a] = reach of the hand {a, b} (0 where the opponent has no such
-// hand, and on the diagonal; those zeros are written once per wavefront,
-// since every node writes the same hand set). Card c's sum is row c read
+// totals (a wavefront sum; under SEQ the CPU's sum in hand order for a row
+// that fails the exactness test above). One wavefront takes kFoldNodes
+// consecutive nodes. The opponent's row is scattered into a symmetric
+// 52 x 52 card matrix in LDS, M[a][b] = M[b][a] = reach of the hand {a, b}
+// (0 where the opponent has no such hand, and on the diagonal; those zeros
+// are written once per wavefront, since every node writes the same hand
+// set). Card c's sum is row c read
 // in column order, i.e. its hands in ascending order of the other card,
 // which is hand (combo) order: the order the CPU accumulates them in
 // (game.rs:478-485). Rows are 52 floats, so lane c's 16-byte reads of row
@@ -933,10 +963,11 @@ struct ShowAcc<true> {
 
 __device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return hi * (hi - 1) / 2 + lo; }
 
-template <int ROWS = 0>
+template <int ROWS = 0, bool SEQ = false>
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
                                                     uint32_t count, int p, int swap) {
     constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
+    static_assert(!(SEQ && R16), "sequential totals are f64 sums of f32 reach rows");
     // totals and card sums in f64 (the CPU's, game.rs:477-484), f32 over
     // f16 reach rows (as the showdown's prefix sums)
     using Acc = typename ShowAcc<R16>::T;
@@ -985,6 +1016,7 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
         const FoldRec r = recs[n];
         const bool live = S.alive[o][pick(r.own, o)] != 0;
         Acc part = 0;
+        uint32_t emin = 255;  // SEQ: smallest exponent field of the row's nonzero entries (subnormals: 1)
         if (live) {
             wave_sync();  // the previous node's reads of M and s_card are done
 #pragma unroll
@@ -994,6 +1026,10 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
                     M[mo_pos[u] & 0xFFFFu] = x;
                     M[mo_pos[u] >> 16] = x;
                     part += (Acc)x;
+                    if constexpr (SEQ) {
+                        const uint32_t b = __builtin_bit_cast(uint32_t, x);
+                        emin = min(emin, b ? max((b >> 23) & 0xFFu, 1u) : 255u);
+                    }
                 }
         }
         // the next node's row, in flight during the rest of this node
@@ -1011,8 +1047,24 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
             if (V16 && lane == 0) S.vsc[r.cfv] = 1.0f;
             continue;
         }
-        const Acc tot = wave_sum(part);
+        Acc tot = wave_sum(part);
         wave_sync();
+        if constexpr (SEQ) {
+            // the CPU engine's total (hand order, one addition at a time)
+            // where the wavefront's sum could round differently; every
+            // entry's lowest set bit is at least 2^(exponent field - 150)
+            const int lmin = (int)-wave_max_f32(-(float)emin) - 150;
+            if (!exact_in_any_order(tot, lmin)) {
+                double t = 0.0;
+#pragma unroll
+                for (int u = 0; u < kMaxH; ++u) {
+                    if (u * kWave >= (int)nho) break;
+                    const float x = lane + u * kWave < nho ? M[mo_pos[u] & 0xFFFFu] : 0.0f;
+                    t = add_lanes_in_order(t, x, min(nho - u * kWave, (uint32_t)kWave));
+                }
+                tot = t;
+            }
+        }
         if (lane < 52) {
             const float4* row = (const float4*)(M + lane * 52);
             Acc acc = 0;
@@ -1063,6 +1115,157 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     }
 }
 
+// One quad lane's turn in a card list's sequential sum (below): lane R of
+// each quad adds its kSeg entries to the running sum y and stores the
+// prefixes, then y is passed to the whole quad.
+template <int R, bool DESC, int kSeg>
+__device__ __forceinline__ double card_list_turn(double y, uint32_t seg, const float (&x)[kSeg],
+                                                 const uint32_t (&cw)[kSeg], double* cpre) {
+    if (seg == (uint32_t)R) {
+        if constexpr (!DESC) {
+#pragma unroll
+            for (int u = 0; u < kSeg; ++u) {
+                cpre[cw[u] >> 16] = y;
+                y += (double)x[u];
+            }
+        } else {
+#pragma unroll
+            for (int u = kSeg - 1; u >= 0; --u) {
+                y += (double)x[u];
+                cpre[cw[u] >> 16] = y;
+            }
+        }
+    }
+    return dpp_zero<R * 0x55>(y);  // quad_perm: lane R into all four
+}
+
+// The showdown sums in the CPU engine's own order (game.rs:513-574), for a
+// node whose reach row fails the exactness test above. Ascending (DESC
+// false): P[k] = rs[0] + rs[1] + ... + rs[k-1] added from the bottom, P[mo]
+// the total, and per card c of the board the same over the reach of the
+// opponent hands holding it (cpre[coff[c] + c + k], the list's total at k =
+// its length); descending (DESC true): P[k] = rs[mo-1] + ... + rs[k] added
+// from the top, per card likewise (P[mo] and the list totals are left as
+// they are: the positions past each end read as 0). Each card list's four
+// quad lanes take turns over their quarters (segments as in the showdown's
+// card prefixes: cw, ctot = the list total's slot); thread kChainThread,
+// past the card lanes, adds up P. The caller's barrier publishes the
+// sums.
+constexpr uint32_t kChainThread = 4 * 52;  // the first thread without a card list
+template <bool DESC, int kSeg>
+__device__ __forceinline__ void cpu_order_sums(uint32_t wave, uint32_t lane, uint32_t card, uint32_t seg,
+                                               uint32_t mo, const float* rs, double* P, double* cpre,
+                                               const uint32_t (&cw)[kSeg], uint32_t ctot) {
+    if (card < 52) {
+        float x[kSeg];
+#pragma unroll
+        for (int u = 0; u < kSeg; ++u) x[u] = rs[cw[u] & 0xFFFFu];  // past the list: the 0 in rs[mo]
+        double y = 0.0;
+        if constexpr (!DESC) {
+            y = card_list_turn<0, false>(y, seg, x, cw, cpre);
+            y = card_list_turn<1, false>(y, seg, x, cw, cpre);
+            y = card_list_turn<2, false>(y, seg, x, cw, cpre);
+            y = card_list_turn<3, false>(y, seg, x, cw, cpre);
+            if (seg == 3) cpre[ctot] = y;
+        } else {
+            y = card_list_turn<3, true>(y, seg, x, cw, cpre);
+            y = card_list_turn<2, true>(y, seg, x, cw, cpre);
+            y = card_list_turn<1, true>(y, seg, x, cw, cpre);
+            (void)card_list_turn<0, true>(y, seg, x, cw, cpre);
+        }
+    }
+    if (wave * kWave + lane == kChainThread) {
+        // one lane (of the wavefront with the fewest card lists) runs P's
+        // chain through LDS, reading kB entries ahead of the additions
+        constexpr uint32_t kB = 16;
+        double t = 0.0;
+        if constexpr (!DESC) {
+            uint32_t k = 0;
+            for (; k + kB <= mo; k += kB) {
+                float x[kB];
+#pragma unroll
+                for (uint32_t i = 0; i < kB; ++i) x[i] = rs[k + i];
+#pragma unroll
+                for (uint32_t i = 0; i < kB; ++i) {
+                    P[k + i] = t;
+                    t += (double)x[i];
+                }
+            }
+            for (; k < mo; ++k) {
+                P[k] = t;
+                t += (double)rs[k];
+            }
+            P[mo] = t;
+        } else {
+            uint32_t k = mo;
+            for (; k >= kB; k -= kB) {
+                float x[kB];
+#pragma unroll
+                for (uint32_t i = 0; i < kB; ++i) x[i] = rs[k - 1 - i];
+#pragma unroll
+                for (uint32_t i = 0; i < kB; ++i) {
+                    t += (double)x[i];
+                    P[k - 1 - i] = t;
+                }
+            }
+            for (; k > 0; --k) {
+                t += (double)rs[k - 1];
+                P[k - 1] = t;
+            }
+        }
+    }
+}
+
+// Phases 2-4 of the showdown (k_up_show) in the CPU engine's order
+// (game.rs:513-574) for a reach row that fails the exactness test: the
+// ascending sums give each hand's L (staged in O), then the descending sums
+// its H, and V from the ascending totals, which the descending pass leaves
+// in place (P[mo], each card list's total). Returns the thread's max
+// |value|. All threads of the workgroup call it.
+template <int kR, int kSeg>
+__device__ __forceinline__ float show_in_cpu_order(uint32_t tid, uint32_t wave, uint32_t lane, uint32_t card,
+                                                   uint32_t seg, uint32_t mo, uint32_t mp, const float* rs, double* P,
+                                                   double* cpre, float* O, const uint32_t (&cw)[kSeg], uint32_t ctot,
+                                                   const uint4 (&ent)[kR], const uint32_t (&pidx)[kR], float win,
+                                                   float lose, float tie) {
+    cpu_order_sums<false>(wave, lane, card, seg, mo, rs, P, cpre, cw, ctot);
+    __syncthreads();
+    const double total = P[mo];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+        const uint32_t t = tid + u * kBlock;
+        if (t >= mp) break;
+        const uint4 e = ent[u];
+        const double* cp1 = cpre + (e.y & 0xFFFFu);
+        const double* cp2 = cpre + (e.y >> 16);
+        O[pidx[u]] = (float)(P[e.x & 0xFFFFu] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
+    }
+    __syncthreads();
+    cpu_order_sums<true>(wave, lane, card, seg, mo, rs, P, cpre, cw, ctot);
+    __syncthreads();
+    float vm = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+        const uint32_t t = tid + u * kBlock;
+        if (t >= mp) break;
+        const uint4 e = ent[u];
+        const uint32_t ub = e.x >> 16, n1 = e.w & 0xFFu, n2 = (e.w >> 8) & 0xFFu;
+        const uint32_t h1 = (e.z >> 16) & 0xFFu, h2 = e.z >> 24;
+        const double* cp1 = cpre + (e.y & 0xFFFFu);
+        const double* cp2 = cpre + (e.y >> 16);
+        const float lower = O[pidx[u]];
+        const float higher = (float)((ub < mo ? P[ub] : 0.0) - (h1 < n1 ? cp1[h1] : 0.0) - (h2 < n2 ? cp2[h2] : 0.0));
+        const uint32_t sp = e.w >> 16;
+        const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
+        const float valid = (float)(total - cp1[n1] - cp2[n2]) + same_r;
+        const float tied = valid - lower - higher;
+        const float val = win * lower + lose * higher + tie * tied;
+        O[pidx[u]] = val;
+        vm = fmaxf(vm, fabsf(val));
+    }
+    return vm;
+}
+
 // ---------------------------------------------------------------------------
 // Up sweep, showdown terminals (game.rs:501-577), all levels in one launch:
 // value = win * L + lose * H + tie * (V - L - H) with L/H/V the opponent
@@ -1084,17 +1287,22 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
 //      V = total - card totals + identical combo; the row is assembled in
 //      LDS in hand order and written out after the next node's first
 //      barrier.
+// A reach row that fails the exactness test (f64 sums only) takes the CPU's
+// order for phases 2-4 instead: its ascending sums give L and V, then its
+// descending ones H (cpu_order_sums).
 // ---------------------------------------------------------------------------
 constexpr int kMaxCardList = 48;  // opponent hands holding one card on a river board: <= 46
 constexpr int kShortSeg = 9;      // card-list entries per lane when no list exceeds 36 (partial ranges)
 // kR: hand rows per thread, kR * kBlock >= either player's hand count (also
 // the strength-order chunk per thread in phase 2); kSeg: entries per lane
 // of a card's list (four lanes per card), 4 kSeg >= the longest list
-template <int kR, int kSeg, int ROWS = 0>
+
+template <int kR, int kSeg, int ROWS = 0, bool SEQ = false>
 __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const ShowRec* __restrict__ recs,
                                                     const uint4* __restrict__ chunks, uint32_t n_chunks, int p,
                                                     int swap) {
     constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
+    static_assert(!(SEQ && R16), "sequential sums are f64 sums of f32 reach rows");
     using Acc = typename ShowAcc<R16>::T;
     extern __shared__ double lds[];
     const uint32_t c = xcd_slice(blockIdx.x, gridDim.x);
@@ -1116,6 +1324,7 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
     uint32_t* coff = (uint32_t*)(base + L.coff);
     ShowNode* nodes = (ShowNode*)(base + L.nodes);
     float* O = (float*)(base + L.O);
+    float* exr = (float*)(base + L.exr);        // per wave: smallest lowest set bit of the reach (SEQ)
 
     // the run's records and alive flags, then the board statics
     if (tid < cnt) {
@@ -1248,69 +1457,98 @@ __global__ __launch_bounds__(kBlock) void k_up_show(DevTree T, DevState S, const
         const Acc incl = wave_incl_sum(part);
         const Acc excl = dpp_zero<0x138>(incl);  // the previous lane's; 0 in lane 0
         if (lane == kWave - 1) wsum[wave] = incl;
+        if constexpr (SEQ) {
+            // the wavefront's smallest lowest set bit (negated: a max)
+            int nl = -1024;
+#pragma unroll
+            for (int u = 0; u < kR; ++u)
+                if (v[u] != 0.0f) nl = max(nl, -lsb_exp(v[u]));
+            const float wl = wave_max_f32((float)nl);
+            if (lane == 0) exr[wave] = wl;
+        }
         __syncthreads();
-        Acc before = 0, total = 0;
+        // SEQ: a row whose chunked f64 sums could round differently from the
+        // CPU engine's takes the CPU's order
+        bool seq = false;
+        if constexpr (SEQ) {
+            float l = -1024.0f;
+            Acc tot = 0;
 #pragma unroll
-        for (int w2 = 0; w2 < kWaves; ++w2) {
-            const Acc t = wsum[w2];
-            before += (uint32_t)w2 < wave ? t : 0.0;
-            total += t;
-        }
-        Acc run = before + excl;
-#pragma unroll
-        for (int u = 0; u < kR; ++u) {
-            const uint32_t k = lo + u;
-            P[k < hi ? k : mo + 1] = run;
-            run += (Acc)v[u];
-        }
-        if (tid == 0) P[mo] = total;
-        // 3. per-card exclusive prefixes in strength order
-        if (card < 52) {
-            float x[kSeg];
-#pragma unroll
-            for (int u = 0; u < (int)kSeg; ++u) x[u] = rs[cw[u] & 0xFFFFu];
-            Acc a = 0;
-#pragma unroll
-            for (int u = 0; u < (int)kSeg; ++u) a += (Acc)x[u];
-            // exclusive scan over the card's four segments (lanes 4*card..+3,
-            // one quad: quad_perm broadcasts each lane's sum to the quad)
-            const Acc a0 = dpp_zero<0x00>(a), a1 = dpp_zero<0x55>(a), a2 = dpp_zero<0xAA>(a), a3 = dpp_zero<0xFF>(a);
-            Acc y = 0;
-            if (seg > 0) y += a0;
-            if (seg > 1) y += a1;
-            if (seg > 2) y += a2;
-#pragma unroll
-            for (int u = 0; u < (int)kSeg; ++u) {
-                cpre[cw[u] >> 16] = y;
-                y += (Acc)x[u];
+            for (int w2 = 0; w2 < kWaves; ++w2) {
+                l = fmaxf(l, exr[w2]);
+                tot += wsum[w2];
             }
-            if (seg == 3) cpre[c0 + card + len] = ((a0 + a1) + a2) + a3;
+            seq = !exact_in_any_order(tot, -(int)l);
         }
-        __syncthreads();
-        // 4. per-hand values, the traverser's hands in strength order (the
-        //    rank bounds, hence the prefix reads, rise with the thread index),
-        //    staged in hand order in LDS
         const float win = nd.win, lose = nd.lose, tie = nd.tie;
         float vm = 0.0f;
+        if constexpr (SEQ) {
+            if (seq)
+                vm = show_in_cpu_order<kR, kSeg>(tid, wave, lane, card, seg, mo, mp, rs, (double*)P, (double*)cpre, O, cw,
+                                                 c0 + card + len, ent, pidx, win, lose, tie);
+        }
+        if (!seq) {
+            Acc before = 0, total = 0;
 #pragma unroll
-        for (int u = 0; u < kR; ++u) {
-            const uint32_t t = tid + u * kBlock;
-            if (t >= mp) break;
-            const uint4 e = ent[u];
-            const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
-            const Acc* cp1 = cpre + (e.y & 0xFFFFu);
-            const Acc* cp2 = cpre + (e.y >> 16);
-            const Acc S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
-            const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
-            const float higher =
-                (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
-            const uint32_t sp = e.w >> 16;
-            const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
-            const float valid = (float)(total - S1 - S2) + same_r;
-            const float tied = valid - lower - higher;
-            const float val = win * lower + lose * higher + tie * tied;
-            O[pidx[u]] = val;
-            vm = fmaxf(vm, fabsf(val));
+            for (int w2 = 0; w2 < kWaves; ++w2) {
+                const Acc t = wsum[w2];
+                before += (uint32_t)w2 < wave ? t : 0.0;
+                total += t;
+            }
+            Acc run = before + excl;
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const uint32_t k = lo + u;
+                P[k < hi ? k : mo + 1] = run;
+                run += (Acc)v[u];
+            }
+            if (tid == 0) P[mo] = total;
+            // 3. per-card exclusive prefixes in strength order
+            if (card < 52) {
+                float x[kSeg];
+#pragma unroll
+                for (int u = 0; u < (int)kSeg; ++u) x[u] = rs[cw[u] & 0xFFFFu];
+                Acc a = 0;
+#pragma unroll
+                for (int u = 0; u < (int)kSeg; ++u) a += (Acc)x[u];
+                // exclusive scan over the card's four segments (lanes 4*card..+3,
+                // one quad: quad_perm broadcasts each lane's sum to the quad)
+                const Acc a0 = dpp_zero<0x00>(a), a1 = dpp_zero<0x55>(a), a2 = dpp_zero<0xAA>(a), a3 = dpp_zero<0xFF>(a);
+                Acc y = 0;
+                if (seg > 0) y += a0;
+                if (seg > 1) y += a1;
+                if (seg > 2) y += a2;
+#pragma unroll
+                for (int u = 0; u < (int)kSeg; ++u) {
+                    cpre[cw[u] >> 16] = y;
+                    y += (Acc)x[u];
+                }
+                if (seg == 3) cpre[c0 + card + len] = ((a0 + a1) + a2) + a3;
+            }
+            __syncthreads();
+            // 4. per-hand values, the traverser's hands in strength order (the
+            //    rank bounds, hence the prefix reads, rise with the thread index),
+            //    staged in hand order in LDS
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const uint32_t t = tid + u * kBlock;
+                if (t >= mp) break;
+                const uint4 e = ent[u];
+                const uint32_t lb = e.x & 0xFFFFu, ub = e.x >> 16;
+                const Acc* cp1 = cpre + (e.y & 0xFFFFu);
+                const Acc* cp2 = cpre + (e.y >> 16);
+                const Acc S1 = cp1[e.w & 0xFFu], S2 = cp2[(e.w >> 8) & 0xFFu];
+                const float lower = (float)(P[lb] - cp1[e.z & 0xFFu] - cp2[(e.z >> 8) & 0xFFu]);
+                const float higher =
+                    (float)((total - P[ub]) - (S1 - cp1[(e.z >> 16) & 0xFFu]) - (S2 - cp2[e.z >> 24]));
+                const uint32_t sp = e.w >> 16;
+                const float same_r = sp != 0xFFFFu ? rs[sp] : 0.0f;
+                const float valid = (float)(total - S1 - S2) + same_r;
+                const float tied = valid - lower - higher;
+                const float val = win * lower + lose * higher + tie * tied;
+                O[pidx[u]] = val;
+                vm = fmaxf(vm, fabsf(val));
+            }
         }
         if (S.vmax) {
             vm = wave_max_f32(vm);

@@ -86,11 +86,16 @@ class GpuSolver(QueryMixin):
         return int(L.lib().gto_hip_river_entries(self._h))
 
     def set_showdown(self, mode):
-        """'exact' (f64 prefix sums, the CPU engine's arithmetic), 'bf16' or
-        'fp8' (same-board showdowns stacked into matrix-core products with
-        bf16 or e4m3 reach, f32 accumulation). The precision forms serve the
-        DCFR sweeps; exploitability and average-strategy values stay exact."""
-        L.check(L.lib().gto_hip_set_showdown(self._h, {"exact": 0, "bf16": 1, "fp8": 2}[mode]), "set showdown")
+        """'exact' (f64 prefix sums, the CPU engine's arithmetic up to the
+        association of f64 sums), 'sequential' ('exact' with the fold and
+        showdown f64 sums in the CPU engine's order wherever the order could
+        change a bit: every sweep, exploitability included, equals the CPU
+        engine's bit for bit; f32 rows only), 'bf16' or 'fp8' (same-board
+        showdowns stacked into matrix-core products with bf16 or e4m3 reach,
+        f32 accumulation). The precision forms serve the DCFR sweeps;
+        exploitability and average-strategy values stay exact."""
+        L.check(L.lib().gto_hip_set_showdown(self._h, {"exact": 0, "bf16": 1, "fp8": 2, "sequential": 3}[mode]),
+                "set showdown")
         self.showdown = mode
 
     def close(self):

@@ -290,7 +290,13 @@ int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint3
 
 /* Showdown evaluation. EXACT (default): strength-order prefix sums in f64,
  * the CPU engine's arithmetic (game.rs:501-577) up to the association of
- * f64 sums. BF16 / FP8: the showdown nodes of each river board stacked into
+ * f64 sums. SEQUENTIAL: EXACT, with the f64 sums of the fold and showdown
+ * terminals (game.rs:470-577) taken in the CPU engine's order wherever the
+ * order could change a bit: the fold totals always, the showdown sums of
+ * any reach row whose nonzero entries span more than 17 binary exponents
+ * (below that every f64 sum of the row is exact in any order). Iterations,
+ * values and exploitability then equal the CPU engine's bit for bit; f32
+ * rows only (not with GTO_STORAGE_HALF_ROWS). BF16 / FP8: the showdown nodes of each river board stacked into
  * matrix products on the matrix cores, value = win*L + lose*H + tie*Q with
  * [L;H;Q] = [W;G;T] x R (0/1 matrices of strictly weaker, strictly stronger
  * and tied card-disjoint opponent hands), R = opponent reach in bf16, or in
@@ -299,6 +305,7 @@ int gto_hip_node_values(gto_hip* h, int p, int mode, uint32_t count, const uint3
 #define GTO_SHOWDOWN_EXACT 0
 #define GTO_SHOWDOWN_BF16 1
 #define GTO_SHOWDOWN_FP8 2
+#define GTO_SHOWDOWN_SEQUENTIAL 3
 int gto_hip_set_showdown(gto_hip* h, int mode);
 
 /* Launch schedule of the up sweep. Every schedule computes the same values

@@ -1,7 +1,9 @@
 """GPU parity: the HIP engine against the CPU oracle and the reference's
 golden trajectory. Exploitability must agree within 1e-4 of the pot
 (f32 arithmetic on both sides; the GPU sums some f64 totals in a different
-order, so per-element arenas agree to float rounding, not bit for bit).
+order, so per-element arenas agree to float rounding, not bit for bit;
+test_gpu_sequential.py covers the mode that sums in the CPU's order and
+matches it bit for bit).
 """
 import numpy as np
 import pytest
