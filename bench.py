@@ -321,6 +321,24 @@ def main():
                 else f"target not reached: {e_t:.4f}% pot after {its_t} iterations")
                 + ("" if e64 is None else f"; FP64 judge {e64:.4f}% pot"))
 
+    # f32 with the fold and showdown f64 sums in the CPU engine's order
+    # (bit-identical to the CPU engine; the tests check every arena entry)
+    seq = None
+    if not args.no_target and not args.no_mixed and args.storage == "f32":
+        ms, _, _ = gpu_rate(spot, args.steps, args.warmup, "f32", "sequential")
+        sec, its_t, e_t, e64 = time_to_target(spot, "f32", args.target, 2000, "sequential", sched, judge=judge)
+        reached = e_t <= args.target
+        seq = {"showdown": "f64 sums in the CPU engine's order where the association could change a bit",
+               "ms_per_step": round(ms, 4), "iters_per_s": round(1e3 / ms, 3),
+               "sec_to_target": round(sec, 3) if reached else None,
+               "iters_to_target": its_t if reached else None,
+               "exploitability_pct_at_stop": round(e_t, 4),
+               "fp64_exploitability_pct": None if e64 is None else round(e64, 4)}
+        log(f"bench: sequential sums: {ms:.3f} ms/iter; " + (
+            f"{its_t} iterations to {e_t:.4f}% pot in {sec:.3f}s" if reached
+            else f"target not reached: {e_t:.4f}% pot after {its_t} iterations")
+            + ("" if e64 is None else f"; FP64 judge {e64:.4f}% pot"))
+
     # the other algorithms of the solver crate on the same spot (f32 arenas)
     algos = None
     if not args.no_target and not args.no_mixed and args.storage == "f32":
@@ -401,6 +419,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "sequential_sums": seq,
         "algorithms": algos,
         "schedule": sched,
         "configs": configs,
