@@ -207,8 +207,9 @@ def main():
     ap.add_argument("--sample-flops", type=int, default=16, help="flops in the batched-flops sample line")
     ap.add_argument("--autotune", type=int, default=2,
                     help="iterations per candidate launch schedule for the autotuner (0: default schedule)")
-    ap.add_argument("--showdown", default="exact", choices=["exact", "bf16", "fp8"],
-                    help="showdown evaluation: exact f64 prefix sums or the BF16 / FP8 matrix-core products")
+    ap.add_argument("--showdown", default="exact", choices=["exact", "sequential", "bf16", "fp8"],
+                    help="showdown evaluation: exact f64 prefix sums, the same with the fold and showdown sums in "
+                         "the CPU engine's order (sequential), or the BF16 / FP8 matrix-core products")
     ap.add_argument("--storage", default="f32", choices=["f32", "compressed", "half", "half_rows"])
     ap.add_argument("--river", default="auto", choices=["auto", "fused", "level"],
                     help="with --autotune 0: the fused river sweep or the level sweeps (auto: the engine default)")
@@ -403,7 +404,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": DTYPE[args.storage]
-                 + ("" if args.showdown == "exact" else f", {args.showdown} showdown products"),
+                 + {"exact": "", "sequential": ", f64 terminal sums in the CPU engine's order"}.get(
+                     args.showdown, f", {args.showdown} showdown products"),
         "data": "synthetic",
         "config": {"workload": f"{args.config} spot, seeded (gtopen_amd.workloads)", "nodes": spot.num_nodes,
                    "hands": list(spot.num_hands), "arena_entries_D": D, "river_arena_entries": Dr,
