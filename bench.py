@@ -378,12 +378,23 @@ def main():
                          f"C++ oracle (restated reference CPU engine) on {threads} threads",
                "cpu_model": info["cpu_model"], "physical_cores": info["physical_cores"],
                "logical_cpus": info["logical_cpus"]}
-        if iters_to_target:
+        if seq is not None and seq["iters_to_target"]:
+            # the sequential-sums run is bit-identical to the CPU engine, so
+            # its count is the CPU's own; only the time per iteration is
+            # extrapolated
+            its_cpu = seq["iters_to_target"]
+            cpu.update({"sec_to_target": round(its_cpu * cdt / n, 1), "iters_to_target": its_cpu,
+                        "extrapolated": True,
+                        "method": f"the CPU engine's iteration count to {args.target}% pot ({its_cpu}, checks every "
+                                  "10: the GPU's sequential-sums run, bit-identical to the CPU engine) times the "
+                                  f"oracle's mean seconds per iteration over its first {n}; later iterations prune "
+                                  "more and run faster, so this overstates the CPU time"})
+        elif iters_to_target:
             cpu.update({"sec_to_target": round(iters_to_target * cdt / n, 1), "extrapolated": True,
                         "method": f"the GPU's f32 iteration count to {args.target}% pot ({iters_to_target}, the "
-                                  "same arithmetic as the oracle's) times the oracle's mean seconds per iteration "
-                                  f"over its first {n}; later iterations prune more and run faster, so this "
-                                  "overstates the CPU time"})
+                                  "same arithmetic as the oracle's up to the order of some f64 sums) times the "
+                                  f"oracle's mean seconds per iteration over its first {n}; later iterations "
+                                  "prune more and run faster, so this overstates the CPU time"})
         log(f"bench: cpu oracle {n} iterations in {cdt:.2f}s on {threads} threads ({info['cpu_model']}, "
             f"{info['physical_cores']} physical cores)")
 

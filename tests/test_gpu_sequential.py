@@ -15,7 +15,10 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def _identical(gpu, cpu, what):
+def _identical(gpu, cpu, what, values=False):
+    """Arenas and exploitability equal; values: also the root rows of the
+    best response (mode 0) and of the average strategy (mode 1), f32 on
+    both sides."""
     rg, sg = gpu.download_arenas()
     rc, sc = cpu.arenas()
     for p in range(2):
@@ -25,6 +28,12 @@ def _identical(gpu, cpu, what):
     eg, ec = gpu.exploitability(), cpu.exploitability()
     _say(f"  {what}: arenas identical, exploitability gpu {eg!r} cpu {ec!r}")
     assert eg == ec
+    if values:
+        for p in range(2):
+            for mode in (0, 1):
+                vg, vc = gpu.root_cfv(p, mode), cpu.root_cfv(p, mode).astype(np.float32)
+                assert np.array_equal(vg, vc), f"{what}: root values of player {p}, mode {mode}"
+        _say(f"  {what}: root best-response and average-strategy rows identical")
 
 
 @pytest.mark.timeout(900)
@@ -45,7 +54,7 @@ def test_turn_full_length_bit_identical(built):
             gpu.iterate(k - gpu.iteration)
             cpu.iterate(k - cpu.iteration)
             if storage == "f32":
-                _identical(gpu, cpu, f"iter {k}")
+                _identical(gpu, cpu, f"iter {k}", values=k == marks[-1])
             else:
                 eg, ec = gpu.exploitability(), cpu.exploitability()
                 _say(f"  iter {k}: exploitability gpu {eg!r} cpu {ec!r}")
@@ -67,7 +76,7 @@ def test_flop_srp_bit_identical(built):
     for k in (2, 10, 20):
         gpu.iterate(k - gpu.iteration)
         cpu.iterate(k - cpu.iteration)
-        _identical(gpu, cpu, f"iter {k}")
+        _identical(gpu, cpu, f"iter {k}", values=k == 20)
     gpu.close()
 
 
@@ -91,7 +100,7 @@ def test_algorithms_rake_and_reference_spot_bit_identical(built):
         cpu = oracle.Oracle(spot, algo=algo)
         gpu.iterate(n)
         cpu.iterate(n)
-        _identical(gpu, cpu, f"iter {n}")
+        _identical(gpu, cpu, f"iter {n}", values=True)
         gpu.close()
 
 
