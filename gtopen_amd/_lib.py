@@ -25,6 +25,7 @@ STORAGE_F32, STORAGE_COMPRESSED, STORAGE_HALF, STORAGE_HALF_ROWS = 0, 1, 2, 3
 STORAGE = {"f32": STORAGE_F32, "compressed": STORAGE_COMPRESSED, "half": STORAGE_HALF,
            "half_rows": STORAGE_HALF_ROWS}
 ALGO_DCFR, ALGO_CFRPLUS, ALGO_PCFRPLUS = 0, 1, 2
+SHOWDOWN = {"exact": 0, "bf16": 1, "fp8": 2, "sequential": 3}  # GTO_SHOWDOWN_*
 
 
 def algo_code(name):

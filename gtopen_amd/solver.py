@@ -94,8 +94,7 @@ class GpuSolver(QueryMixin):
         showdowns stacked into matrix-core products with bf16 or e4m3 reach,
         f32 accumulation). The precision forms serve the DCFR sweeps;
         exploitability and average-strategy values stay exact."""
-        L.check(L.lib().gto_hip_set_showdown(self._h, {"exact": 0, "bf16": 1, "fp8": 2, "sequential": 3}[mode]),
-                "set showdown")
+        L.check(L.lib().gto_hip_set_showdown(self._h, L.SHOWDOWN[mode]), "set showdown")
         self.showdown = mode
 
     def close(self):

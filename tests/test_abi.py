@@ -25,6 +25,21 @@ def test_library_exports_every_declared_symbol(built):
     assert set(names) <= bound, set(names) - bound
 
 
+def test_python_constants_match_the_header():
+    """The storage, algorithm and showdown codes the Python layer passes
+    are the header's #defines."""
+    from gtopen_amd import _lib
+
+    text = open(os.path.join(ROOT, "include", "gtopen_amd.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define\s+GTO_(\w+)\s+(-?\d+)\b", text)}
+    for name, code in _lib.STORAGE.items():
+        assert defs["STORAGE_" + name.upper()] == code, name
+    assert (defs["ALGO_DCFR"], defs["ALGO_CFRPLUS"], defs["ALGO_PCFRPLUS"]) == (
+        _lib.ALGO_DCFR, _lib.ALGO_CFRPLUS, _lib.ALGO_PCFRPLUS)
+    for name, code in _lib.SHOWDOWN.items():
+        assert defs["SHOWDOWN_" + name.upper()] == code, name
+
+
 def test_engine_reports_device_errors_without_crashing(built):
     """Without a GPU the engine's create fails with an error string, never a
     silent CPU path."""
