@@ -511,19 +511,29 @@ def game_value_rate():
     ws = [[np.clip(w[q] + rng.normal(0.0, 0.02, size=w[q].shape), 0.0, 1.0).astype(np.float32) for q in range(2)]
           for _ in range(64)]
     res = {}
-    for key, gs in (("ranges", False), ("ranges_and_strategies", True)):
-        gv.value_and_grad(0, grad_strategy=gs)
+    for key, gs, dev in (("ranges", False, False), ("ranges_and_strategies", True, False),
+                         ("ranges_and_strategies_on_gpu", True, True)):
+        run = gv.value_and_grad_device if dev else gv.value_and_grad
+        run(0, grad_strategy=gs)
         t0 = time.perf_counter()
         for wk in ws:
             gv.set_weights(*wk)
-            gv.value_and_grad(0, grad_strategy=gs)
+            run(0, grad_strategy=gs)
         dt = time.perf_counter() - t0
         res[key] = {"evals_per_s": round(64 / dt, 2), "ms_per_eval": round(dt * 1e3 / 64, 3)}
+    n_sigma = spot.data_size[0] + spot.data_size[1]
+    # per evaluation the strategy arenas are read (down pass) and the strategy
+    # gradient written: 8 bytes per entry of both players' arenas
+    gbs = 8.0 * n_sigma * res["ranges_and_strategies_on_gpu"]["evals_per_s"] / 1e9
+    res["ranges_and_strategies_on_gpu"]["arena_GB_per_s"] = round(gbs, 1)
     res["workload"] = (f"turn spot ({spot.num_nodes} nodes), 500 DCFR iterations, 64 range perturbations; "
-                       "gradients copied to host arrays (strategy gradient: "
-                       f"{spot.data_size[0] + spot.data_size[1]} floats)")
+                       f"strategy gradient {n_sigma} floats, copied to host arrays, or left on the GPU as torch "
+                       "tensors (_on_gpu; arena_GB_per_s: 8 bytes per strategy entry per evaluation, sigma read "
+                       "and gradient written)")
     log(f"bench: game value + gradient: {res['ranges']['evals_per_s']} evals/s (ranges), "
-        f"{res['ranges_and_strategies']['evals_per_s']} evals/s (ranges and strategies)")
+        f"{res['ranges_and_strategies']['evals_per_s']} evals/s (ranges and strategies), "
+        f"{res['ranges_and_strategies_on_gpu']['evals_per_s']} evals/s with the gradients left on the GPU "
+        f"({gbs:.1f} GB/s of strategy arena bytes)")
     return res
 
 

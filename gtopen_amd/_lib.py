@@ -171,6 +171,10 @@ SIGNATURES = [
     ("gto_hip_game_value", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f64p, c_f32p, c_f32p, c_f32p, c_f32p]),
     ("gto_hip_game_value_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f64p, c_f32p, c_f32p, c_f32p,
                                                   c_f32p]),
+    # device addresses (e.g. torch.Tensor.data_ptr() of tensors on the engine's GPU)
+    ("gto_hip_game_value_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_f64p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("gto_hip_upload_strategy_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("gto_hip_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("gto_hip_profile_kind_name", ctypes.c_char_p, [ctypes.c_int]),
     ("gto_hip_profile_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(HipProfile)]),

@@ -57,6 +57,18 @@ constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 float* g_scratch_ptr[kMaxDevices] = {};
 
+// A caller's buffer that must be device memory of `device` (the
+// device-resident game value): anything else is an invalid argument. A
+// failed attribute query leaves its error behind, so it is cleared here
+// rather than reported by a later launch check.
+void require_device_ptr(const void* ptr, int device, const char* what) {
+    hipPointerAttribute_t a{};
+    const hipError_t r = hipPointerGetAttributes(&a, ptr);
+    if (r != hipSuccess) (void)hipGetLastError();
+    if (r != hipSuccess || (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) || a.device != device)
+        throw std::invalid_argument(std::string(what) + ": not device memory of the engine's GPU");
+}
+
 // This device's address of `scratch` (the device must be current); looked
 // up once per device, at the first engine created on it.
 float* scratch_on(int device) {
@@ -430,8 +442,17 @@ class Engine {
     // device scratch (at most kScratchFloats entries per player, cleared by a
     // fill kernel) instead of a buffer the engine allocates on first use; no
     // device or host allocation happens in the call.
-    void game_value(int p, double* U, float* gw[2], float* gs[2], bool noalloc = false) {
+    //
+    // dev: gw and gs are device memory of this GPU (checked); the root rows
+    // are copied there and the strategy gradient is written in place, so
+    // nothing but the value crosses to the host.
+    void game_value(int p, double* U, float* gw[2], float* gs[2], bool noalloc = false, bool dev = false) {
         if (compressed_) throw std::runtime_error("game values need an engine with f32 arenas");
+        if (dev)
+            for (int q = 0; q < 2; ++q) {
+                if (gw[q]) require_device_ptr(gw[q], opts_.device, "range gradient");
+                if (gs[q] && data_size_[q]) require_device_ptr(gs[q], opts_.device, "strategy gradient");
+            }
         // the strategy is an input per node; under suit isomorphism copied
         // branches would silently take their representatives' strategies
         if (iso_) throw std::runtime_error("game values need an engine created with isomorphism off");
@@ -451,14 +472,22 @@ class Engine {
             const int t = pass == 0 ? p : o;  // traverser of this sweep
             if (pass == 1 && !gw[o] && !gs[o]) break;
             launch_up(t, 4, pass);
-            HIP_OK(hipMemcpyAsync(c.data(), S_.cfv + (size_t)plan_.cfv_slot[0] * plan_.nh_max,
-                                  sizeof(float) * plan_.nh[t], hipMemcpyDeviceToHost, stream_));
-            HIP_OK(hipStreamSynchronize(stream_));
+            const float* root = S_.cfv + (size_t)plan_.cfv_slot[0] * plan_.nh_max;
+            if (dev && gw[t])
+                HIP_OK(hipMemcpyAsync(gw[t], root, sizeof(float) * plan_.nh[t], hipMemcpyDeviceToDevice, stream_));
+            if (!dev || (pass == 0 && U)) {
+                HIP_OK(hipMemcpyAsync(c.data(), root, sizeof(float) * plan_.nh[t], hipMemcpyDeviceToHost, stream_));
+                HIP_OK(hipStreamSynchronize(stream_));
+            }
             if (pass == 0 && U) *U = dot(p, c.data(), plan_.nh[p]);
-            if (gw[t]) std::copy(c.begin(), c.begin() + plan_.nh[t], gw[t]);
+            if (!dev && gw[t]) std::copy(c.begin(), c.begin() + plan_.nh[t], gw[t]);
             if (gs[t] && data_size_[t]) {
                 float* buf;
-                if (noalloc) {
+                if (dev) {
+                    buf = gs[t];
+                    const uint32_t blocks = (uint32_t)std::min<uint64_t>((data_size_[t] + kBlock - 1) / kBlock, 8192);
+                    launch(PK_OTHER, k_fill, blocks, 0, buf, (uint64_t)data_size_[t], 0.0f);
+                } else if (noalloc) {
                     buf = scratch_;
                     const uint32_t blocks = (uint32_t)std::min<uint64_t>((data_size_[t] + kBlock - 1) / kBlock, 8192);
                     launch(PK_OTHER, k_fill, blocks, 0, buf, (uint64_t)data_size_[t], 0.0f);
@@ -474,10 +503,26 @@ class Engine {
                     launch(PK_OTHER, k_grad_sigma, wave_grid(D.n_act[t]), 0, T_, S_, (const ActRec*)D.act[t],
                            D.n_act[t], t, buf);
                 }
-                HIP_OK(hipMemcpyAsync(gs[t], buf, sizeof(float) * data_size_[t], hipMemcpyDeviceToHost, stream_));
-                HIP_OK(hipStreamSynchronize(stream_));
+                if (!dev)
+                    HIP_OK(hipMemcpyAsync(gs[t], buf, sizeof(float) * data_size_[t], hipMemcpyDeviceToHost, stream_));
             }
         }
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+
+    // the strategy arenas (the game value's input) from device memory of
+    // this GPU, f32 arenas only
+    void upload_strategy_device(const float* s0, const float* s1) {
+        if (compressed_) throw std::invalid_argument("device strategy uploads need an engine with f32 arenas");
+        const float* src[2] = {s0, s1};
+        for (int q = 0; q < 2; ++q)
+            if (src[q] && data_size_[q]) require_device_ptr(src[q], opts_.device, "strategy");
+        reach_valid_[0] = reach_valid_[1] = false;
+        for (int q = 0; q < 2; ++q)
+            if (src[q] && data_size_[q])
+                HIP_OK(hipMemcpyAsync(S_.S[q], src[q], sizeof(float) * data_size_[q], hipMemcpyDeviceToDevice,
+                                      stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
     }
 
     void download(int which, int p, float* dst) {
@@ -1987,6 +2032,27 @@ int gto_hip_game_value_noalloc(gto_hip* h, int player, double* value, float* gra
         float* gs[2] = {grad_s0, grad_s1};
         h->e->game_value(player, value, gw, gs, true);
     });
+}
+
+int gto_hip_game_value_device(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
+                              float* grad_s0, float* grad_s1) {
+    if (!h || player < 0 || player > 1) {
+        set_last_error("gto_hip_game_value_device: null handle or bad player");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] {
+        float* gw[2] = {grad_w0, grad_w1};
+        float* gs[2] = {grad_s0, grad_s1};
+        h->e->game_value(player, value, gw, gs, false, true);
+    });
+}
+
+int gto_hip_upload_strategy_device(gto_hip* h, const float* s0, const float* s1) {
+    if (!h) {
+        set_last_error("gto_hip_upload_strategy_device: null handle");
+        return GTO_ERR_INVALID;
+    }
+    return guarded([&] { h->e->upload_strategy_device(s0, s1); });
 }
 
 int gto_hip_download_arenas(gto_hip* h, float* r0, float* r1, float* s0, float* s1) {

@@ -108,8 +108,58 @@ class GameValue:
         )
         return val.value, gw, gs
 
+    def set_strategy_device(self, sigma0=None, sigma1=None):
+        """set_strategy from float32 torch tensors on the engine's GPU
+        (contiguous, data_size[q] entries), copied on the device."""
+        ptr = []
+        for q, t in enumerate((sigma0, sigma1)):
+            if t is not None:
+                _check_device_tensor(t, (self.spot.data_size[q],), f"strategy of player {q}")
+            ptr.append(None if t is None else ctypes.c_void_p(t.data_ptr()))
+        _sync_producer()
+        L.check(L.lib().gto_hip_upload_strategy_device(self.engine._h, ptr[0], ptr[1]), "upload strategy (device)")
+
+    def value_and_grad_device(self, player, grad_weights=True, grad_strategy=True):
+        """value_and_grad with the gradients left on the GPU: (U, [dU/dw0,
+        dU/dw1], [dU/dsigma0, dU/dsigma1]) as float32 torch tensors on the
+        engine's device (gto_hip_game_value_device); skipped parts are None."""
+        import torch
+
+        sp = self.spot
+        dev = torch.device("cuda", self.engine.device)
+        gw = [torch.empty(sp.num_hands[q], dtype=torch.float32, device=dev) if grad_weights else None
+              for q in range(2)]
+        gs = [torch.empty(sp.data_size[q], dtype=torch.float32, device=dev) if grad_strategy else None
+              for q in range(2)]
+        val = ctypes.c_double()
+
+        def p(t):
+            return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+        _sync_producer()
+        L.check(L.lib().gto_hip_game_value_device(self.engine._h, player, ctypes.byref(val), p(gw[0]), p(gw[1]),
+                                                  p(gs[0]), p(gs[1])), "game value (device)")
+        return val.value, gw, gs
+
     def value(self, player):
         return self.value_and_grad(player, grad_weights=False, grad_strategy=False)[0]
+
+
+def _check_device_tensor(t, shape, what):
+    import torch
+
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise ValueError(f"{what}: needs a contiguous float32 tensor on the GPU")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what}: needs shape {tuple(shape)}, got {tuple(t.shape)}")
+
+
+def _sync_producer():
+    """The engine runs on its own stream: work queued on torch's streams (the
+    tensors' producers) completes first."""
+    import torch
+
+    torch.cuda.synchronize()
 
 
 def _autograd():
@@ -127,14 +177,25 @@ def _autograd():
         @staticmethod
         def forward(ctx, gv, player, w0, w1, sigma0, sigma1):
             dev = w0.device
+            # ranges are small and go through the host; strategies and
+            # gradients of GPU tensors stay on the GPU
             gv.set_weights(w0.detach().float().cpu().numpy(), w1.detach().float().cpu().numpy())
-            gv.set_strategy(sigma0.detach().float().cpu().numpy(), sigma1.detach().float().cpu().numpy())
+            on_gpu = sigma0.is_cuda and sigma1.is_cuda and sigma0.device == sigma1.device
+            if on_gpu:
+                gv.set_strategy_device(sigma0.detach().float().contiguous(), sigma1.detach().float().contiguous())
+            else:
+                gv.set_strategy(sigma0.detach().float().cpu().numpy(), sigma1.detach().float().cpu().numpy())
             need_w = ctx.needs_input_grad[2] or ctx.needs_input_grad[3]
             need_s = ctx.needs_input_grad[4] or ctx.needs_input_grad[5]
-            u, gw, gs = gv.value_and_grad(player, grad_weights=need_w, grad_strategy=need_s)
+            if on_gpu:
+                u, gw, gs = gv.value_and_grad_device(player, grad_weights=need_w, grad_strategy=need_s)
+            else:
+                u, gw, gs = gv.value_and_grad(player, grad_weights=need_w, grad_strategy=need_s)
+                gw = [None if g is None else torch.from_numpy(g) for g in gw]
+                gs = [None if g is None else torch.from_numpy(g) for g in gs]
             grads = []
             for g, like in ((gw[0], w0), (gw[1], w1), (gs[0], sigma0), (gs[1], sigma1)):
-                grads.append(None if g is None else torch.from_numpy(g).to(device=like.device, dtype=like.dtype))
+                grads.append(None if g is None else g.to(device=like.device, dtype=like.dtype))
             ctx.grads = grads
             return torch.tensor(u, dtype=torch.float64, device=dev)
 

@@ -39,6 +39,7 @@ class GpuSolver(QueryMixin):
         copies, as after the reference's ensure_symmetric."""
         self.spot = spot  # keeps the view's arrays alive
         self.isomorphism = bool(isomorphism)
+        self.device = int(device)
         opts = L.HipOptions(
             L.STORAGE[storage],
             L.algo_code(algo),

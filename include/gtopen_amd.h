@@ -392,6 +392,21 @@ int gto_hip_game_value(gto_hip* h, int player, double* value, float* grad_w0, fl
  * consecutive calls are independent (the array is cleared where needed). */
 int gto_hip_game_value_noalloc(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
                                float* grad_s0, float* grad_s1);
+/* The same value and gradients for callers that keep ranges, strategies
+ * and gradients on the GPU, such as an autograd graph of device tensors (no
+ * counterpart in the reference): grad_w0/grad_w1/grad_s0/grad_s1 are device
+ * memory of the engine's GPU (any may be NULL); the root rows are copied
+ * there and the strategy gradient is written in place, so only the value
+ * crosses to the host. A pointer that is not device memory of that GPU is
+ * GTO_ERR_INVALID. Returns when the gradients are written; the caller's
+ * earlier writes to device memory must be complete. */
+int gto_hip_game_value_device(gto_hip* h, int player, double* value, float* grad_w0, float* grad_w1,
+                              float* grad_s0, float* grad_s1);
+/* The strategy arenas (the game value's sigma) from device memory of the
+ * engine's GPU, as gto_hip_upload_arenas(h, NULL, NULL, s0, s1) does from
+ * host memory (f32 arenas; NULL keeps a player's; no counterpart in the
+ * reference). Not device memory of that GPU: GTO_ERR_INVALID. */
+int gto_hip_upload_strategy_device(gto_hip* h, const float* s0, const float* s1);
 
 /* Node locks: lock_off[num_nodes] (-1 = unlocked, else offset into sigma),
  * sigma = concatenated na*nh_actor blocks. Replaces update_locks /

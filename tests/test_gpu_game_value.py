@@ -208,3 +208,42 @@ def test_autograd_function(built):
     np.testing.assert_allclose(s1.grad.cpu().numpy(), 2.0 * gs[1], rtol=1e-6)
     g1326 = to_combos(spot, 0, gw[0])
     assert g1326.shape == (1326,) and np.count_nonzero(g1326) <= spot.num_hands[0]
+
+
+def test_device_resident_path_matches_host_path(built):
+    """gto_hip_game_value_device / gto_hip_upload_strategy_device: the same
+    value and gradients bit for bit as the host-pointer entries, left on the
+    GPU; host memory where device memory is required is GTO_ERR_INVALID."""
+    import ctypes
+
+    import torch
+
+    from gtopen_amd import _lib as L
+    from gtopen_amd import workloads
+
+    _say("case: river spot, device-resident game value vs host path")
+    spot, sigma, gv, _ = _solved(workloads.river_spot(), 40)
+    for p in range(2):
+        u_h, gw_h, gs_h = gv.value_and_grad(p)
+        u_d, gw_d, gs_d = gv.value_and_grad_device(p)
+        assert u_d == u_h
+        for a, b in zip(gw_h + gs_h, gw_d + gs_d):
+            assert b.is_cuda and np.array_equal(a, b.cpu().numpy())
+    # the strategy from device tensors: a changed sigma moves the value the
+    # same way through both uploads
+    s0 = sigma[0] * np.float32(0.5)
+    gv.set_strategy(s0, sigma[1])
+    u_h = gv.value(0)
+    gv.set_strategy(*sigma)
+    gv.set_strategy_device(torch.tensor(s0, device="cuda"), torch.tensor(sigma[1], device="cuda"))
+    u_d, _, _ = gv.value_and_grad_device(0, grad_weights=False, grad_strategy=False)
+    assert u_d == u_h
+    host = np.zeros(spot.num_hands[0], dtype=np.float32)
+    val = ctypes.c_double()
+    rc = L.lib().gto_hip_game_value_device(gv.engine._h, 0, ctypes.byref(val), ctypes.c_void_p(host.ctypes.data),
+                                           None, None, None)
+    assert rc == -1  # GTO_ERR_INVALID
+    rc = L.lib().gto_hip_upload_strategy_device(gv.engine._h, ctypes.c_void_p(s0.ctypes.data), None)
+    assert rc == -1
+    # the engine is still usable after the rejected calls
+    assert gv.value(0) == u_h
