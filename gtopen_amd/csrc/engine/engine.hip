@@ -1656,13 +1656,26 @@ class Engine {
             uint32_t n0 = D.n_act[0], n1 = D.n_act[1];
             if (absorb && p == 0) l1 = D.act_keep[1], n1 = D.n_act_keep[1];
             if (absorb && p == 1) l0 = D.act_keep[0], n0 = D.n_act_keep[0];
-            if (n0 + n1)
-                launch(PK_UP_ACTION, up_kernel(mode, rows), wave_grid(n0 + n1), 0, T_, S_, l0, n0, l1, n1, p,
-                       absorb ? 1 : 0);
+            if (n0 + n1) {
+                const uint32_t ns = up_slices(n0 + n1, rows);
+                launch(PK_UP_ACTION, up_kernel(mode, rows), wave_grid((n0 + n1) * ns), 0, T_, S_, l0, n0, l1, n1, p,
+                       absorb ? 1 : 0, ns);
+            }
         }
     }
 
-    using UpFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, const ActRec*, uint32_t, int, int);
+    using UpFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, const ActRec*, uint32_t, int, int, uint32_t);
+    // Wavefronts per node of an update launch over n nodes: a level with
+    // few nodes (the upper levels; every level of a small tree) splits each
+    // node's traverser hands over up to kMaxSlices wavefronts of at least
+    // 256 hands, for about kFillWaves wavefronts in all (f32 arenas and
+    // rows only: the other forms reduce over whole rows).
+    static constexpr uint32_t kFillWaves = 8192, kMaxSlices = 8;
+    uint32_t up_slices(uint32_t n, int rows) const {
+        if (rows || compressed_ || half_ || n >= kFillWaves) return 1;
+        const uint32_t by_hands = std::max<uint32_t>(1, std::max(plan_.nh[0], plan_.nh[1]) / 256);
+        return std::max<uint32_t>(1, std::min({kFillWaves / n, by_hands, kMaxSlices}));
+    }
     // the up-sweep action kernel for a sweep mode (0 DCFR update, 1 best
     // response, 2 average, 3 current, 4 given strategy) and the sweep's rows
     UpFn up_kernel(int mode, int rows) const {

@@ -2726,20 +2726,24 @@ __device__ __forceinline__ float absorbed_sources(const DevTree& T, const DevSta
 }
 
 template <int MODE, int STORE, int ROWS = 0>
-// Returns this lane's max |value| over the row entries it wrote.
+// Returns this lane's max |value| over the row entries it wrote. [hbeg,
+// hend): the traverser hands this wavefront takes (a slice of the row; the
+// whole row by default; slices start at multiples of 64).
 __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState& S, const ActRec& r, int p, int q,
-                                                uint32_t lane, int absorb = 0) {
+                                                uint32_t lane, int absorb = 0, uint32_t hbeg = 0,
+                                                uint32_t hend = 0xFFFFFFFFu) {
     constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
     static_assert(ROWS == 0 || (MODE == 0 && STORE == kStoreH), "f16 rows serve the DCFR sweeps of f16 arenas");
     float vm = 0.0f;
     const int o = 1 - p;
     const uint32_t nhp = T.nh[p];
+    const uint32_t h1 = min(hend, nhp);  // end of this wavefront's hands
     const uint32_t nh_max = T.nh_max;
     const int na = r.na;
     const uint32_t* kid_cfv = T.kids + r.kid;
     if (!S.alive[o][pick(r.own, o)]) {
         const RowOut<V16> z(S.cfv, (size_t)r.cfv * nh_max, RowEnc(0.0f));
-        for (uint32_t i = lane; i < nhp; i += kWave) z.set(i, 0.0f);
+        for (uint32_t i = hbeg + lane; i < h1; i += kWave) z.set(i, 0.0f);
         if (V16 && lane == 0) S.vsc[r.cfv] = 1.0f;
         return 0.0f;
     }
@@ -2810,7 +2814,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                 RowIn<V16> rw[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                for (uint32_t i0 = lane; i0 < nhp; i0 += kSumHands * kWave) {
+                for (uint32_t i0 = hbeg + lane; i0 < h1; i0 += kSumHands * kWave) {
                     float v[kSumHands][NA];
 #pragma unroll
                     for (int h = 0; h < kSumHands; ++h) {
@@ -2821,7 +2825,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
 #pragma unroll
                     for (int h = 0; h < kSumHands; ++h) {
                         const uint32_t i = i0 + h * kWave;
-                        if (i >= nhp) break;
+                        if (i >= h1) break;
                         float acc = 0.0f;
                         if (kind == 0) {
 #pragma unroll
@@ -2839,7 +2843,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     }
                 }
             } else {
-                for (uint32_t i = lane; i < nhp; i += kWave) {
+                for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                     float acc = kind == 2 ? row(0)[i] : 0.0f;
                     for (int a = 0; a < na; ++a) {
                         const float v = row(a)[i];
@@ -2866,7 +2870,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
         dispatch_na4(na, [&](auto NAc) {
             constexpr int NA = decltype(NAc)::value;
             if constexpr (NA > 0) {
-                for (uint32_t i = lane; i < nhp; i += kWave) {
+                for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                     HandSigma<NA, KIND> hs;
                     hs.load(arena, nhp, i);
                     float acc = 0.0f;
@@ -2876,7 +2880,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     vm = fmaxf(vm, fabsf(acc));
                 }
             } else {
-                for (uint32_t i = lane; i < nhp; i += kWave) {
+                for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                     SigmaHand<KIND> sh;
                     sh.init(arena, nhp, i, na);
                     float acc = 0.0f;
@@ -2913,14 +2917,14 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                             done = true;
                             const float* src[NA][kMaxAbsorb];
                             (void)absorbed_sources<NA>(T, S, r, o, src);
-                            for (uint32_t i0 = lane; i0 < nhp; i0 += kStepAbsorb * kWave) {
+                            for (uint32_t i0 = hbeg + lane; i0 < h1; i0 += kStepAbsorb * kWave) {
                                 HandSigma<NA, kCurrentF32> hs[kStepAbsorb];
                                 float x[kStepAbsorb][NA][kMaxAbsorb], sm[kStepAbsorb][NA], reach[kStepAbsorb];
                                 bool ok[kStepAbsorb];
 #pragma unroll
                                 for (int h = 0; h < kStepAbsorb; ++h) {
                                     const uint32_t i = i0 + h * kWave;
-                                    ok[h] = i < nhp;
+                                    ok[h] = i < h1;
                                     const uint32_t ii = ok[h] ? i : i0;
 #pragma unroll
                                     for (int a = 0; a < NA; ++a) {
@@ -2969,14 +2973,14 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     RowIn<V16> rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                    for (uint32_t i0 = lane; i0 < nhp; i0 += kStepHands * kWave) {
+                    for (uint32_t i0 = hbeg + lane; i0 < h1; i0 += kStepHands * kWave) {
                         HandSigma<NA, kCurrentF32> hs[kStepHands];
                         float v[kStepHands][NA], sm[kStepHands][NA], reach[kStepHands], rr[kStepHands][PRED ? NA : 1];
                         bool ok[kStepHands];
 #pragma unroll
                         for (int h = 0; h < kStepHands; ++h) {
                             const uint32_t i = i0 + h * kWave;
-                            ok[h] = i < nhp;
+                            ok[h] = i < h1;
                             const uint32_t ii = ok[h] ? i : i0;
 #pragma unroll
                             for (int a = 0; a < NA; ++a) {
@@ -3013,7 +3017,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                         }
                     }
                 } else {
-                    for (uint32_t i = lane; i < nhp; i += kWave) {
+                    for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                         SigmaHand<kCurrentF32> sh;
                         sh.init(PRED ? Qp : R, nhp, i, na);
                         float acc = 0.0f;
@@ -3143,14 +3147,14 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                     RowIn<V16> rw[NA];
 #pragma unroll
                     for (int a = 0; a < NA; ++a) rw[a] = row(a);
-                    for (uint32_t i0 = lane; i0 < nhp; i0 += kStepH * kWave) {
+                    for (uint32_t i0 = hbeg + lane; i0 < h1; i0 += kStepH * kWave) {
                         HandSigma<NA, kCurrentH> hs[kStepH];
                         float v[kStepH][NA], sm[kStepH][NA], reach[kStepH];
                         bool ok[kStepH];
 #pragma unroll
                         for (int h = 0; h < kStepH; ++h) {
                             const uint32_t i = i0 + h * kWave;
-                            ok[h] = i < nhp;
+                            ok[h] = i < h1;
                             const uint32_t ii = ok[h] ? i : i0;
                             hs[h].fetch(Rh, nhp, ii);
 #pragma unroll
@@ -3184,7 +3188,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                         }
                     }
                 } else {
-                    for (uint32_t i = lane; i < nhp; i += kWave) {
+                    for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                         SigmaHand<kCurrentH> sh;
                         sh.init(Rh, nhp, i, na);
                         float acc = 0.0f;
@@ -3268,7 +3272,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                         return acc;
                     }
                 };
-                for (uint32_t i = lane; i < nhp; i += kWave) {
+                for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                     const float acc = hand(i, [&](size_t, float nr, float ns) {
                         rmax = fmaxf(rmax, fabsf(nr));
                         smax = fmaxf(smax, ns);
@@ -3282,7 +3286,7 @@ __device__ __forceinline__ float up_action_node(const DevTree& T, const DevState
                 sok = SM > 0.0f && isfinite(SM);
                 rk = rok ? 32767.0f / RM : 0.0f;
                 sk = sok ? 65535.0f / SM : 0.0f;
-                for (uint32_t i = lane; i < nhp; i += kWave) {
+                for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                     hand(i, [&](size_t idx, float nr, float ns) {
                         Rq[idx] = rok ? quant_i16(nr * rk) : (int16_t)0;
                         Sq[idx] = sok ? quant_u16(fmaxf(ns, 0.0f) * sk) : (uint16_t)0;
@@ -3318,20 +3322,35 @@ __global__ __launch_bounds__(kBlock) void k_up_action(DevTree T, DevState S, con
 // else item m - k0(m) of player 1's, so every stretch of the grid (each
 // XCD's slice among them) holds both lists in proportion and the lighter
 // opponent-node sums fill in beside the traverser's updates.
+// slices > 1 (levels with few nodes, f32 arenas and rows): each node's
+// traverser hands split over `slices` wavefronts, 64-aligned, so a small
+// level still fills the GPU; every hand's arithmetic is unchanged.
 template <int MODE, int STORE, int ROWS = 0>
 __global__ __launch_bounds__(kBlock) void k_up_action2(DevTree T, DevState S, const ActRec* __restrict__ r0,
                                                        uint32_t c0, const ActRec* __restrict__ r1, uint32_t c1, int p,
-                                                       int absorb) {
-    const uint32_t m = wave_item();
+                                                       int absorb, uint32_t slices) {
+    // f16 and i16/u16 arenas keep per-node maxima and f16 rows per-row
+    // scales, reductions over the whole row: those take whole rows
+    constexpr bool kSliceable = ROWS == 0 && (STORE == kStoreF32 || STORE == kStoreF32P);
+    if constexpr (!kSliceable) slices = 1;
+    const uint32_t w = wave_item();
+    const uint32_t m = w / slices, sl = w - m * slices;
     const uint32_t tot = c0 + c1;
     if (m >= tot) return;
+    uint32_t hb = 0, he = 0xFFFFFFFFu;
+    if (slices > 1) {
+        const uint32_t ch = ((T.nh[p] + slices - 1) / slices + kWave - 1) & ~(uint32_t)(kWave - 1);
+        hb = sl * ch;
+        he = hb + ch;
+        if (hb >= T.nh[p]) return;
+    }
     const uint32_t k0 = (uint32_t)(((uint64_t)m * c0) / tot);
     const uint32_t k1 = (uint32_t)(((uint64_t)(m + 1) * c0) / tot);
     const int q = __builtin_amdgcn_readfirstlane(k1 > k0 ? 0 : 1);
     const uint32_t lane = threadIdx.x & 63;
     const ActRec r = q == 0 ? r0[k0] : r1[m - k0];
-    float vm = up_action_node<MODE, STORE, ROWS>(T, S, r, p, q, lane, absorb);
-    if (S.vmax) {
+    float vm = up_action_node<MODE, STORE, ROWS>(T, S, r, p, q, lane, absorb, hb, he);
+    if (S.vmax) {  // engines with row maxima launch whole rows (slices == 1)
         vm = wave_max_f32(vm);
         if (lane == 0) S.vmax[r.cfv] = vm;
     }
