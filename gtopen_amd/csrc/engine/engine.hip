@@ -91,6 +91,7 @@ const char* kProfNames[GTO_PROF_KINDS] = {"root", "down_action", "down_chance", 
 struct DevLevel {
     ActRec* act[2] = {nullptr, nullptr};
     uint32_t n_act[2] = {0, 0};
+    uint32_t max_na[2] = {0, 0};  // most actions of a node in act[q] (and act_up[q])
     // q's nodes that no node of the other player absorbs (Plan act_keep)
     ActRec* act_keep[2] = {nullptr, nullptr};
     uint32_t n_act_keep[2] = {0, 0};
@@ -1263,6 +1264,7 @@ class Engine {
             for (int q = 0; q < 2; ++q) {
                 D.n_act[q] = (uint32_t)L.act_rec[q].size();
                 if (D.n_act[q]) D.act[q] = upload_vec(L.act_rec[q]);
+                for (const ActRec& r : L.act_rec[q]) D.max_na[q] = std::max<uint32_t>(D.max_na[q], r.na);
                 D.n_act_keep[q] = (uint32_t)L.act_keep[q].size();
                 if (D.n_act_keep[q]) D.act_keep[q] = upload_vec(L.act_keep[q]);
             }
@@ -1460,6 +1462,13 @@ class Engine {
                 if (!((mask >> q) & 1) || !n) continue;
                 const ActRec* recs = upper ? D.act_up[q] : D.act[q];
                 const uint32_t e = edges_done ? 0u : ne;
+                if (!h16 && n < kNodeWgMax && D.max_na[q] <= (uint32_t)kMaxUnroll) {
+                    // few nodes: a workgroup per node, then the edges
+                    launch(PK_DOWN_ACTION, node_kernel(kind), wave_grid(n + (e + kWaves - 1) / kWaves, 1), 0, T_, S_,
+                           recs, n, q, er, e, mask);
+                    edges_done = true;
+                    continue;
+                }
                 launch(PK_DOWN_ACTION, down_kernel(kind, h16), wave_grid(n + e), 0, T_, S_, recs, n, q, er, e, mask);
                 edges_done = true;
             }
@@ -1470,6 +1479,16 @@ class Engine {
         }
     }
 
+    // levels with fewer action nodes than this (of one player) take a
+    // workgroup per node in the down sweep (k_down_node)
+    static constexpr uint32_t kNodeWgMax = 2048;
+    using NodeFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, int, const EdgeRec*, uint32_t, int);
+    NodeFn node_kernel(int kind) const {
+        if (kind == kRawF32) return k_down_node<kRawF32>;
+        if (kind == kAverageF32)
+            return half_ ? k_down_node<kAverageH> : compressed_ ? k_down_node<kAverageQ> : k_down_node<kAverageF32>;
+        return half_ ? k_down_node<kCurrentH> : compressed_ ? k_down_node<kCurrentQ> : k_down_node<kCurrentF32>;
+    }
     using DownFn = void (*)(DevTree, DevState, const ActRec*, uint32_t, int, const EdgeRec*, uint32_t, int);
     // the down-sweep level kernel for a strategy kind (current / average /
     // given) of this engine's arenas; h16: f16 reach rows (DCFR sweeps of

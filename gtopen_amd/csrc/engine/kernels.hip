@@ -716,10 +716,21 @@ __global__ void k_begin_iteration(DevState S) {
 // (cfr.rs:478-492, 509-515) and the child's "any reach > 0" flag, which is
 // the zero-reach pruning test of cfr.rs:424-430 for the subtree below.
 // ---------------------------------------------------------------------------
-template <int KIND, bool H16 = false>
+// PART (k_down_node): the wavefront takes hands [hbeg, hend) of the row and,
+// instead of writing each child's alive flag and reach maximum, leaves its
+// share in part (bit a of part->any: some reach of child a > 0 among its
+// hands; part->mx[a]: their maximum) for the workgroup to combine.
+struct DownPart {
+    uint64_t any;
+    float mx[kMaxUnroll];
+};
+template <int KIND, bool H16 = false, bool PART = false>
 __device__ __forceinline__ void down_action_node(const DevTree& T, const DevState& S, const ActRec& r, int q,
-                                                 uint32_t lane) {
+                                                 uint32_t lane, uint32_t hbeg = 0, uint32_t hend = 0xFFFFFFFFu,
+                                                 DownPart* part = nullptr) {
+    static_assert(!(PART && H16), "split rows: f32 reach rows only");
     const uint32_t nh = T.nh[q];
+    const uint32_t h1 = min(hend, nh);  // end of this wavefront's hands
     const int na = r.na;
     const uint32_t* kid_own = T.kids + r.kid + na;
     const uint32_t src = pick(r.own, q);
@@ -730,7 +741,12 @@ __device__ __forceinline__ void down_action_node(const DevTree& T, const DevStat
     if (!S.alive[q][src]) {
         for (int a = 0; a < na; ++a) {
             const RowOut<H16> dst = child(a);
-            for (uint32_t i = lane; i < nh; i += kWave) dst.set(i, 0.0f);
+            for (uint32_t i = hbeg + lane; i < h1; i += kWave) dst.set(i, 0.0f);
+        }
+        if (PART) {  // every wavefront of the node sees the same flag
+            if (lane == 0) part->any = 0;
+            if (lane < kMaxUnroll) part->mx[lane] = 0.0f;
+            return;
         }
         for (int a = lane; a < na; a += kWave) {
             S.alive[q][kid_own[a]] = 0;
@@ -748,7 +764,7 @@ __device__ __forceinline__ void down_action_node(const DevTree& T, const DevStat
     for (int a = 0; a < kMaxUnroll; ++a) mmax[a] = 0.0f;
     if (r.lock_off >= 0) {
         const float* sig = T.lock_sigma + r.lock_off;
-        for (uint32_t i = lane; i < nh; i += kWave) {
+        for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
             const float pv = pr[i];
             for (int a = 0; a < na; ++a) {
                 float v = pv * sig[(size_t)a * nh + i];
@@ -764,7 +780,7 @@ __device__ __forceinline__ void down_action_node(const DevTree& T, const DevStat
                 RowOut<H16> dst[NA];
 #pragma unroll
                 for (int a = 0; a < NA; ++a) dst[a] = child(a);
-                for (uint32_t i0 = lane; i0 < nh; i0 += kDownHands * kWave) {
+                for (uint32_t i0 = hbeg + lane; i0 < h1; i0 += kDownHands * kWave) {
                     HandSigma<NA, KIND> hs[kDownHands];
                     float pv[kDownHands];
 #pragma unroll
@@ -776,7 +792,7 @@ __device__ __forceinline__ void down_action_node(const DevTree& T, const DevStat
 #pragma unroll
                     for (int h = 0; h < kDownHands; ++h) {
                         const uint32_t i = i0 + h * kWave;
-                        if (i >= nh) break;
+                        if (i >= h1) break;
                         hs[h].finish();
 #pragma unroll
                         for (int a = 0; a < NA; ++a) {
@@ -788,7 +804,7 @@ __device__ __forceinline__ void down_action_node(const DevTree& T, const DevStat
                     }
                 }
             } else {
-                for (uint32_t i = lane; i < nh; i += kWave) {
+                for (uint32_t i = hbeg + lane; i < h1; i += kWave) {
                     SigmaHand<KIND> sh;
                     sh.init(arena, nh, i, na);
                     const float pv = pr[i];
@@ -802,6 +818,27 @@ __device__ __forceinline__ void down_action_node(const DevTree& T, const DevStat
         });
     }
     const bool fast = r.lock_off < 0 && na <= kMaxUnroll;  // mmax filled in the loops above
+    if constexpr (PART) {
+        // na <= kMaxUnroll (the launch checks it); locked nodes read back
+        // their own hands' writes
+        uint64_t anyw = 0;
+        for (int a = 0; a < na; ++a) {
+            if (__ballot((anym >> a) & 1ull) != 0ull) anyw |= 1ull << a;
+            float mx = 0.0f;
+            if (fast) {
+#pragma unroll
+                for (int a2 = 0; a2 < kMaxUnroll; ++a2) mx = a2 == a ? mmax[a2] : mx;
+            } else {
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                const RowIn<H16> row(S.reach[q], (size_t)kid_own[a] * nh, enc.sc);
+                for (uint32_t i = hbeg + lane; i < h1; i += kWave) mx = fmaxf(mx, row[i]);
+            }
+            mx = wave_max_f32(mx);
+            if (lane == 0) part->mx[a] = mx;
+        }
+        if (lane == 0) part->any = anyw;
+        return;
+    }
     for (int a = 0; a < na; ++a) {
         const bool any = __ballot((anym >> a) & 1ull) != 0ull;
         float mx = 0.0f;
@@ -812,7 +849,7 @@ __device__ __forceinline__ void down_action_node(const DevTree& T, const DevStat
             // this wavefront's own writes above: read back in order
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             const RowIn<H16> row(S.reach[q], (size_t)kid_own[a] * nh, enc.sc);
-            for (uint32_t i = lane; i < nh; i += kWave) mx = fmaxf(mx, row[i]);
+            for (uint32_t i = hbeg + lane; i < h1; i += kWave) mx = fmaxf(mx, row[i]);
         }
         mx = wave_max_f32(mx);
         if (lane == 0) {
@@ -896,6 +933,43 @@ __global__ __launch_bounds__(kBlock) void k_down_level(DevTree T, DevState S, co
     const uint32_t lane = threadIdx.x & 63;
     if (__builtin_amdgcn_readfirstlane(k1 > k0 ? 1u : 0u)) down_action_node<KIND, H16>(T, S, acts[k0], q, lane);
     else down_edge<H16>(T, S, edges[m - k0], mask, lane);
+}
+
+// A level of few action nodes, f32 reach rows, at most kMaxUnroll actions:
+// one workgroup per node, each wavefront a quarter of the hands
+// (64-aligned), so the level fills more of the GPU; the children's alive
+// flags and reach maxima are combined in LDS. Every hand's arithmetic is
+// down_action_node's. The level's chance edges follow, a wavefront each
+// (workgroups na.., four edges each).
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_down_node(DevTree T, DevState S, const ActRec* __restrict__ acts,
+                                                      uint32_t na, int q, const EdgeRec* __restrict__ edges,
+                                                      uint32_t ne, int mask) {
+    __shared__ DownPart part[kWaves];
+    const uint32_t n = xcd_slice(blockIdx.x, gridDim.x);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (n >= na) {  // the whole workgroup: edges
+        const uint32_t e = (n - na) * kWaves + wave;
+        if (e < ne) down_edge<false>(T, S, edges[e], mask, lane);
+        return;
+    }
+    const uint32_t ch = ((T.nh[q] + kWaves - 1) / kWaves + kWave - 1) & ~(uint32_t)(kWave - 1);
+    const ActRec r = acts[n];
+    down_action_node<KIND, false, true>(T, S, r, q, lane, wave * ch, (wave + 1) * ch, &part[wave]);
+    __syncthreads();
+    if (wave == 0 && lane < (uint32_t)r.na) {
+        const uint32_t a = lane;
+        const uint32_t kid = T.kids[r.kid + r.na + a];
+        bool any = false;
+        float mx = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            any = any || ((part[w].any >> a) & 1ull);
+            mx = fmaxf(mx, part[w].mx[a]);
+        }
+        S.alive[q][kid] = any ? 1 : 0;
+        S.rmax[q][kid] = mx;
+    }
 }
 
 // Exactness test for the f64 sums of the fold and showdown terminals. The
