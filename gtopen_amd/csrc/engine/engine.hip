@@ -123,8 +123,12 @@ uint32_t wave_grid(uint32_t count, uint32_t waves = kWaves) {
     return (blocks + 7u) & ~7u;
 }
 
-// wavefronts of the fold kernel for n nodes (kFoldNodes per wavefront)
-uint32_t fold_waves(uint32_t n) { return (n + kFoldNodes - 1) / kFoldNodes; }
+// Fold nodes per wavefront for a launch over n nodes that runs alone (a
+// level's folds): kFoldNodes (the card matrix's zeroing and the hand
+// tables amortised over 16 nodes) once that still gives about 8192
+// wavefronts, fewer on small trees.
+uint32_t fold_per(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(kFoldNodes, (n + 8191) / 8192)); }
+uint32_t fold_waves(uint32_t n) { return (n + fold_per(n) - 1) / fold_per(n); }
 
 // Discounts::for_iteration (cfr.rs:44-63), computed in f64 and stored as f32.
 void discounts(int algo, uint32_t it, float* out3) {
@@ -1538,7 +1542,7 @@ class Engine {
         try {
             if (fold)
                 launch(PK_UP_FOLD, k_up_fold<0>, wave_grid(fold_waves(n_fold_up_)), fold_lds_[p], T_, S_,
-                       (const FoldRec*)fold_up_, n_fold_up_, p, 0);
+                       (const FoldRec*)fold_up_, n_fold_up_, p, 0, fold_per(n_fold_up_));
         } catch (...) {
             err = std::current_exception();
         }
@@ -1592,7 +1596,7 @@ class Engine {
         if (rows) return show_kernel_of<kRows16Reach | kRows16Value>();
         return seq ? show_kernel_of<0, true>() : show_kernel_of<0>();
     }
-    using FoldFn = void (*)(DevTree, DevState, const FoldRec*, uint32_t, int, int);
+    using FoldFn = void (*)(DevTree, DevState, const FoldRec*, uint32_t, int, int, uint32_t);
     static FoldFn fold_kernel(int rows, bool seq = false) {
         if (rows == kRows16Reach) return k_up_fold<kRows16Reach>;
         if (rows) return k_up_fold<kRows16Reach | kRows16Value>;
@@ -1633,8 +1637,11 @@ class Engine {
             std::exception_ptr err;
             try {
                 if (fold)
-                    launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(n_fold_all_)), fold_lds_[p], T_, S_,
-                           (const FoldRec*)fold_all_, n_fold_all_, p, swap);
+                    // beside the showdown on the second stream: full batches
+                    // (fewer waves measured faster here even on small trees:
+                    // turn spot 0.853 against 0.891 ms per iteration)
+                    launch(PK_UP_FOLD, fold_fn, wave_grid((n_fold_all_ + kFoldNodes - 1) / kFoldNodes), fold_lds_[p],
+                           T_, S_, (const FoldRec*)fold_all_, n_fold_all_, p, swap, kFoldNodes);
             } catch (...) {
                 err = std::current_exception();
             }
@@ -1660,7 +1667,7 @@ class Engine {
                 // this level's terminals, just before the kernels that read them
                 if (D.n_fold)
                     launch(PK_UP_FOLD, fold_fn, wave_grid(fold_waves(D.n_fold)), fold_lds_[p], T_, S_, (const FoldRec*)D.fold,
-                           D.n_fold, p, swap);
+                           D.n_fold, p, swap, fold_per(D.n_fold));
                 if (!mma && D.n_show_chunks[cap_])
                     launch(PK_UP_SHOW, show_fn, wave_grid(D.n_show_chunks[cap_], 1), show_lds_[p], T_, S_,
                            (const ShowRec*)D.show, (const uint4*)D.show_chunks[cap_], D.n_show_chunks[cap_], p, swap);

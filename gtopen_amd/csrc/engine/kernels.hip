@@ -1004,7 +1004,8 @@ __device__ __forceinline__ double add_lanes_in_order(double t, float x, uint32_t
 // Up sweep, fold terminals (game.rs:470-496), all levels in one launch:
 // value = amount * (opponent reach compatible with the hand), with f64
 // totals (a wavefront sum; under SEQ the CPU's sum in hand order for a row
-// that fails the exactness test above). One wavefront takes kFoldNodes
+// that fails the exactness test above). One wavefront takes `per` (at most
+// kFoldNodes; fewer on small trees, so they still fill the GPU)
 // consecutive nodes. The opponent's row is scattered into a symmetric
 // 52 x 52 card matrix in LDS, M[a][b] = M[b][a] = reach of the hand {a, b}
 // (0 where the opponent has no such hand, and on the diagonal; those zeros
@@ -1018,7 +1019,7 @@ __device__ __forceinline__ double add_lanes_in_order(double t, float x, uint32_t
 // ---------------------------------------------------------------------------
 constexpr int kMaxH = 21;         // hand rows per lane: 21 * 64 >= 1326
 constexpr int kRowPer = (1326 + kBlock - 1) / kBlock;  // hand rows per thread of a workgroup
-constexpr uint32_t kFoldNodes = 16;                     // fold nodes per wavefront
+constexpr uint32_t kFoldNodes = 16;                     // fold nodes per wavefront, at most
 constexpr size_t kFoldWaveBytes = 52 * 52 * 4 + 52 * 8;  // card matrix + per-card sums
 
 // combo_index (cards.rs:100-103) of two distinct cards, hi > lo
@@ -1039,7 +1040,7 @@ __device__ __forceinline__ uint32_t combo_of(uint32_t hi, uint32_t lo) { return 
 
 template <int ROWS = 0, bool SEQ = false>
 __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const FoldRec* __restrict__ recs,
-                                                    uint32_t count, int p, int swap) {
+                                                    uint32_t count, int p, int swap, uint32_t per) {
     constexpr bool R16 = (ROWS & kRows16Reach) != 0, V16 = (ROWS & kRows16Value) != 0;
     static_assert(!(SEQ && R16), "sequential totals are f64 sums of f32 reach rows");
     // totals and card sums in f64 (the CPU's, game.rs:477-484), f32 over
@@ -1048,9 +1049,9 @@ __global__ __launch_bounds__(kBlock) void k_up_fold(DevTree T, DevState S, const
     extern __shared__ double lds[];
     const int o = 1 - p;
     const uint32_t nho = T.nh[o], nhp = T.nh[p];
-    const uint32_t first = wave_item() * kFoldNodes;
+    const uint32_t first = wave_item() * per;
     if (first >= count) return;
-    const uint32_t last = min(first + kFoldNodes, count);
+    const uint32_t last = min(first + per, count);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     char* base = (char*)lds + wave * kFoldWaveBytes;
