@@ -104,6 +104,28 @@ def test_algorithms_rake_and_reference_spot_bit_identical(built):
         gpu.close()
 
 
+def test_locked_solve_bit_identical(built):
+    """A node lock (query.rs:702-776, the root raked to a target mix) on the
+    turn spot: 20 iterations, the lock on both engines, 30 more, identical
+    to the oracle."""
+    import oracle
+    from gtopen_amd import GpuSolver, Spot, workloads
+
+    _say("case: turn spot, root locked to a mix after 20 iterations, sequential sums, 50 iterations")
+    spot = Spot(workloads.turn_spot())
+    gpu, cpu = GpuSolver(spot, showdown="sequential"), oracle.Oracle(spot)
+    gpu.iterate(20)
+    cpu.iterate(20)
+    na = int(spot.num_children[0])
+    freqs = [float(k + 1) for k in range(na)]
+    gpu.lock_node([], {"kind": "range", "freqs": freqs}, "root mix")
+    cpu.set_locks(gpu.locks)
+    gpu.iterate(30)
+    cpu.iterate(30)
+    _identical(gpu, cpu, "iter 50", values=True)
+    gpu.close()
+
+
 def test_sequential_needs_f32_rows(built):
     """The sequential sums are f64 sums of f32 reach rows: refused with f16
     rows; switching back to the default association is allowed."""
