@@ -995,7 +995,6 @@ __device__ __forceinline__ bool exact_in_any_order(double total, int lsb_min) {
 // (the same in every lane)
 __device__ __forceinline__ double add_lanes_in_order(double t, float x, uint32_t m) {
     const int xi = __builtin_bit_cast(int, x);
-#pragma unroll 8
     for (uint32_t l = 0; l < m; ++l) t += (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(xi, (int)l));
     return t;
 }
