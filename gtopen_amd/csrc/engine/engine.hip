@@ -1694,12 +1694,12 @@ class Engine {
     // Wavefronts per node of an update launch over n nodes: a level with
     // few nodes (the upper levels; every level of a small tree) splits each
     // node's traverser hands over up to kMaxSlices wavefronts of at least
-    // 256 hands, for about kFillWaves wavefronts in all (f32 arenas and
+    // 128 hands, for about kFillWaves wavefronts in all (f32 arenas and
     // rows only: the other forms reduce over whole rows).
     static constexpr uint32_t kFillWaves = 8192, kMaxSlices = 8;
     uint32_t up_slices(uint32_t n, int rows) const {
         if (rows || compressed_ || half_ || n >= kFillWaves) return 1;
-        const uint32_t by_hands = std::max<uint32_t>(1, std::max(plan_.nh[0], plan_.nh[1]) / 256);
+        const uint32_t by_hands = std::max<uint32_t>(1, std::max(plan_.nh[0], plan_.nh[1]) / 128);
         return std::max<uint32_t>(1, std::min({kFillWaves / n, by_hands, kMaxSlices}));
     }
     // the up-sweep action kernel for a sweep mode (0 DCFR update, 1 best
