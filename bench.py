@@ -145,12 +145,15 @@ def time_to_target(spot, storage, target, cap, showdown="exact", sched=None, alg
     return dt, its, e, e64
 
 
-def gpu_rate(spot, steps, warmup, storage="f32", showdown="exact"):
+def gpu_rate(spot, steps, warmup, storage="f32", showdown="exact", autotune=0):
     """(ms per iteration, iterations/s, arena entries D) of graph-replayed
-    iterations."""
+    iterations; autotune > 0: the autotuner's schedule first (that many
+    iterations per candidate, before the warm-up)."""
     from gtopen_amd import GpuSolver
 
     g = GpuSolver(spot, storage=storage, showdown=showdown)
+    if autotune > 0:
+        g.autotune(autotune)
     g.iterate(warmup)
     g.synchronize()
     t0 = time.perf_counter()
@@ -450,7 +453,7 @@ def other_configs(args):
     out = {}
     for name, steps in (("river", 500), ("turn", 200)):
         spot = Spot(workloads.CONFIGS[name]())
-        ms, ips, D = gpu_rate(spot, steps, 10)
+        ms, ips, D = gpu_rate(spot, steps, 10, autotune=args.autotune)
         gbs = 20.0 * D / (ms * 1e-3) / 1e9
         out[name] = {"iters_per_s": round(ips, 2), "ms_per_step": round(ms, 4), "arena_entries_D": D,
                      "nodes": spot.num_nodes, "hands": list(spot.num_hands),
